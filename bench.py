@@ -1,0 +1,167 @@
+"""Throughput of the per-timestep advance on one MI355X.
+
+Workload (default): a population of independent random-tree cells, 200-5000 compartments each
+(uniform), drawn until 10M compartments; leak + HH Na/K as two-state kinetic schemes on every
+segment, per-cell gNa/gK scaled logU(0.5, 2); Poisson 5 Hz current pulses (0.5 nA x 1 ms) at every
+root; propagator pruned at 1e-9; FP32 state, FP64 propagator construction; dt = 0.1 ms.
+
+A step is one staggered time step of every compartment.  W warm-up steps run first, then exactly K
+timed steps, bracketed by torch.cuda.synchronize().  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--compartments", type=float, default=1e7)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--launch-steps", type=int, default=0,
+                    help="steps per kernel launch (0: all timed steps in one launch)")
+    ap.add_argument("--block-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample-cells", type=int, default=12)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pop, cells, args):
+    """FP64 oracle step (scipy CSR SpMV + numpy, one thread) on a bounded sample of the same
+    cells, fed the same pruned FP64 propagator the device built."""
+    import threadpoolctl
+    from oracle import neuwon_oracle as O
+    from neuwon_amd import population
+    sample = list(range(min(args.cpu_sample_cells, pop.num_cells)))
+    sims = []
+    n_total = 0
+    tables = {}
+    for g in "mhn":
+        deriv = O.two_state_derivative(O.hh_gate_rate(g, pop.celsius))
+        tables[g] = O.node_table_to_polynomial(O.lti_node_table(deriv, 2, pop.dt))
+    for c in sample:
+        r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
+        P = population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1], pop.C[r0:r1],
+                                           0.5 * pop.dt, pop.propagator.prune)
+        gbar = {k: pop.g_bar[k][r0:r1] for k in ("na", "k", "l")}
+        sim = O.HHSim(P, pop.C[r0:r1], gbar, v0=-70.0, dt=pop.dt, celsius=pop.celsius,
+                      gate_mode="table", tables=tables)
+        sims.append(sim)
+        n_total += r1 - r0
+    with threadpoolctl.threadpool_limits(1):
+        for s in sims:
+            s.advance()  # warm-up
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            for s in sims:
+                s.advance()
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_seconds or steps >= 1000:
+                break
+    rate = n_total * steps / el
+    return {"value": rate, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{len(sample)} cells / {n_total} compartments of the same population, "
+                      f"{steps} steps, FP64 scipy CSR SpMV + numpy (oracle), 1 thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (cells are "
+              "independent, a multi-GPU run would shard whole cells with no per-step exchange)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from neuwon_amd import population, synth
+    t0 = time.time()
+    cells = synth.population(total=int(args.compartments), seed=args.seed)
+    log(f"generated {len(cells)} cells in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    pop = population.HHPopulation(cells, dt=0.1, celsius=6.3, prune=1e-9, gate_mode="table",
+                                  stim={"mode": "poisson", "amp_na": 0.5, "rate_hz": 5.0,
+                                        "duration_ms": 1.0, "seed": args.seed},
+                                  block_threads=args.block_threads)
+    st = pop.propagator.stats
+    nnz_bar = pop.propagator.mean_nnz_per_row
+    log(f"population ready in {time.time() - t0:.1f}s: {pop.num_segments} compartments, "
+        f"nnz/row {nnz_bar:.1f}, propagator build {st['build_seconds']:.1f}s, "
+        f"{st['device_bytes'] / 1e9:.2f} GB on device")
+    N = pop.num_segments
+    K, W = args.steps, args.warmup
+    L = args.launch_steps or K
+    if W:
+        pop.advance(W)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = []
+    t_start = time.perf_counter()
+    done = 0
+    while done < K:
+        k = min(L, K - done)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        pop.advance(k)
+        e1.record(stream)
+        ev.append((k, e0, e1))
+        done += k
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kern_ms = sum(a.elapsed_time(b) for _, a, b in ev)
+    launches = len(ev)
+    value = N * K / wall
+    # algorithmic bytes per compartment-step: 2 half steps x (8 nnz + 28) + 68 mechanism bytes
+    b_step = 16.0 * nnz_bar + 124.0
+    avg_launch_s = kern_ms / 1e3 / launches
+    bytes_per_launch = b_step * N * (K / launches)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    stored_bytes_step = (st["nnz"] * 6.0 * 2) / N  # FP32 value + u16 column, two half steps
+    out = {
+        "metric": "compartment-steps/s @10M cmpts: FP32 batched vs FP64 plain; adjoint grads/s",
+        "value": value,
+        "unit": "compartment-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-tree cells, Poisson stimulus)",
+        "config": {"workload": "batched HH population, FP32 state, FP64-built propagator pruned 1e-9",
+                   "compartments": N, "cells": pop.num_cells, "dt_ms": 0.1,
+                   "nnz_per_row": round(nnz_bar, 2), "steps_per_launch": L,
+                   "propagator_build_s": round(st["build_seconds"], 2)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "nw::k_hh_step", "avg_launch_ms": avg_launch_s * 1e3,
+                     "algorithmic_bytes_per_cmpt_step": b_step,
+                     "stored_propagator_bytes_per_cmpt_step": stored_bytes_step},
+    }
+    log(f"timed {K} steps: {wall:.3f}s wall, kernel {kern_ms:.1f} ms, {value / 1e9:.3f} G cmpt-steps/s")
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pop, cells, args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

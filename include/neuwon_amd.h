@@ -1,0 +1,167 @@
+/*
+ * neuwon_amd -- C ABI of the MI355X-native per-timestep advance of NEUWON.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative code on failure; the message of the
+ *     last failure on the calling thread is returned by nw_last_error().  Nothing aborts.
+ *   - Device pointers are plain addresses of device memory owned by the caller (the Python host
+ *     allocates them as torch tensors); host pointers are named *_host.
+ *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *   - Per-cell CSR propagators (nw_propagator) are owned by the library.
+ *   - Calls that share device buffers on one stream are ordered by that stream; a handle is not
+ *     thread-safe (the Python host serialises calls with a lock).
+ *
+ * Units follow the reference: mV, ms, Siemens, Farads, Amperes.
+ */
+#ifndef NEUWON_AMD_H
+#define NEUWON_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NW_OK 0
+#define NW_ERR_ARG (-1)
+#define NW_ERR_HIP (-2)
+#define NW_ERR_BLAS (-3)
+#define NW_ERR_ALLOC (-4)
+
+/* Message of the last failure on this thread ("" if none). */
+const char* nw_last_error(void);
+
+/* Library version string, device name / CU count / LDS bytes per workgroup of `device`. */
+const char* nw_version(void);
+int nw_device_info(int device, char* name_out, int name_len, int* num_cus, int* lds_bytes);
+
+/* ------------------------------------------------------------------------------------------
+ * Propagator (replaces Electric._compute_propagator_matrix, neuwon/rxd/neuron/electric.py:79-117)
+ *
+ * Segments are grouped by cell (the reference sorts Segment by 'neuron', neuron.py:12, which makes
+ * the global matrix block diagonal).  cell_ptr_host[c]..cell_ptr_host[c+1] are the segments of cell
+ * c; parent_local_host holds each segment's parent as an index local to its cell, -1 for the root.
+ * R_host (Ohm) and C_host (F) are the per-segment axial resistance and capacitance.
+ * For every cell the library builds A*dt exactly as electric.py:92-108, takes expm in FP64 on the
+ * device, drops entries below `prune` (electric.py:114-115), and keeps per-cell CSR rows with FP32
+ * values and 16-bit local column indices.  The diagonal is stored so that every FP32 row sums, in
+ * FP64, to the FP64 row sum of the pruned matrix.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nw_propagator nw_propagator;
+
+typedef struct nw_propagator_stats {
+    int64_t num_cells;
+    int64_t num_rows;
+    int64_t nnz;             /* stored entries (FP32 value + u16 column each)                     */
+    int32_t max_cell_rows;
+    int32_t max_row_nnz;
+    double  build_seconds;
+    int64_t device_bytes;    /* bytes held on the device by the handle                            */
+} nw_propagator_stats;
+
+int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_host,
+                        const int32_t* parent_local_host, const double* R_host, const double* C_host,
+                        double time_step_ms, double prune, nw_propagator** out);
+int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* out);
+int nw_propagator_destroy(nw_propagator* p);
+
+/* One cell's pruned FP64 propagator, built by the same device path, copied back as CSR.
+ * Two calls: with val_host == NULL it only returns the entry count in *nnz_out. */
+int nw_propagator_cell_f64(int device, int32_t n, const int32_t* parent_local_host,
+                           const double* R_host, const double* C_host, double time_step_ms,
+                           double prune, int64_t* nnz_out, int64_t* row_ptr_host,
+                           int32_t* col_host, double* val_host);
+
+/* Copy one cell's stored (FP32 value, u16 column) CSR back to the host (for checks). */
+int nw_propagator_cell_export(const nw_propagator* p, int64_t cell, uint32_t* row_off_host,
+                              uint16_t* col_host, float* val_host);
+
+/* ------------------------------------------------------------------------------------------
+ * Electric half step (replaces Electric._advance_electric, electric.py:57-76)
+ *   dv = dt*I/C ; alpha = exp(-dt*1e-3*g/C) ; x = d - (d - v)*alpha + dv ; v <- P x
+ * All arrays are FP32 per segment, in the cell-grouped order used to build `p`.
+ * integral_v may be NULL (the reference writes it, electric.py:76, but nothing reads it).
+ * ------------------------------------------------------------------------------------------ */
+int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, const float* sum_g,
+                          const float* drive_v, const float* sum_i, const float* inv_c,
+                          float* integral_v, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Accumulation (replaces _SpeciesType._apply_accumulators / _NonspecificConductance, species.py:
+ * 147-160, 252-260, and rxd_model.py:93-109): segmented reduction over instances sorted by
+ * segment.  seg_inst_ptr[s]..seg_inst_ptr[s+1] are the instances on segment s (CSR offsets, so no
+ * atomics and a deterministic order).  Every instance carries a conductance and a reversal
+ * potential; nonspec_i (may be NULL) is the per-segment nonspecific current.
+ *   sum_g[s] = sum g ; drive_v[s] = sum(g E) / sum_g (0 where sum_g == 0) ; sum_i[s] = nonspec_i
+ * ------------------------------------------------------------------------------------------ */
+int nw_accumulate(int64_t num_segments, const int64_t* seg_inst_ptr, const float* g_inst,
+                  const float* e_inst, const float* nonspec_i, float* sum_g, float* drive_v,
+                  float* sum_i, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * LTI kinetic advance (replaces the kernel generated by lti_sim/codegen.py:39-106)
+ *   node_table: [num_nodes][S][S] FP32 propagators (row-major: entry [node][row][col]) on a linear
+ *   voltage grid vmin..vmax with num_nodes-1 buckets; linear interpolation between nodes.
+ *   states: S arrays of M FP32 (SoA, array s at states + s*state_stride).
+ *   voltage is gathered through seg_index (may be NULL: instance i reads voltage[i]).
+ *   conserve_sum <= 0 disables the CONSERVE renormalisation.
+ *   n_steps > 1 repeats the advance with the inputs held constant (every step identical to one
+ *   call with n_steps = 1).
+ * If cond_out != NULL, cond_out[i] = gmax[i] * sum_s weights[s] * state_s  (the breakpoint of a
+ * kinetic mechanism whose conductance is a weighted sum of states, e.g. ampa13.mod BREAKPOINT).
+ * ------------------------------------------------------------------------------------------ */
+int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const float* node_table,
+                   int64_t M, const float* voltage, const int32_t* seg_index, float* states,
+                   int64_t state_stride, float conserve_sum, int32_t n_steps, const float* weights,
+                   const float* gmax, float* cond_out, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Fused staggered step for cells carrying leak + HH Na (m^3 h) + HH K (n^4), every gate a two
+ * state kinetic scheme advanced by its LTI table (replaces RxD_Model.advance, rxd_model.py:63-123,
+ * for this mechanism set).  One workgroup advances one cell through n_steps steps:
+ *   half step (dt/2) -> mechanisms (dt, voltage at t+dt/2) -> half step (dt/2) -> AP detector.
+ * gate_table: 3 gates (m, h, n) x num_nodes x {a, b}: open' = a(v) + b(v)*open, linear
+ *   interpolation between nodes of the vmin..vmax grid (a = P_OC, b = P_OO - P_OC of the 2x2
+ *   column-normalised expm, lti_model.py:51-65).
+ * Per segment FP32 arrays: v, m, h, n, g_na, g_k, g_l (Siemens = gbar*area), inv_c (1/F).
+ * Stimulus: per cell one segment (stim_local) receives stim_amp_a * k amperes on input tick t, where
+ *   k = 1 for a constant clamp (stim_mode 1) or the number of Poisson pulse onsets within the last
+ *   pulse_ticks ticks (stim_mode 2; onset on tick t iff hash(seed, cell, t) < poisson_threshold).
+ * gate_mode 1 instead advances each gate with the exact solution of hh.mod's DERIVATIVE block
+ *   (x' = inf + (x - inf) exp(-(alpha+beta) dt), nmodl/solver.py:48-55; rates hh.mod:62-94).
+ * conductances_valid = 0 reproduces the reference's first step, whose first half step sees zero
+ *   accumulated conductance.
+ * AP detector (neuron.py:72-76) on segment ap_local of each cell: ap_state (u8 per cell) holds
+ *   the over-threshold flag, ap_count (u32 per cell) counts detections.
+ * trace (may be NULL): trace[cell * trace_stride + step] = v of segment ap_local after the step.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nw_hh_params {
+    float   dt_ms;
+    float   vmin, vmax;
+    int32_t num_nodes;
+    float   e_na, e_k, e_l;
+    int32_t stim_mode;        /* 0 none, 1 constant, 2 Poisson pulses                              */
+    float   stim_amp_a;
+    uint32_t stim_seed;
+    uint32_t poisson_threshold;
+    int32_t pulse_ticks;
+    float   ap_threshold;
+    int32_t conductances_valid;
+    int64_t step0;            /* global step index of the first step (input ticks continue)      */
+    int32_t block_threads;    /* 0 = library default                                              */
+    int32_t gate_mode;        /* 0: two-state LTI tables; 1: exact exponential update of hh.mod    */
+    float   q10;              /* gate_mode 1: rate factor 3^((celsius-6.3)/10), hh.mod:67         */
+    int32_t pad_;
+} nw_hh_params;
+
+int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
+               const float* gate_table, float* v, float* m, float* h, float* n, const float* g_na,
+               const float* g_k, const float* g_l, const float* inv_c, const int32_t* stim_local,
+               const int32_t* ap_local, uint8_t* ap_state, uint32_t* ap_count, float* trace,
+               int64_t trace_stride, void* stream);
+
+/* Stimulus generator, the same hash the step kernel evaluates (exposed for checks). */
+uint32_t nw_stim_hash(uint32_t seed, uint32_t cell, uint32_t tick);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

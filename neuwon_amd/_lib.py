@@ -1,0 +1,92 @@
+"""ctypes binding of ``libneuwon_amd.so`` (declared in ``include/neuwon_amd.h``).
+
+The library is loaded from the package directory only; a missing library raises ImportError-like
+``RuntimeError`` at first use.  There is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libneuwon_amd.so")
+
+c_i32, c_i64, c_u32, c_f32, c_f64, c_vp = (ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32,
+                                           ctypes.c_float, ctypes.c_double, ctypes.c_void_p)
+
+
+class NwPropagatorStats(ctypes.Structure):
+    _fields_ = [("num_cells", c_i64), ("num_rows", c_i64), ("nnz", c_i64),
+                ("max_cell_rows", c_i32), ("max_row_nnz", c_i32), ("build_seconds", c_f64),
+                ("device_bytes", c_i64)]
+
+
+class NwHHParams(ctypes.Structure):
+    _fields_ = [("dt_ms", c_f32), ("vmin", c_f32), ("vmax", c_f32), ("num_nodes", c_i32),
+                ("e_na", c_f32), ("e_k", c_f32), ("e_l", c_f32), ("stim_mode", c_i32),
+                ("stim_amp_a", c_f32), ("stim_seed", c_u32), ("poisson_threshold", c_u32),
+                ("pulse_ticks", c_i32), ("ap_threshold", c_f32), ("conductances_valid", c_i32),
+                ("step0", c_i64), ("block_threads", c_i32), ("gate_mode", c_i32), ("q10", c_f32),
+                ("pad_", c_i32)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "nw_last_error": (ctypes.c_char_p, []),
+    "nw_version": (ctypes.c_char_p, []),
+    "nw_device_info": (c_i32, [c_i32, ctypes.c_char_p, c_i32, ctypes.POINTER(c_i32),
+                               ctypes.POINTER(c_i32)]),
+    "nw_propagator_build": (c_i32, [c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_f64, c_f64,
+                                    ctypes.POINTER(c_vp)]),
+    "nw_propagator_stats_get": (c_i32, [c_vp, ctypes.POINTER(NwPropagatorStats)]),
+    "nw_propagator_destroy": (c_i32, [c_vp]),
+    "nw_propagator_cell_f64": (c_i32, [c_i32, c_i32, c_vp, c_vp, c_vp, c_f64, c_f64,
+                                       ctypes.POINTER(c_i64), c_vp, c_vp, c_vp]),
+    "nw_propagator_cell_export": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "nw_electric_half_step": (c_i32, [c_vp, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "nw_accumulate": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "nw_lti_advance": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
+                               c_f32, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "nw_hh_step": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
+                           c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nw_stim_hash": (c_u32, [c_u32, c_u32, c_u32]),
+}
+
+_lib = None
+_lock = threading.RLock()
+
+
+def load():
+    """Load the library (once).  Raises RuntimeError when it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m neuwon_amd.build`")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+        return _lib
+
+
+def lock():
+    """The library lock: calls that touch shared device state are serialised with it."""
+    return _lock
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().nw_last_error().decode(errors="replace")
+        raise RuntimeError(f"neuwon_amd error {rc}: {msg}")
+    return rc
+
+
+def ptr(t):
+    """Device (or host) address of a torch tensor / numpy array as a ctypes void pointer."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(t.ctypes.data)

@@ -1,0 +1,222 @@
+"""Kinetic channel models and their FP64 propagator tables.
+
+* Hodgkin-Huxley gates (nmodl_library/artificial/hh.mod) written as two-state kinetic schemes
+  ``~ C <-> O (alpha(v), beta(v))``; the exact propagator of a two-state scheme is closed form.
+* Generic linear-time-invariant schemes: ``LTIScheme`` builds the propagator at every node of a
+  voltage grid with scipy.linalg.expm, as LTI_Model.make_matrix does
+  (neuwon/rxd/lti_sim/lti_model.py:51-65: column j of A is the derivative at unit state e_j,
+  P = expm(A dt), every column divided by its sum).
+* ``LinearInput`` mirrors lti_sim/inputs.py:72-84 (bucket mapping of a linear input).
+
+Tables are built in FP64 and stored on the device in FP32.  Between grid nodes the propagator is
+linearly interpolated: this is the reference's per-bucket polynomial table with terms (1, frac)
+(lti_sim/codegen.py:29-37, 68-94) whose coefficients are (T[b], T[b+1] - T[b]).
+"""
+import math
+
+import numpy as np
+import scipy.linalg
+
+HH_REVERSAL = {"na": 60.0, "k": -88.0, "l": -54.3}        # hh.mod:32-34 (mV)
+HH_GBAR = {"na": 0.12, "k": 0.036, "l": 0.0003}            # hh.mod:29-31 (S/cm^2)
+CM2_TO_UM2 = (1e-6 * 1e-6) / (1e-2 * 1e-2)                 # nmodl/__init__.py:106
+
+
+def q10_factor(celsius):
+    """hh.mod:67."""
+    return 3.0 ** ((float(celsius) - 6.3) / 10.0)
+
+
+def _vtrap(x, y):
+    """hh.mod:88-94."""
+    x = np.asarray(x, dtype=np.float64)
+    r = x / y
+    out = np.empty_like(r)
+    small = np.abs(r) < 1e-6
+    out[small] = y * (1.0 - r[small] / 2.0)
+    out[~small] = x[~small] / np.expm1(r[~small])
+    return out
+
+
+def hh_rates(v, celsius=6.3):
+    """{gate: (alpha, beta)} in 1/ms including q10; hh.mod:62-86."""
+    v = np.atleast_1d(np.asarray(v, dtype=np.float64))
+    q = q10_factor(celsius)
+    return {
+        "m": (q * 0.1 * _vtrap(-(v + 40.0), 10.0), q * 4.0 * np.exp(-(v + 65.0) / 18.0)),
+        "h": (q * 0.07 * np.exp(-(v + 65.0) / 20.0), q * 1.0 / (np.exp(-(v + 35.0) / 10.0) + 1.0)),
+        "n": (q * 0.01 * _vtrap(-(v + 55.0), 10.0), q * 0.125 * np.exp(-(v + 65.0) / 80.0)),
+    }
+
+
+def hh_steady_state(v, celsius=6.3):
+    """Open fraction of every gate at rest (INITIAL block, hh.mod:48-53)."""
+    r = hh_rates(v, celsius)
+    return {g: float((a / (a + b))[0]) for g, (a, b) in r.items()}
+
+
+class LinearInput:
+    """Bucket mapping of a linear input; lti_sim/inputs.py:36-48, 72-84."""
+
+    def __init__(self, minimum, maximum, num_buckets):
+        self.minimum = float(minimum)
+        self.maximum = float(maximum)
+        assert self.minimum < self.maximum
+        self.num_buckets = int(num_buckets)
+        self.range = self.maximum - self.minimum
+        self.bucket_frq = self.num_buckets / self.range
+        self.bucket_width = self.range / self.num_buckets
+
+    def get_bucket_value(self, value):
+        return (value - self.minimum) * self.bucket_frq
+
+    def get_input_value(self, bucket_value):
+        return self.minimum + bucket_value * self.bucket_width
+
+    def get_bucket_location(self, value):
+        location = self.get_bucket_value(value)
+        b = int(location)
+        if b < 0:
+            return 0, 0.0
+        if b >= self.num_buckets:
+            return self.num_buckets - 1, 1.0
+        return b, location - b
+
+    def nodes(self):
+        return self.get_input_value(np.arange(self.num_buckets + 1, dtype=np.float64))
+
+
+def two_state_propagator(alpha, beta, dt):
+    """Exact expm([[-a, b], [a, -b]] dt), state order (C, O), vectorised over a and b.
+    Returns (P_OC, P_OO): open' = P_OC * closed + P_OO * open."""
+    s = alpha + beta
+    e = np.exp(-s * dt)
+    oinf = alpha / s
+    return oinf * (1.0 - e), oinf + (1.0 - oinf) * e
+
+
+def hh_gate_table(dt, celsius=6.3, grid=None):
+    """FP64 table [3 gates (m, h, n)][node][2]: (a, b) with open' = a + b * open, where
+    a = P_OC and b = P_OO - P_OC (the CONSERVE sum of a two-state scheme is 1)."""
+    grid = grid or LinearInput(-100.0, 100.0, 2000)
+    v = grid.nodes()
+    rates = hh_rates(v, celsius)
+    out = np.empty((3, len(v), 2))
+    for gi, g in enumerate("mhn"):
+        poc, poo = two_state_propagator(*rates[g], dt)
+        out[gi, :, 0] = poc
+        out[gi, :, 1] = poo - poc
+    return out
+
+
+class LTIScheme:
+    """A kinetic scheme that is linear in its states with a single voltage input.
+
+    ``derivative(v, *states) -> list of d(state)/dt`` (1/ms); ``conserve_sum`` as the CONSERVE
+    statement (None when absent).  Mirrors LTI_Model (lti_sim/lti_model.py:5-78)."""
+
+    def __init__(self, name, state_names, derivative, conserve_sum=None, time_step=0.1,
+                 grid=None, weights=None):
+        self.name = str(name)
+        self.state_names = list(state_names)
+        self.num_states = len(self.state_names)
+        self.derivative = derivative
+        self.conserve_sum = None if conserve_sum is None else float(conserve_sum)
+        self.time_step = float(time_step)
+        self.grid = grid or LinearInput(-100.0, 100.0, 2000)
+        self.weights = None if weights is None else np.asarray(weights, dtype=np.float64)
+        self._check_is_lti()
+
+    def _check_is_lti(self):
+        """lti_model.py:40-49."""
+        rng = np.random.default_rng(12345)
+        for _ in range(3):
+            v = rng.uniform(self.grid.minimum, self.grid.maximum)
+            s1 = rng.uniform(0.0, 1.0, self.num_states)
+            d1 = np.asarray(self.derivative(v, *s1), dtype=np.float64)
+            d2 = np.asarray(self.derivative(v, *(2.0 * s1)), dtype=np.float64)
+            if not (np.all(np.isfinite(d1)) and np.all(np.abs(d1 - d2 / 2.0) < 1e-12)):
+                raise ValueError(f"{self.name}: non-linear system detected")
+
+    def rate_matrix(self, v):
+        S = self.num_states
+        A = np.empty((S, S))
+        for col in range(S):
+            unit = [float(x == col) for x in range(S)]
+            A[:, col] = self.derivative(v, *unit)
+        return A
+
+    def make_matrix(self, v, time_step=None):
+        """lti_model.py:51-65."""
+        dt = self.time_step if time_step is None else float(time_step)
+        P = scipy.linalg.expm(self.rate_matrix(v) * dt)
+        P /= P.sum(axis=0, keepdims=True)
+        return P
+
+    def node_table(self):
+        """FP64 [node][row][col] propagators on the grid nodes."""
+        return np.stack([self.make_matrix(v) for v in self.grid.nodes()])
+
+    def initial_state(self, v):
+        """Steady state at voltage v: expm(A * 1 hour) applied to the uniform state
+        (lti_model.py:67-78, evaluated at the given voltage)."""
+        if self.conserve_sum is None:
+            return np.zeros(self.num_states)
+        P = self.make_matrix(v, time_step=3600e3)
+        return P.dot(np.full(self.num_states, self.conserve_sum / self.num_states))
+
+
+def voltage_markov_scheme(edges, a, b, time_step=0.1, name="markov", conserve_sum=1.0,
+                          num_states=None, weights=None):
+    """Scheme with transitions i -> j at rate a_ij exp(b_ij v) (1/ms, v in mV).
+
+    edges: list of (i, j) directed transitions; a, b: arrays of the same length."""
+    edges = [(int(i), int(j)) for i, j in edges]
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    S = num_states or (max(max(e) for e in edges) + 1)
+
+    def derivative(v, *states):
+        d = [0.0] * S
+        for (i, j), ai, bi in zip(edges, a, b):
+            flux = ai * math.exp(bi * v) * states[i]
+            d[i] -= flux
+            d[j] += flux
+        return d
+
+    scheme = LTIScheme(name, [f"s{k}" for k in range(S)], derivative, conserve_sum, time_step,
+                       weights=weights)
+    scheme.edges, scheme.a, scheme.b = edges, a, b
+
+    def rate_matrix(v):
+        A = np.zeros((S, S))
+        for (i, j), ai, bi in zip(edges, a, b):
+            k = ai * math.exp(bi * v)
+            A[i, i] -= k
+            A[j, i] += k
+        return A
+
+    scheme.rate_matrix = rate_matrix
+    return scheme
+
+
+# AMPA13 topology (lti_sim/tests/ampa13.mod, KINETIC kstates): states
+# C0..C4 (0..4), D1..D4 (5..8), O1..O4 (9..12); 12 reversible transitions.
+AMPA13_STATES = ["C0", "C1", "C2", "C3", "C4", "D1", "D2", "D3", "D4", "O1", "O2", "O3", "O4"]
+AMPA13_REACTIONS = [(0, 1), (1, 2), (2, 3), (3, 4),      # C0<->C1 ... C3<->C4
+                    (1, 5), (2, 6), (3, 7), (4, 8),      # Ck<->Dk
+                    (1, 9), (2, 10), (3, 11), (4, 12)]   # Ck<->Ok
+AMPA13_WEIGHTS = [0.0] * 9 + [0.25, 0.5, 0.75, 1.0]      # g = gmax (O4 + .75 O3 + .5 O2 + .25 O1)
+
+
+def random_ampa13_scheme(seed, time_step=0.1):
+    """13-state voltage-dependent scheme on the AMPA13 graph: every transition (both directions)
+    has rate a exp(b v), a ~ logU(1e-2, 1) /ms, b ~ U(-0.05, 0.05) /mV."""
+    rng = np.random.default_rng(seed)
+    edges = []
+    for i, j in AMPA13_REACTIONS:
+        edges += [(i, j), (j, i)]
+    a = np.exp(rng.uniform(np.log(1e-2), np.log(1.0), len(edges)))
+    b = rng.uniform(-0.05, 0.05, len(edges))
+    return voltage_markov_scheme(edges, a, b, time_step=time_step, name="markov13",
+                                 conserve_sum=1.0, num_states=13, weights=AMPA13_WEIGHTS)

@@ -1,0 +1,382 @@
+// Fused staggered step for HH cells, the LTI kinetic kernel and the segmented accumulation.
+//
+// Fused step (replaces RxD_Model.advance, neuwon/rxd/rxd_model.py:63-123, for cells whose every
+// segment carries leak + HH Na + HH K written as two-state kinetic schemes).  Cells are
+// independent (the propagator is block diagonal), so one workgroup advances one cell through any
+// number of steps without a grid-wide barrier.  Per step:
+//   A  x = relax(v, g, d, I)                 thread per segment, x -> LDS      (electric.py:71-74)
+//   B  v = P x                               wavefront per CSR row, v -> LDS   (electric.py:75)
+//   C  gates <- LTI table(v); g, d; x        thread per segment                (codegen.py:39-106,
+//                                                                               species.py:155-160)
+//   D  v = P x                               wavefront per CSR row
+//   AP detector and optional trace of one segment per cell                     (neuron.py:72-76)
+// The voltage of the cell stays in LDS across steps; gates and per-segment constants stream from
+// HBM (they are a few percent of the propagator's bytes).
+#include <algorithm>
+
+#include "nw_internal.h"
+
+namespace nw {
+
+struct HHArgs {
+    const int64_t* cell_row0;
+    const int64_t* cell_nz0;
+    const uint32_t* row_off;
+    const float* vals;
+    const uint16_t* cols;
+    const double* rowsum;
+    const int32_t* order;
+    int32_t nmax;
+    int32_t n_steps;
+    nw_hh_params prm;
+    const float* table;  // [3][num_nodes][2]
+    float* v;
+    float* m;
+    float* h;
+    float* n;
+    const float* g_na;
+    const float* g_k;
+    const float* g_l;
+    const float* inv_c;
+    const int32_t* stim_local;
+    const int32_t* ap_local;
+    uint8_t* ap_state;
+    uint32_t* ap_count;
+    float* trace;
+    int64_t trace_stride;
+};
+
+// Stimulus current (A) of the cell's stimulated segment on input tick `tick` (1-based).
+__device__ __forceinline__ float stim_current(const nw_hh_params& p, uint32_t cell, int64_t tick) {
+    if (p.stim_mode == 1) return p.stim_amp_a;
+    if (p.stim_mode == 2) {
+        int k = 0;
+        for (int j = 0; j < p.pulse_ticks; ++j) {
+            int64_t t = tick - j;
+            if (t < 0) break;
+            k += stim_hash(p.stim_seed, cell, (uint32_t)t) < p.poisson_threshold ? 1 : 0;
+        }
+        return p.stim_amp_a * (float)k;
+    }
+    return 0.0f;
+}
+
+// open' = a(v) + b(v) * open with (a, b) linearly interpolated between grid nodes.
+__device__ __forceinline__ double gate_advance(const float2* tbl, int b, double f, double open) {
+    const float2 t0 = tbl[b];
+    const float2 t1 = tbl[b + 1];
+    const double a = fma(f, (double)t1.x - (double)t0.x, (double)t0.x);
+    const double s = fma(f, (double)t1.y - (double)t0.y, (double)t0.y);
+    return fma(s, open, a);
+}
+
+// hh.mod:88-94; x / (exp(x/y) - 1) evaluated with expm1 (same value, no cancellation near 0).
+__device__ __forceinline__ double vtrap(double x, double y) {
+    const double r = x / y;
+    return (fabs(r) < 1e-6) ? y * (1.0 - r * 0.5) : x / expm1(r);
+}
+
+// Exact update of one hh.mod gate (solver.py:48-55): x' = inf + (x - inf) exp(-(a+b) q10 dt).
+__device__ __forceinline__ double gate_exact(double alpha, double beta, double q10, double dt,
+                                             double x) {
+    const double s = q10 * (alpha + beta);
+    const double inf = alpha / (alpha + beta);
+    return fma(x - inf, exp(-s * dt), inf);
+}
+
+__device__ __forceinline__ void hh_exact(double v, double q10, double dt, double& m, double& h,
+                                         double& n) {
+    // rates, hh.mod:62-86
+    const double am = 0.1 * vtrap(-(v + 40.0), 10.0);
+    const double bm = 4.0 * exp(-(v + 65.0) / 18.0);
+    const double ah = 0.07 * exp(-(v + 65.0) / 20.0);
+    const double bh = 1.0 / (exp(-(v + 35.0) / 10.0) + 1.0);
+    const double an = 0.01 * vtrap(-(v + 55.0), 10.0);
+    const double bn = 0.125 * exp(-(v + 65.0) / 80.0);
+    m = gate_exact(am, bm, q10, dt, m);
+    h = gate_exact(ah, bh, q10, dt, h);
+    n = gate_exact(an, bn, q10, dt, n);
+}
+
+// Breakpoint of the three channels (hh.mod:41-46) and the driving voltage sum(g E)/sum(g)
+// (rxd_model.py:105-109; 0 where the conductance is 0).
+__device__ __forceinline__ void hh_conductance(double gna_bar, double gk_bar, double gl, double m,
+                                               double h, double n, double e_na, double e_k,
+                                               double e_l, double& g, double& d) {
+    const double gna = gna_bar * m * m * m * h;
+    const double gk = gk_bar * n * n * n * n;
+    g = gna + gk + gl;
+    const double num = gna * e_na + gk * e_k + gl * e_l;
+    d = (g != 0.0) ? num / g : 0.0;
+}
+
+// Every per-segment quantity is computed in FP64 registers from FP32 storage; voltages live in LDS
+// as FP32 offsets from kVRef and the row products accumulate in FP64 (row_apply).
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
+    extern __shared__ float lds[];
+    float* xs = lds;            // x - kVRef of the current half step
+    float* vs = lds + a.nmax;   // v - kVRef
+    const nw_hh_params& p = a.prm;
+    const int cell = a.order[blockIdx.x];
+    const int64_t row0 = a.cell_row0[cell];
+    const int nrows = (int)(a.cell_row0[cell + 1] - row0);
+    const uint32_t* roff = a.row_off + row0 + cell;
+    const float* vals = a.vals + a.cell_nz0[cell];
+    const uint16_t* cols = a.cols + a.cell_nz0[cell];
+    const double* rs = a.rowsum + row0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    constexpr int NW = BLOCK / kWave;
+    const double hdt = 0.5 * (double)p.dt_ms;
+    const double hdt_s = hdt * 1e-3;
+    const int nb = p.num_nodes - 1;
+    const double vmin = p.vmin;
+    const double bucket_frq = (double)nb / ((double)p.vmax - (double)p.vmin);
+    const double e_na = p.e_na, e_k = p.e_k, e_l = p.e_l;
+    const float2* tm = reinterpret_cast<const float2*>(a.table);
+    const float2* th = tm + p.num_nodes;
+    const float2* tn = th + p.num_nodes;
+    const int stim_row = a.stim_local ? a.stim_local[cell] : -1;
+    const int ap_row = a.ap_local ? a.ap_local[cell] : -1;
+    uint8_t ap_state = 0;
+    uint32_t ap_count = 0;
+    if (threadIdx.x == 0 && ap_row >= 0) {
+        ap_state = a.ap_state[cell];
+        ap_count = a.ap_count[cell];
+    }
+
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) vs[i] = (float)((double)a.v[row0 + i] - kVRef);
+    __syncthreads();
+
+    for (int step = 0; step < a.n_steps; ++step) {
+        const int64_t gstep = p.step0 + step;
+        const int64_t tick_a = 2 * gstep + 1;
+        const bool g_valid = p.conductances_valid || step > 0;
+        // ---- A: relax with the conductances of the previous mechanism step
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+            const int64_t gi = row0 + i;
+            const double invc = a.inv_c[gi];
+            const double v = kVRef + (double)vs[i];
+            double g = 0.0, d = 0.0;
+            if (g_valid)
+                hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], a.m[gi], a.h[gi], a.n[gi], e_na,
+                               e_k, e_l, g, d);
+            const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick_a) : 0.0;
+            const double alpha = exp(-hdt_s * g * invc);
+            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - kVRef);
+        }
+        __syncthreads();
+        // ---- B: v = P x
+        for (int r = wave; r < nrows; r += NW) {
+            const double s = row_apply(roff, vals, cols, xs, rs[r], r, lane);
+            if (lane == 0) vs[r] = (float)(s - kVRef);
+        }
+        __syncthreads();
+        // ---- C: mechanisms at v(t + dt/2), then relax for the second half step
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+            const int64_t gi = row0 + i;
+            const double v = kVRef + (double)vs[i];
+            double m = a.m[gi], h = a.h[gi], n = a.n[gi];
+            if (p.gate_mode == 1) {
+                hh_exact(v, (double)p.q10, (double)p.dt_ms, m, h, n);
+            } else {
+                int b;
+                double f;
+                bucket_of(v, vmin, bucket_frq, nb, b, f);
+                m = gate_advance(tm, b, f, m);
+                h = gate_advance(th, b, f, h);
+                n = gate_advance(tn, b, f, n);
+            }
+            const float m32 = (float)m, h32 = (float)h, n32 = (float)n;
+            a.m[gi] = m32;
+            a.h[gi] = h32;
+            a.n[gi] = n32;
+            double g, d;  // conductances of the stored (FP32) states
+            hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], m32, h32, n32, e_na, e_k, e_l, g, d);
+            const double invc = a.inv_c[gi];
+            const double I =
+                (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick_a + 1) : 0.0;
+            const double alpha = exp(-hdt_s * g * invc);
+            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - kVRef);
+        }
+        __syncthreads();
+        // ---- D: v = P x
+        for (int r = wave; r < nrows; r += NW) {
+            const double s = row_apply(roff, vals, cols, xs, rs[r], r, lane);
+            if (lane == 0) vs[r] = (float)(s - kVRef);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && ap_row >= 0) {
+            const double vap = kVRef + (double)vs[ap_row];
+            const bool over = vap >= (double)p.ap_threshold;
+            if (over && !ap_state) ++ap_count;
+            ap_state = over ? 1 : 0;
+            if (a.trace) a.trace[(int64_t)cell * a.trace_stride + step] = (float)vap;
+        }
+    }
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) a.v[row0 + i] = (float)(kVRef + (double)vs[i]);
+    if (threadIdx.x == 0 && ap_row >= 0) {
+        a.ap_state[cell] = ap_state;
+        a.ap_count[cell] = ap_count;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// LTI kinetic kernel: thread per instance, S states in registers.
+// ----------------------------------------------------------------------------------------------
+template <int S>
+__global__ void __launch_bounds__(256)
+k_lti(int num_nodes, float vmin, float vmax, const float* __restrict__ table, int64_t M,
+      const float* __restrict__ voltage, const int32_t* __restrict__ seg_index, float* states,
+      int64_t stride, float conserve_sum, int n_steps, const float* __restrict__ weights,
+      const float* __restrict__ gmax, float* __restrict__ cond_out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int nb = num_nodes - 1;
+    const float bucket_frq = (float)nb / (vmax - vmin);
+    const float v = voltage[seg_index ? (int64_t)seg_index[i] : i];
+    int b;
+    float f;
+    bucket_of<float>(v, vmin, bucket_frq, nb, b, f);
+    const float* T0 = table + (int64_t)b * S * S;
+    const float* T1 = T0 + S * S;
+    float s[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) s[k] = states[k * stride + i];
+    for (int step = 0; step < n_steps; ++step) {
+        float out[S];
+#pragma unroll
+        for (int r = 0; r < S; ++r) out[r] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < S; ++c) {  // column-major walk, as the reference kernel (codegen.py:81-94)
+#pragma unroll
+            for (int r = 0; r < S; ++r) {
+                const float t0 = T0[r * S + c];
+                const float pe = fmaf(f, T1[r * S + c] - t0, t0);
+                out[r] = fmaf(pe, s[c], out[r]);
+            }
+        }
+        if (conserve_sum > 0.0f) {  // codegen.py:95-102
+            float tot = 0.0f;
+#pragma unroll
+            for (int r = 0; r < S; ++r) tot += out[r];
+            const float corr = conserve_sum / tot;
+#pragma unroll
+            for (int r = 0; r < S; ++r) out[r] *= corr;
+        }
+#pragma unroll
+        for (int r = 0; r < S; ++r) s[r] = out[r];
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) states[k * stride + i] = s[k];
+    if (cond_out) {
+        float w = 0.0f;
+#pragma unroll
+        for (int k = 0; k < S; ++k) w = fmaf(weights[k], s[k], w);
+        cond_out[i] = gmax[i] * w;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Segmented accumulation: thread per segment over its instances (sorted by segment).
+// ----------------------------------------------------------------------------------------------
+__global__ void k_accumulate(int64_t num_segments, const int64_t* __restrict__ ptr,
+                             const float* __restrict__ g_inst, const float* __restrict__ e_inst,
+                             const float* __restrict__ nonspec_i, float* __restrict__ sum_g,
+                             float* __restrict__ drive_v, float* __restrict__ sum_i) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= num_segments) return;
+    float g = 0.0f, num = 0.0f;
+    for (int64_t k = ptr[s]; k < ptr[s + 1]; ++k) {
+        const float gi = g_inst[k];
+        g += gi;
+        num = fmaf(gi, e_inst[k], num);
+    }
+    sum_g[s] = g;
+    drive_v[s] = (g != 0.0f) ? num / g : 0.0f;
+    sum_i[s] = nonspec_i ? nonspec_i[s] : 0.0f;
+}
+
+}  // namespace nw
+
+using namespace nw;
+
+extern "C" {
+
+int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
+               const float* gate_table, float* v, float* m, float* h, float* n, const float* g_na,
+               const float* g_k, const float* g_l, const float* inv_c, const int32_t* stim_local,
+               const int32_t* ap_local, uint8_t* ap_state, uint32_t* ap_count, float* trace,
+               int64_t trace_stride, void* stream) {
+    NW_REQUIRE(p && prm && gate_table && v && m && h && n && g_na && g_k && g_l && inv_c,
+               "nw_hh_step: null argument");
+    NW_REQUIRE(n_steps >= 0, "n_steps must be non-negative");
+    NW_REQUIRE(prm->num_nodes >= 2 && prm->vmax > prm->vmin, "bad gate table grid");
+    NW_REQUIRE(!ap_local || (ap_state && ap_count), "AP detector needs ap_state and ap_count");
+    NW_REQUIRE(prm->stim_mode == 0 || stim_local, "stimulus needs stim_local");
+    NW_REQUIRE(prm->stim_mode != 2 || (prm->pulse_ticks >= 0 && prm->pulse_ticks <= 100000),
+               "pulse_ticks out of range");
+    if (n_steps == 0) return NW_OK;
+    HHArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
+             p->max_cell_rows,
+             n_steps, *prm, gate_table, v, m, h, n, g_na, g_k, g_l, inv_c, stim_local, ap_local,
+             ap_state, ap_count, trace, trace_stride};
+    const size_t lds = 2 * sizeof(float) * (size_t)p->max_cell_rows;
+    NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
+    hipStream_t s = (hipStream_t)stream;
+    const int bt = prm->block_threads ? prm->block_threads : 256;
+    const unsigned grid = (unsigned)p->num_cells;
+    switch (bt) {
+        case 128: k_hh_step<128><<<grid, 128, lds, s>>>(a); break;
+        case 256: k_hh_step<256><<<grid, 256, lds, s>>>(a); break;
+        case 512: k_hh_step<512><<<grid, 512, lds, s>>>(a); break;
+        case 1024: k_hh_step<1024><<<grid, 1024, lds, s>>>(a); break;
+        default: NW_REQUIRE(false, "block_threads must be 128, 256, 512 or 1024 (got %d)", bt);
+    }
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const float* node_table,
+                   int64_t M, const float* voltage, const int32_t* seg_index, float* states,
+                   int64_t state_stride, float conserve_sum, int32_t n_steps, const float* weights,
+                   const float* gmax, float* cond_out, void* stream) {
+    NW_REQUIRE(node_table && voltage && states, "nw_lti_advance: null argument");
+    NW_REQUIRE(num_nodes >= 2 && vmax > vmin, "bad table grid");
+    NW_REQUIRE(M >= 0 && state_stride >= M && n_steps >= 0, "bad sizes");
+    NW_REQUIRE(!cond_out || (weights && gmax), "conductance output needs weights and gmax");
+    if (M == 0) return NW_OK;
+    const unsigned grid = (unsigned)((M + 255) / 256);
+    hipStream_t s = (hipStream_t)stream;
+#define NW_LTI_CASE(K)                                                                             \
+    case K:                                                                                        \
+        k_lti<K><<<grid, 256, 0, s>>>(num_nodes, vmin, vmax, node_table, M, voltage, seg_index,    \
+                                      states, state_stride, conserve_sum, n_steps, weights, gmax,  \
+                                      cond_out);                                                   \
+        break;
+    switch (S) {
+        NW_LTI_CASE(1) NW_LTI_CASE(2) NW_LTI_CASE(3) NW_LTI_CASE(4) NW_LTI_CASE(5)
+        NW_LTI_CASE(6) NW_LTI_CASE(7) NW_LTI_CASE(8) NW_LTI_CASE(9) NW_LTI_CASE(10)
+        NW_LTI_CASE(11) NW_LTI_CASE(12) NW_LTI_CASE(13) NW_LTI_CASE(14) NW_LTI_CASE(15)
+        NW_LTI_CASE(16)
+        default: NW_REQUIRE(false, "state count %d not supported (1..16)", S);
+    }
+#undef NW_LTI_CASE
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_accumulate(int64_t num_segments, const int64_t* seg_inst_ptr, const float* g_inst,
+                  const float* e_inst, const float* nonspec_i, float* sum_g, float* drive_v,
+                  float* sum_i, void* stream) {
+    NW_REQUIRE(num_segments >= 0 && seg_inst_ptr && sum_g && drive_v && sum_i,
+               "nw_accumulate: null argument");
+    if (num_segments == 0) return NW_OK;
+    k_accumulate<<<(unsigned)((num_segments + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        num_segments, seg_inst_ptr, g_inst, e_inst, nonspec_i, sum_g, drive_v, sum_i);
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,590 @@
+// Exact-integration electric propagator: FP64 construction on the device and the CSR half step.
+//
+// Construction replaces Electric._compute_propagator_matrix (neuwon/rxd/neuron/electric.py:79-117),
+// which assembles A*dt from the segment tree and calls scipy.sparse.linalg.expm on the whole,
+// block-diagonal matrix.  Here every cell block is exponentiated on its own, densely, in FP64:
+//
+//   M = A*dt has zero row sums and non-negative off-diagonals.  With lambda = max |M_ii| the
+//   shifted matrix B = M + lambda*I is entrywise non-negative with row sums exactly lambda, so
+//     expm(M) = (exp(-lambda/2^s) * expm(B/2^s))^(2^s)
+//   where lambda/2^s <= 1/2.  expm(B/2^s) is a degree-15 Taylor polynomial evaluated by the
+//   Paterson-Stockmeyer scheme (6 GEMMs); s squarings follow.  Every term and every product is
+//   non-negative, so no cancellation occurs and tiny entries keep full relative accuracy -- the
+//   pruning threshold is applied to accurate values.  GEMMs run on rocBLAS dgemm (FP64 matrix
+//   cores); the remaining steps are the kernels below.
+//
+// The half step replaces Electric._advance_electric (electric.py:57-76).  One workgroup owns one
+// cell: the relaxed vector x = d - (d - v)*alpha + dt*I/C is staged in LDS, then every wavefront
+// takes whole rows of the cell's CSR block (rows hold hundreds of entries), streams the FP32 values
+// and 16-bit local columns with coalesced loads, gathers x from LDS and reduces across its lanes.
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "nw_internal.h"
+
+namespace nw {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+#define NW_BLAS(expr)                                                                      \
+    do {                                                                                   \
+        rocblas_status s_ = (expr);                                                        \
+        if (s_ != rocblas_status_success) {                                                \
+            ::nw::set_error("%s failed: rocblas status %d (%s:%d)", #expr, (int)s_,        \
+                            __FILE__, __LINE__);                                           \
+            return NW_ERR_BLAS;                                                            \
+        }                                                                                  \
+    } while (0)
+
+// ----------------------------------------------------------------------------------------------
+// Construction kernels (row-major n x n FP64 matrices)
+// ----------------------------------------------------------------------------------------------
+__global__ void k_scatter(double* X, int n, int m, const int32_t* ri, const int32_t* ci,
+                          const double* vals) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) X[(int64_t)ri[k] * n + ci[k]] = vals[k];
+}
+
+__global__ void k_set_diag(double* X, int n, const double* d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) X[(int64_t)i * n + i] = d[i];
+}
+
+// out = a0*I + a1*X1 + a2*X2 + a3*X3
+__global__ void k_lincomb(double* out, const double* X1, const double* X2, const double* X3,
+                          int n, double a0, double a1, double a2, double a3) {
+    int64_t total = (int64_t)n * n;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double v = a1 * X1[k] + a2 * X2[k] + a3 * X3[k];
+        int64_t i = k / n, j = k - i * n;
+        if (i == j) v += a0;
+        out[k] = v;
+    }
+}
+
+// One wavefront per row: number of entries kept (diagonal always kept, off-diagonal kept when
+// |P| >= prune, electric.py:114).
+__global__ void k_count(const double* P, int n, double prune, int32_t* counts) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (row >= n) return;
+    const double* pr = P + (int64_t)row * n;
+    int cnt = 0;
+    for (int j0 = 0; j0 < n; j0 += kWave) {
+        int j = j0 + lane;
+        bool keep = (j < n) && (j == row || fabs(pr[j]) >= prune);
+        cnt += __popcll(__ballot(keep));
+    }
+    if (lane == 0) counts[row] = cnt;
+}
+
+// One wavefront per row: store kept entries in column order.  FP32 variant: the diagonal is set so
+// that the FP32 row sums (in FP64) to the FP64 sum of the kept entries.
+template <bool kF32>
+__global__ void k_fill(const double* P, int n, double prune, const uint32_t* row_off_local,
+                       float* vals32, uint16_t* cols16, double* rowsum, double* vals64,
+                       int32_t* cols32) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (row >= n) return;
+    const double* pr = P + (int64_t)row * n;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    float diag32 = 0.0f;
+    if (kF32) {
+        double s64 = 0.0, s32 = 0.0;
+        for (int j0 = 0; j0 < n; j0 += kWave) {
+            int j = j0 + lane;
+            if (j < n) {
+                double p = pr[j];
+                if (j == row || fabs(p) >= prune) {
+                    s64 += p;
+                    if (j != row) s32 += (double)(float)p;
+                }
+            }
+        }
+        s64 = wave_sum(s64);
+        s32 = wave_sum(s32);
+        diag32 = (float)(s64 - s32);
+        if (lane == 0) rowsum[row] = s32 + (double)diag32;
+    }
+    uint32_t pos = row_off_local[row];
+    for (int j0 = 0; j0 < n; j0 += kWave) {
+        int j = j0 + lane;
+        double p = (j < n) ? pr[j] : 0.0;
+        bool keep = (j < n) && (j == row || fabs(p) >= prune);
+        uint64_t mask = __ballot(keep);
+        if (keep) {
+            uint32_t at = pos + (uint32_t)__popcll(mask & lt);
+            if (kF32) {
+                vals32[at] = (j == row) ? diag32 : (float)p;
+                cols16[at] = (uint16_t)j;
+            } else {
+                vals64[at] = p;
+                cols32[at] = j;
+            }
+        }
+        pos += (uint32_t)__popcll(mask);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Host-side construction driver
+// ----------------------------------------------------------------------------------------------
+struct Workspace {
+    int nmax = 0;
+    double* X[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int32_t *d_ri = nullptr, *d_ci = nullptr, *d_counts = nullptr;
+    double *d_v = nullptr, *d_diag = nullptr;
+    uint32_t* d_off = nullptr;
+    rocblas_handle blas = nullptr;
+    hipStream_t stream = nullptr;
+
+    int init(int n) {
+        nmax = n;
+        size_t nn = (size_t)n * n * sizeof(double);
+        for (auto& x : X) NW_HIP(hipMalloc(&x, nn));
+        NW_HIP(hipMalloc(&d_ri, sizeof(int32_t) * 2 * n));
+        NW_HIP(hipMalloc(&d_ci, sizeof(int32_t) * 2 * n));
+        NW_HIP(hipMalloc(&d_v, sizeof(double) * 2 * n));
+        NW_HIP(hipMalloc(&d_diag, sizeof(double) * n));
+        NW_HIP(hipMalloc(&d_counts, sizeof(int32_t) * n));
+        NW_HIP(hipMalloc(&d_off, sizeof(uint32_t) * (n + 1)));
+        NW_HIP(hipStreamCreate(&stream));
+        NW_BLAS(rocblas_create_handle(&blas));
+        NW_BLAS(rocblas_set_stream(blas, stream));
+        NW_BLAS(rocblas_set_pointer_mode(blas, rocblas_pointer_mode_host));
+        return NW_OK;
+    }
+    ~Workspace() {
+        for (auto& x : X)
+            if (x) (void)hipFree(x);
+        if (d_ri) (void)hipFree(d_ri);
+        if (d_ci) (void)hipFree(d_ci);
+        if (d_v) (void)hipFree(d_v);
+        if (d_diag) (void)hipFree(d_diag);
+        if (d_counts) (void)hipFree(d_counts);
+        if (d_off) (void)hipFree(d_off);
+        if (blas) rocblas_destroy_handle(blas);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static int gemm(Workspace& w, int n, const double* A, const double* B, double* C, double beta) {
+    const double one = 1.0;
+    NW_BLAS(rocblas_dgemm(w.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                          A, n, B, n, &beta, C, n));
+    return NW_OK;
+}
+
+// Dense FP64 expm of one cell's A*dt; on success *result points at one of the workspace buffers.
+static int expm_cell(Workspace& w, int n, const int32_t* parent, const double* R, const double* C,
+                     double dt_ms, double** result) {
+    const double dt = dt_ms * 1e-3;  // seconds, electric.py:86
+    std::vector<double> diag(n, 0.0);
+    std::vector<int32_t> ri, ci;
+    std::vector<double> vv;
+    ri.reserve(2 * n);
+    ci.reserve(2 * n);
+    vv.reserve(2 * n);
+    for (int c = 0; c < n; ++c) {  // electric.py:92-108
+        int p = parent[c];
+        if (p < 0) continue;
+        NW_REQUIRE(p < n, "parent index %d out of range for cell of %d segments", p, n);
+        double r = R[c];
+        ri.push_back(p); ci.push_back(c); vv.push_back(+dt / (r * C[p]));
+        diag[c] -= dt / (r * C[c]);
+        ri.push_back(c); ci.push_back(p); vv.push_back(+dt / (r * C[c]));
+        diag[p] -= dt / (r * C[p]);
+    }
+    double lambda = 0.0;
+    for (double d : diag) lambda = std::max(lambda, -d);
+    int s = 0;
+    while (lambda * std::ldexp(1.0, -s) > 0.5) ++s;
+    const double scale = std::ldexp(1.0, -s);
+    const double mu = lambda * scale;
+    for (auto& x : vv) x *= scale;
+    for (int i = 0; i < n; ++i) diag[i] = (diag[i] + lambda) * scale;
+    const int m = (int)vv.size();
+
+    double *X1 = w.X[0], *X2 = w.X[1], *X3 = w.X[2], *X4 = w.X[3], *T = w.X[4], *E = w.X[5];
+    const size_t nn = (size_t)n * n;
+    NW_HIP(hipMemsetAsync(X1, 0, nn * sizeof(double), w.stream));
+    if (m > 0) {
+        NW_HIP(hipMemcpyAsync(w.d_ri, ri.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, w.stream));
+        NW_HIP(hipMemcpyAsync(w.d_ci, ci.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, w.stream));
+        NW_HIP(hipMemcpyAsync(w.d_v, vv.data(), m * sizeof(double), hipMemcpyHostToDevice, w.stream));
+        k_scatter<<<(m + 255) / 256, 256, 0, w.stream>>>(X1, n, m, w.d_ri, w.d_ci, w.d_v);
+    }
+    NW_HIP(hipMemcpyAsync(w.d_diag, diag.data(), n * sizeof(double), hipMemcpyHostToDevice, w.stream));
+    k_set_diag<<<(n + 255) / 256, 256, 0, w.stream>>>(X1, n, w.d_diag);
+    NW_HIP(hipGetLastError());
+
+    // Taylor coefficients exp(-mu)/k!, k = 0..15
+    double c[16];
+    c[0] = std::exp(-mu);
+    for (int k = 1; k < 16; ++k) c[k] = c[k - 1] / k;
+    int rc;
+    if ((rc = gemm(w, n, X1, X1, X2, 0.0))) return rc;
+    if ((rc = gemm(w, n, X2, X1, X3, 0.0))) return rc;
+    if ((rc = gemm(w, n, X2, X2, X4, 0.0))) return rc;
+    const int lb = 256;
+    const int lg = (int)std::min<int64_t>(((int64_t)nn + lb - 1) / lb, 8192);
+    // Paterson-Stockmeyer: E = Q0 + X4 (Q1 + X4 (Q2 + X4 Q3)),  Qj = sum_{i<4} c_{4j+i} X^i
+    k_lincomb<<<lg, lb, 0, w.stream>>>(T, X1, X2, X3, n, c[12], c[13], c[14], c[15]);
+    k_lincomb<<<lg, lb, 0, w.stream>>>(E, X1, X2, X3, n, c[8], c[9], c[10], c[11]);
+    if ((rc = gemm(w, n, X4, T, E, 1.0))) return rc;
+    k_lincomb<<<lg, lb, 0, w.stream>>>(T, X1, X2, X3, n, c[4], c[5], c[6], c[7]);
+    if ((rc = gemm(w, n, X4, E, T, 1.0))) return rc;
+    k_lincomb<<<lg, lb, 0, w.stream>>>(E, X1, X2, X3, n, c[0], c[1], c[2], c[3]);
+    if ((rc = gemm(w, n, X4, T, E, 1.0))) return rc;
+    NW_HIP(hipGetLastError());
+    double* cur = E;
+    double* nxt = T;
+    for (int k = 0; k < s; ++k) {
+        if ((rc = gemm(w, n, cur, cur, nxt, 0.0))) return rc;
+        std::swap(cur, nxt);
+    }
+    *result = cur;
+    return NW_OK;
+}
+
+// Row counts -> local CSR offsets on the host; returns entries of the cell.
+static int count_rows(Workspace& w, const double* P, int n, double prune,
+                      std::vector<uint32_t>& off, int32_t& max_row) {
+    const int rows_per_block = 4;
+    k_count<<<(n + rows_per_block - 1) / rows_per_block, rows_per_block * kWave, 0, w.stream>>>(
+        P, n, prune, w.d_counts);
+    NW_HIP(hipGetLastError());
+    std::vector<int32_t> counts(n);
+    NW_HIP(hipMemcpyAsync(counts.data(), w.d_counts, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          w.stream));
+    NW_HIP(hipStreamSynchronize(w.stream));
+    off.assign(n + 1, 0);
+    max_row = 0;
+    uint64_t acc = 0;
+    for (int i = 0; i < n; ++i) {
+        off[i] = (uint32_t)acc;
+        acc += (uint64_t)counts[i];
+        max_row = std::max(max_row, counts[i]);
+    }
+    NW_REQUIRE(acc < 0xFFFFFFFFull, "cell block too large for 32-bit row offsets");
+    off[n] = (uint32_t)acc;
+    return NW_OK;
+}
+
+static int grow(nw_propagator* p, int64_t need) {
+    if (need <= p->vals_capacity) return NW_OK;
+    int64_t cap = std::max<int64_t>(need, (int64_t)(p->vals_capacity * 1.5));
+    float* nv = nullptr;
+    uint16_t* nc = nullptr;
+    if (hipMalloc(&nv, cap * sizeof(float)) != hipSuccess ||
+        hipMalloc(&nc, cap * sizeof(uint16_t)) != hipSuccess) {
+        if (nv) (void)hipFree(nv);
+        set_error("out of device memory growing the propagator to %lld entries", (long long)cap);
+        return NW_ERR_ALLOC;
+    }
+    if (p->nnz > 0) {
+        NW_HIP(hipMemcpy(nv, p->vals, p->nnz * sizeof(float), hipMemcpyDeviceToDevice));
+        NW_HIP(hipMemcpy(nc, p->cols, p->nnz * sizeof(uint16_t), hipMemcpyDeviceToDevice));
+    }
+    if (p->vals) (void)hipFree(p->vals);
+    if (p->cols) (void)hipFree(p->cols);
+    p->vals = nv;
+    p->cols = nc;
+    p->vals_capacity = cap;
+    return NW_OK;
+}
+
+static void free_prop(nw_propagator* p) {
+    if (!p) return;
+    if (p->cell_row0) (void)hipFree(p->cell_row0);
+    if (p->cell_nz0) (void)hipFree(p->cell_nz0);
+    if (p->row_off) (void)hipFree(p->row_off);
+    if (p->vals) (void)hipFree(p->vals);
+    if (p->cols) (void)hipFree(p->cols);
+    if (p->rowsum) (void)hipFree(p->rowsum);
+    if (p->order) (void)hipFree(p->order);
+    delete p;
+}
+
+static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const int32_t* parent,
+                 const double* R, const double* C, double dt_ms, double prune, nw_propagator* p) {
+    auto t0 = std::chrono::steady_clock::now();
+    NW_REQUIRE(num_cells > 0, "num_cells must be positive");
+    NW_REQUIRE(cell_ptr[0] == 0, "cell_ptr[0] must be 0");
+    NW_REQUIRE(dt_ms > 0.0 && prune >= 0.0, "time step must be positive and prune non-negative");
+    int nmax = 0;
+    for (int64_t c = 0; c < num_cells; ++c) {
+        int64_t n = cell_ptr[c + 1] - cell_ptr[c];
+        NW_REQUIRE(n >= 1 && n <= 65535, "cell %lld has %lld segments (1..65535 supported)",
+                   (long long)c, (long long)n);
+        nmax = std::max<int>(nmax, (int)n);
+    }
+    NW_HIP(hipSetDevice(device));
+    p->device = device;
+    p->num_cells = num_cells;
+    p->num_rows = cell_ptr[num_cells];
+    p->max_cell_rows = nmax;
+    p->h_cell_row0.assign(cell_ptr, cell_ptr + num_cells + 1);
+    p->h_cell_nz0.assign(num_cells, 0);
+    NW_HIP(hipMalloc(&p->row_off, sizeof(uint32_t) * (p->num_rows + num_cells)));
+    NW_HIP(hipMalloc(&p->rowsum, sizeof(double) * p->num_rows));
+    // capacity guess; grown on demand
+    int64_t guess = 0;
+    for (int64_t c = 0; c < num_cells; ++c) {
+        int64_t n = cell_ptr[c + 1] - cell_ptr[c];
+        guess += n * std::min<int64_t>(n, 640);
+    }
+    int rc;
+    if ((rc = grow(p, std::max<int64_t>(guess, 1)))) return rc;
+
+    Workspace w;
+    if ((rc = w.init(nmax))) return rc;
+    std::vector<uint32_t> off;
+    std::vector<int64_t> cell_nnz(num_cells);
+    for (int64_t c = 0; c < num_cells; ++c) {
+        int64_t r0 = cell_ptr[c];
+        int n = (int)(cell_ptr[c + 1] - r0);
+        double* P = nullptr;
+        if ((rc = expm_cell(w, n, parent + r0, R + r0, C + r0, dt_ms, &P))) return rc;
+        int32_t max_row = 0;
+        if ((rc = count_rows(w, P, n, prune, off, max_row))) return rc;
+        p->max_row_nnz = std::max(p->max_row_nnz, max_row);
+        int64_t cnt = off[n];
+        if ((rc = grow(p, p->nnz + cnt))) return rc;
+        uint32_t* dev_off = p->row_off + r0 + c;
+        NW_HIP(hipMemcpyAsync(dev_off, off.data(), (n + 1) * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, w.stream));
+        k_fill<true><<<(n + 3) / 4, 4 * kWave, 0, w.stream>>>(P, n, prune, dev_off,
+                                                              p->vals + p->nnz, p->cols + p->nnz,
+                                                              p->rowsum + r0, nullptr, nullptr);
+        NW_HIP(hipGetLastError());
+        p->h_cell_nz0[c] = p->nnz;
+        cell_nnz[c] = cnt;
+        p->nnz += cnt;
+    }
+    NW_HIP(hipStreamSynchronize(w.stream));
+    NW_HIP(hipMalloc(&p->cell_row0, sizeof(int64_t) * (num_cells + 1)));
+    NW_HIP(hipMalloc(&p->cell_nz0, sizeof(int64_t) * num_cells));
+    NW_HIP(hipMemcpy(p->cell_row0, cell_ptr, sizeof(int64_t) * (num_cells + 1),
+                     hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->cell_nz0, p->h_cell_nz0.data(), sizeof(int64_t) * num_cells,
+                     hipMemcpyHostToDevice));
+    // launch order: largest stored blocks first (longest-processing-time scheduling)
+    std::vector<int32_t> order(num_cells);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return cell_nnz[a] > cell_nnz[b]; });
+    NW_HIP(hipMalloc(&p->order, sizeof(int32_t) * num_cells));
+    NW_HIP(hipMemcpy(p->order, order.data(), sizeof(int32_t) * num_cells, hipMemcpyHostToDevice));
+    p->build_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return NW_OK;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Half step
+// ----------------------------------------------------------------------------------------------
+struct HalfArgs {
+    const int64_t* cell_row0;
+    const int64_t* cell_nz0;
+    const uint32_t* row_off;
+    const float* vals;
+    const uint16_t* cols;
+    const double* rowsum;
+    const int32_t* order;
+    float half_dt_ms;
+    float* v;
+    const float* sum_g;
+    const float* drive_v;
+    const float* sum_i;
+    const float* inv_c;
+    float* integral_v;
+};
+
+// FP32 storage, FP64 arithmetic: x is formed in FP64 and staged in LDS as an FP32 offset from
+// kVRef; the row products accumulate in FP64 (row_apply).
+__global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
+    extern __shared__ float xs[];
+    const int cell = a.order[blockIdx.x];
+    const int64_t row0 = a.cell_row0[cell];
+    const int n = (int)(a.cell_row0[cell + 1] - row0);
+    const double dt_ms = a.half_dt_ms;
+    const double dt_s = dt_ms * 1e-3;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int64_t gi = row0 + i;
+        const double invc = a.inv_c[gi];
+        const double d = a.drive_v[gi];
+        const double v = a.v[gi];
+        const double dv = dt_ms * (double)a.sum_i[gi] * invc;
+        const double expo = -dt_s * (double)a.sum_g[gi] * invc;
+        const double alpha = exp(expo);
+        const double diff = d - v;
+        xs[i] = (float)((d - diff * alpha + dv) - kVRef);
+        if (a.integral_v)
+            a.integral_v[gi] = (float)(dt_s * d - expo * diff * alpha + 0.5 * dt_s * dv);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nwaves = blockDim.x / kWave;
+    const uint32_t* roff = a.row_off + row0 + cell;
+    const float* vals = a.vals + a.cell_nz0[cell];
+    const uint16_t* cols = a.cols + a.cell_nz0[cell];
+    const double* rs = a.rowsum + row0;
+    for (int r = wave; r < n; r += nwaves) {
+        const double s = row_apply(roff, vals, cols, xs, rs[r], r, lane);
+        if (lane == 0) a.v[row0 + r] = (float)s;
+    }
+}
+
+}  // namespace nw
+
+using namespace nw;
+
+extern "C" {
+
+const char* nw_last_error(void) { return nw::g_err; }
+
+const char* nw_version(void) { return "neuwon_amd 0.1 (gfx950)"; }
+
+int nw_device_info(int device, char* name_out, int name_len, int* num_cus, int* lds_bytes) {
+    hipDeviceProp_t prop;
+    NW_HIP(hipGetDeviceProperties(&prop, device));
+    if (name_out && name_len > 0) {
+        snprintf(name_out, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    if (lds_bytes) *lds_bytes = (int)prop.sharedMemPerBlock;
+    return NW_OK;
+}
+
+uint32_t nw_stim_hash(uint32_t seed, uint32_t cell, uint32_t tick) {
+    return nw::stim_hash(seed, cell, tick);
+}
+
+int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_host,
+                        const int32_t* parent_local_host, const double* R_host,
+                        const double* C_host, double time_step_ms, double prune,
+                        nw_propagator** out) {
+    NW_REQUIRE(out && cell_ptr_host && parent_local_host && R_host && C_host,
+               "nw_propagator_build: null argument");
+    *out = nullptr;
+    auto* p = new nw_propagator();
+    int rc = build(device, num_cells, cell_ptr_host, parent_local_host, R_host, C_host,
+                   time_step_ms, prune, p);
+    if (rc) {
+        free_prop(p);
+        return rc;
+    }
+    *out = p;
+    return NW_OK;
+}
+
+int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
+    NW_REQUIRE(p && s, "nw_propagator_stats_get: null argument");
+    s->num_cells = p->num_cells;
+    s->num_rows = p->num_rows;
+    s->nnz = p->nnz;
+    s->max_cell_rows = p->max_cell_rows;
+    s->max_row_nnz = p->max_row_nnz;
+    s->build_seconds = p->build_seconds;
+    s->device_bytes = p->vals_capacity * (int64_t)(sizeof(float) + sizeof(uint16_t)) +
+                      (p->num_rows + p->num_cells) * (int64_t)sizeof(uint32_t) +
+                      p->num_rows * (int64_t)sizeof(double) +
+                      p->num_cells * (int64_t)(2 * sizeof(int64_t) + sizeof(int32_t));
+    return NW_OK;
+}
+
+int nw_propagator_destroy(nw_propagator* p) {
+    free_prop(p);
+    return NW_OK;
+}
+
+int nw_propagator_cell_f64(int device, int32_t n, const int32_t* parent_local_host,
+                           const double* R_host, const double* C_host, double time_step_ms,
+                           double prune, int64_t* nnz_out, int64_t* row_ptr_host,
+                           int32_t* col_host, double* val_host) {
+    NW_REQUIRE(n >= 1 && n <= 65535, "cell size %d out of range", n);
+    NW_REQUIRE(parent_local_host && R_host && C_host && nnz_out, "null argument");
+    NW_HIP(hipSetDevice(device));
+    Workspace w;
+    int rc;
+    if ((rc = w.init(n))) return rc;
+    double* P = nullptr;
+    if ((rc = expm_cell(w, n, parent_local_host, R_host, C_host, time_step_ms, &P))) return rc;
+    std::vector<uint32_t> off;
+    int32_t max_row;
+    if ((rc = count_rows(w, P, n, prune, off, max_row))) return rc;
+    *nnz_out = off[n];
+    if (!val_host) return NW_OK;
+    NW_REQUIRE(row_ptr_host && col_host, "null output array");
+    double* dv = nullptr;
+    int32_t* dc = nullptr;
+    NW_HIP(hipMalloc(&dv, sizeof(double) * std::max<int64_t>(off[n], 1)));
+    NW_HIP(hipMalloc(&dc, sizeof(int32_t) * std::max<int64_t>(off[n], 1)));
+    NW_HIP(hipMemcpyAsync(w.d_off, off.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          w.stream));
+    k_fill<false><<<(n + 3) / 4, 4 * kWave, 0, w.stream>>>(P, n, prune, w.d_off, nullptr, nullptr,
+                                                           nullptr, dv, dc);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(val_host, dv, sizeof(double) * off[n],
+                                            hipMemcpyDeviceToHost, w.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(col_host, dc, sizeof(int32_t) * off[n],
+                                            hipMemcpyDeviceToHost, w.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+    (void)hipFree(dv);
+    (void)hipFree(dc);
+    if (e != hipSuccess) {
+        set_error("nw_propagator_cell_f64: %s", hipGetErrorString(e));
+        return NW_ERR_HIP;
+    }
+    for (int i = 0; i <= n; ++i) row_ptr_host[i] = off[i];
+    return NW_OK;
+}
+
+int nw_propagator_cell_export(const nw_propagator* p, int64_t cell, uint32_t* row_off_host,
+                              uint16_t* col_host, float* val_host) {
+    NW_REQUIRE(p && cell >= 0 && cell < p->num_cells, "cell index out of range");
+    NW_HIP(hipSetDevice(p->device));
+    int64_t r0 = p->h_cell_row0[cell];
+    int n = (int)(p->h_cell_row0[cell + 1] - r0);
+    std::vector<uint32_t> off(n + 1);
+    NW_HIP(hipMemcpy(off.data(), p->row_off + r0 + cell, (n + 1) * sizeof(uint32_t),
+                     hipMemcpyDeviceToHost));
+    if (row_off_host) std::memcpy(row_off_host, off.data(), (n + 1) * sizeof(uint32_t));
+    int64_t z0 = p->h_cell_nz0[cell];
+    if (col_host)
+        NW_HIP(hipMemcpy(col_host, p->cols + z0, off[n] * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    if (val_host)
+        NW_HIP(hipMemcpy(val_host, p->vals + z0, off[n] * sizeof(float), hipMemcpyDeviceToHost));
+    return NW_OK;
+}
+
+int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, const float* sum_g,
+                          const float* drive_v, const float* sum_i, const float* inv_c,
+                          float* integral_v, void* stream) {
+    NW_REQUIRE(p && v && sum_g && drive_v && sum_i && inv_c, "nw_electric_half_step: null argument");
+    HalfArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
+               half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v};
+    size_t lds = sizeof(float) * (size_t)p->max_cell_rows;
+    NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
+    k_half_step<<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+"""Device-resident batches of independent HH cells advanced by the fused step kernel.
+
+This is the throughput path: thousands of cells of uneven size share one block-diagonal
+propagator and one launch advances all of them through any number of steps
+(RxD_Model.advance, neuwon/rxd/rxd_model.py:63-123, once per step for every cell).
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib, channels, geometry, synth
+
+STIM_NONE, STIM_CONSTANT, STIM_POISSON = 0, 1, 2
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("neuwon_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+class Propagator:
+    """Owner of an ``nw_propagator`` handle (per-cell pruned CSR, FP32 values)."""
+
+    def __init__(self, cell_ptr, parent_local, R, C, half_dt_ms, prune, device=0):
+        lib = _lib.load()
+        self.cell_ptr = np.ascontiguousarray(cell_ptr, dtype=np.int64)
+        self._parent = np.ascontiguousarray(parent_local, dtype=np.int32)
+        self._R = np.ascontiguousarray(R, dtype=np.float64)
+        self._C = np.ascontiguousarray(C, dtype=np.float64)
+        self.half_dt_ms, self.prune, self.device = float(half_dt_ms), float(prune), int(device)
+        h = ctypes.c_void_p()
+        with _lib.lock():
+            _lib.check(lib.nw_propagator_build(self.device, len(self.cell_ptr) - 1,
+                                               _lib.ptr(self.cell_ptr), _lib.ptr(self._parent),
+                                               _lib.ptr(self._R), _lib.ptr(self._C),
+                                               self.half_dt_ms, self.prune, ctypes.byref(h)))
+        self.handle = h
+        st = _lib.NwPropagatorStats()
+        _lib.check(lib.nw_propagator_stats_get(self.handle, ctypes.byref(st)))
+        self.stats = {k: getattr(st, k) for k, _ in st._fields_}
+
+    @property
+    def mean_nnz_per_row(self):
+        return self.stats["nnz"] / max(1, self.stats["num_rows"])
+
+    def export_cell(self, cell):
+        """Stored (FP32 value, u16 column) CSR block of one cell as a scipy matrix (FP64)."""
+        import scipy.sparse
+        n = int(self.cell_ptr[cell + 1] - self.cell_ptr[cell])
+        off = np.empty(n + 1, dtype=np.uint32)
+        _lib.check(_lib.load().nw_propagator_cell_export(self.handle, cell, _lib.ptr(off), None, None))
+        cols = np.empty(int(off[-1]), dtype=np.uint16)
+        vals = np.empty(int(off[-1]), dtype=np.float32)
+        _lib.check(_lib.load().nw_propagator_cell_export(self.handle, cell, None, _lib.ptr(cols),
+                                                         _lib.ptr(vals)))
+        return scipy.sparse.csr_matrix((vals.astype(np.float64), cols.astype(np.int64),
+                                        off.astype(np.int64)), shape=(n, n))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().nw_propagator_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cell_propagator_f64(parent_local, R, C, half_dt_ms, prune, device=0):
+    """Pruned FP64 propagator of one cell from the device construction path (scipy CSR)."""
+    import scipy.sparse
+    lib = _lib.load()
+    parent_local = np.ascontiguousarray(parent_local, dtype=np.int32)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    n = len(parent_local)
+    nnz = ctypes.c_int64()
+    args = (device, n, _lib.ptr(parent_local), _lib.ptr(R), _lib.ptr(C), float(half_dt_ms),
+            float(prune))
+    _lib.check(lib.nw_propagator_cell_f64(*args, ctypes.byref(nnz), None, None, None))
+    rp = np.empty(n + 1, dtype=np.int64)
+    ci = np.empty(nnz.value, dtype=np.int32)
+    va = np.empty(nnz.value, dtype=np.float64)
+    _lib.check(lib.nw_propagator_cell_f64(*args, ctypes.byref(nnz), _lib.ptr(rp), _lib.ptr(ci),
+                                          _lib.ptr(va)))
+    return scipy.sparse.csr_matrix((va, ci, rp), shape=(n, n))
+
+
+class HHPopulation:
+    """Cells carrying leak + HH Na (m^3 h) + HH K (n^4) on every segment.
+
+    cells: list of ``synth.CellSpec``.  Mechanisms are inserted after each cell is grown, so
+    conductances use the final surface area (magnitude = area, nmodl/__init__.py:499) while the
+    capacitance uses the area at creation (electric.py:45-54).
+
+    stim: dict(mode='constant'|'poisson'|None, amp_na, rate_hz, duration_ms, seed) applied to the
+    root (segment 0) of every cell.
+    """
+
+    def __init__(self, cells, dt=0.1, celsius=6.3, v0=-70.0, cytoplasmic_resistance=100.0,
+                 membrane_capacitance=1.0, gbar=None, reversal=None, prune=1e-9, gate_mode="table",
+                 stim=None, ap_threshold=20.0, device=0, block_threads=0, insert_area="final"):
+        torch = _torch()
+        self.torch = torch
+        self.dev = torch.device("cuda", device)
+        self.dt = float(dt)
+        self.celsius = float(celsius)
+        self.gate_mode = gate_mode
+        self.block_threads = int(block_threads)
+        g = synth.concat(cells)
+        self.num_cells = len(cells)
+        self.cell_ptr = g["cell_ptr"]
+        self.num_segments = int(self.cell_ptr[-1])
+        geo = geometry.geometry(g["parent"], g["length"], g["diam"])
+        R, C = geometry.passive(geo["length"], geo["cross_sectional_area"], geo["area_at_creation"],
+                                cytoplasmic_resistance, membrane_capacitance)
+        self.R, self.C, self.geo, self.parent_local = R, C, geo, g["parent_local"]
+        self.propagator = Propagator(self.cell_ptr, g["parent_local"], R, C, 0.5 * self.dt, prune,
+                                     device)
+        gbar = dict(channels.HH_GBAR, **(gbar or {}))
+        self.reversal = dict(channels.HH_REVERSAL, **(reversal or {}))
+        # magnitude = surface area when the mechanism is inserted: after the cell is grown
+        # ("final") or right after each segment is created ("creation")
+        area = geo["surface_area"] if insert_area == "final" else geo["area_at_creation"]
+        sna = np.repeat([c.scale_na for c in cells], np.diff(self.cell_ptr))
+        sk = np.repeat([c.scale_k for c in cells], np.diff(self.cell_ptr))
+        self.g_bar = {"na": gbar["na"] * channels.CM2_TO_UM2 * area * sna,
+                      "k": gbar["k"] * channels.CM2_TO_UM2 * area * sk,
+                      "l": gbar["l"] * channels.CM2_TO_UM2 * area}
+        f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=self.dev)
+        ss = channels.hh_steady_state(v0, celsius)
+        N = self.num_segments
+        self.v = f32(np.full(N, v0))
+        self.m = f32(np.full(N, ss["m"]))
+        self.h = f32(np.full(N, ss["h"]))
+        self.n = f32(np.full(N, ss["n"]))
+        self.g_na, self.g_k, self.g_l = f32(self.g_bar["na"]), f32(self.g_bar["k"]), f32(self.g_bar["l"])
+        self.inv_c = f32(1.0 / C)
+        self.grid = channels.LinearInput(-100.0, 100.0, 2000)
+        self.table64 = channels.hh_gate_table(self.dt, celsius, self.grid)
+        self.table = f32(self.table64)
+        zeros_i32 = np.zeros(self.num_cells, dtype=np.int32)
+        self.stim_local = torch.as_tensor(zeros_i32, device=self.dev)
+        self.ap_local = torch.as_tensor(zeros_i32, device=self.dev)
+        self.ap_state = torch.zeros(self.num_cells, dtype=torch.uint8, device=self.dev)
+        self.ap_count = torch.zeros(self.num_cells, dtype=torch.int32, device=self.dev)
+        self.steps = 0
+        p = _lib.NwHHParams()
+        p.dt_ms, p.vmin, p.vmax, p.num_nodes = self.dt, self.grid.minimum, self.grid.maximum, \
+            self.grid.num_buckets + 1
+        p.e_na, p.e_k, p.e_l = self.reversal["na"], self.reversal["k"], self.reversal["l"]
+        p.ap_threshold = ap_threshold
+        p.gate_mode = {"table": 0, "exact": 1}[gate_mode]
+        p.q10 = channels.q10_factor(celsius)
+        p.block_threads = self.block_threads
+        self.params = p
+        self.set_stim(stim)
+
+    def set_stim(self, stim):
+        """Stimulus of the root of every cell from the next step on (None switches it off)."""
+        p = self.params
+        stim = stim or {}
+        mode = stim.get("mode")
+        self.stim = stim
+        if mode == "constant":
+            p.stim_mode = STIM_CONSTANT
+            p.stim_amp_a = float(stim["amp_na"]) * 1e-9
+        elif mode == "poisson":
+            p.stim_mode = STIM_POISSON
+            p.stim_amp_a = float(stim["amp_na"]) * 1e-9
+            p.stim_seed = int(stim.get("seed", 0)) & 0xFFFFFFFF
+            half = 0.5 * self.dt
+            p.poisson_threshold = int(math.floor(float(stim["rate_hz"]) * half * 1e-3 * 4294967296.0))
+            p.pulse_ticks = int(math.ceil(float(stim["duration_ms"]) / half))
+        else:
+            p.stim_mode = STIM_NONE
+
+    def advance(self, n_steps=1, trace=None, stream=None):
+        """Advance every cell by n_steps time steps (one launch).  trace: optional float32 device
+        tensor [num_cells, >= n_steps] receiving the root voltage after each step."""
+        torch = self.torch
+        p = self.params
+        p.step0 = self.steps
+        p.conductances_valid = 1 if self.steps > 0 else 0
+        s = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        tr_ptr, tr_stride = None, 0
+        if trace is not None:
+            assert trace.dtype == torch.float32 and trace.is_contiguous()
+            assert trace.shape[0] == self.num_cells and trace.shape[1] >= n_steps
+            tr_ptr, tr_stride = _lib.ptr(trace), trace.shape[1]
+        P = _lib.ptr
+        with _lib.lock():
+            _lib.check(_lib.load().nw_hh_step(
+                self.propagator.handle, ctypes.byref(p), int(n_steps), P(self.table), P(self.v),
+                P(self.m), P(self.h), P(self.n), P(self.g_na), P(self.g_k), P(self.g_l),
+                P(self.inv_c), P(self.stim_local), P(self.ap_local), P(self.ap_state),
+                P(self.ap_count), tr_ptr, tr_stride, ctypes.c_void_p(s.cuda_stream)))
+        self.steps += n_steps
+
+    def time(self):
+        return self.steps * self.dt

@@ -1,0 +1,465 @@
+"""FP64 CPU restatement of NEUWON's per-timestep advance -- TEST INFRASTRUCTURE ONLY.
+
+This module is the oracle: tests, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` use it as the checker.  The product (``neuwon_amd``) never imports it.
+
+Every function restates one piece of the reference algorithm in plain numpy/scipy FP64 and cites
+the reference file:line it follows (paths relative to the reference repository root).  Loops are
+written the way the reference reads, not for speed.
+
+Pinning (what each function is checked against, see ``tests/test_oracle.py``):
+  * geometry / passive R,C      -- closed-form areas, reference section counts, HH golden trace
+  * propagator                  -- scipy.sparse.linalg.expm (the reference's own routine), P.1 = 1,
+                                   c^T P = c^T, RC relaxation 1-1/e, injected charge dQ = I.t
+  * half_step                   -- RC closed form, charge injection, length-constant attenuation
+  * hh gates (exact)            -- HH golden trace (neuwon/rxd/tests/test_model.py:77-113)
+  * LTI tables / kernel         -- 2-state closed-form expm (exact), scipy.linalg.expm, column sums,
+                                   bucket-mapping known answers (lti_sim/tests/test_inputs.py:5-35)
+  * accumulation                -- np.add.at brute force
+  * stimulus hash               -- integer identities (bit-exact uint32 arithmetic)
+"""
+import math
+
+import numpy as np
+import scipy.linalg
+import scipy.sparse
+import scipy.sparse.linalg
+
+EPSILON = float(np.finfo(np.float64).eps)   # neuwon/database/dtypes.py:4
+NULL = -1                                    # root marker (reference uses 0xFFFFFFFF, dtypes.py:6)
+
+
+# ---------------------------------------------------------------------------------------------
+# Geometry and passive electric properties
+# ---------------------------------------------------------------------------------------------
+def geometry(parent, coords, diam):
+    """Per-segment geometry, segments given in creation order (parent index < child index).
+
+    Follows neuwon/rxd/neuron/geometry.py:75-155 and segment.py:27-39:
+      * root is a sphere: length = 2/3 d, area = pi d^2, volume = 4/3 pi r^3
+      * other segments are cylinders; the first child of a cylinder is "primary", all other
+        children (and every child of a sphere) are secondary branches (geometry.py:80-87)
+      * secondary length subtracts the parent's radius unless that would envelop it (112-127)
+      * surface area: lateral cylinder + tip disk for leaves - attachment disks of non-primary
+        children (129-144).  The reference computes the *capacitance* once when the segment is
+        created (electric.py:45-54 called from segment.py:37), when it has no children yet, so
+        ``area_at_creation`` (no attachment disks, leaf tip included) is kept separately from the
+        final ``surface_area`` (recomputed on the parent whenever a child is added, geometry.py:91).
+    """
+    parent = np.asarray(parent, dtype=np.int64)
+    coords = np.asarray(coords, dtype=np.float64)
+    diam = np.asarray(diam, dtype=np.float64)
+    n = len(parent)
+    length = np.empty(n)
+    primary = np.zeros(n, dtype=bool)
+    children = [[] for _ in range(n)]
+    area_at_creation = np.empty(n)
+    surface_area = np.empty(n)
+
+    def is_sphere(i):
+        return parent[i] == NULL
+
+    def secondary_length(i):
+        L = length[i]
+        if not primary[i]:
+            r = 0.5 * diam[parent[i]]
+            if not (L < r + EPSILON):
+                L -= r
+        return L
+
+    def area_of(i):
+        d = diam[i]
+        if is_sphere(i):
+            a = math.pi * d * d
+        else:
+            a = math.pi * d * secondary_length(i)
+            if len(children[i]) == 0:
+                a += 0.25 * math.pi * d * d
+        for c in children[i]:
+            if not primary[c]:
+                a -= 0.25 * math.pi * min(d, diam[c]) ** 2
+        return a
+
+    for i in range(n):
+        p = parent[i]
+        if p != NULL:
+            children[p].append(i)
+        if is_sphere(i):
+            primary[i] = False
+            length[i] = (2.0 / 3.0) * diam[i]
+        else:
+            primary[i] = False if is_sphere(p) else (len(children[p]) < 2)
+            length[i] = float(np.linalg.norm(coords[i] - coords[p]))
+        surface_area[i] = area_of(i)
+        area_at_creation[i] = surface_area[i]
+        if p != NULL:
+            surface_area[p] = area_of(p)
+    xarea = 0.25 * math.pi * diam ** 2
+    volume = np.empty(n)
+    for i in range(n):
+        if is_sphere(i):
+            volume[i] = (4.0 / 3.0) * math.pi * (0.5 * diam[i]) ** 3
+        else:
+            volume[i] = xarea[i] * secondary_length(i)
+    return dict(length=length, primary=primary, surface_area=surface_area,
+                area_at_creation=area_at_creation, cross_sectional_area=xarea, volume=volume)
+
+
+def passive(length, xarea, area, cytoplasmic_resistance=100.0, membrane_capacitance=1.0):
+    """Axial resistance (Ohm) and capacitance (F).  neuwon/rxd/neuron/electric.py:49-54."""
+    Ra = cytoplasmic_resistance * 1e4
+    Cm = membrane_capacitance * 1e-14
+    return Ra * np.asarray(length) / np.asarray(xarea), Cm * np.asarray(area)
+
+
+# ---------------------------------------------------------------------------------------------
+# Electric propagator and half step
+# ---------------------------------------------------------------------------------------------
+def coefficient_matrix(parent, R, C, time_step_ms):
+    """Sparse A*dt (dt in seconds) of dV/dt = A V.  electric.py:85-111.
+
+    Per edge (child c, parent p): A[p,c]=+dt/(r_c C_p), A[c,c]=-dt/(r_c C_c),
+    A[c,p]=+dt/(r_c C_c), A[p,p]=-dt/(r_c C_p); duplicates sum (COO -> CSC)."""
+    dt = time_step_ms * 1e-3
+    src, dst, coef = [], [], []
+    for child, p in enumerate(parent):
+        if p == NULL:
+            continue
+        r = R[child]
+        src += [child, child, p, p]
+        dst += [p, child, child, p]
+        coef += [+dt / (r * C[p]), -dt / (r * C[child]), +dt / (r * C[child]), -dt / (r * C[p])]
+    n = len(parent)
+    return scipy.sparse.csc_matrix((coef, (dst, src)), shape=(n, n), dtype=np.float64)
+
+
+def propagator(parent, R, C, time_step_ms, prune=EPSILON, method="dense"):
+    """P = expm(A*dt), entries with |P| < prune dropped.  electric.py:79-117.
+
+    ``method="sparse"`` calls scipy.sparse.linalg.expm exactly as the reference does
+    (electric.py:112); ``"dense"`` uses scipy.linalg.expm (same result to ~1e-14, faster for the
+    small trees the tests use).  Returns a scipy CSR matrix."""
+    A = coefficient_matrix(parent, R, C, time_step_ms)
+    if method == "sparse":
+        P = scipy.sparse.linalg.expm(A).tocsr()
+    else:
+        P = scipy.sparse.csr_matrix(scipy.linalg.expm(A.toarray()))
+    P.data[np.abs(P.data) < prune] = 0.0
+    P.eliminate_zeros()
+    return P
+
+
+def half_step(v, g, d, I, C, P, time_step_ms):
+    """One electric half step; ``time_step_ms`` is the half step (dt/2).  electric.py:57-76.
+
+    dv = dt_ms*I/C ; exponent = -dt_s*g/C ; alpha = exp(exponent);
+    v <- P (d - (d - v) alpha + dv); integral_v = dt_s d - exponent (d-v) alpha + 0.5 dt_s dv."""
+    dt = time_step_ms * 1e-3
+    dv_currents = time_step_ms * I / C
+    exponent = -dt * g / C
+    alpha = np.exp(exponent)
+    diff_v = d - v
+    v_new = P.dot(d - diff_v * alpha + dv_currents)
+    integral_v = dt * d - exponent * diff_v * alpha + (0.5 * dt * dv_currents)
+    return v_new, integral_v
+
+
+# ---------------------------------------------------------------------------------------------
+# Hodgkin-Huxley rates and the two gate integrators
+# ---------------------------------------------------------------------------------------------
+def vtrap(x, y):
+    """nmodl_library/artificial/hh.mod:88-94."""
+    x = np.asarray(x, dtype=np.float64)
+    small = np.abs(x / y) < 1e-6
+    with np.errstate(divide="ignore", invalid="ignore"):
+        big = x / (np.exp(x / y) - 1.0)
+    return np.where(small, y * (1.0 - x / y / 2.0), big)
+
+
+def hh_alpha_beta(v, celsius=6.3):
+    """Per-gate (alpha, beta) in /ms including q10.  hh.mod:62-86 (q10 at 67)."""
+    v = np.asarray(v, dtype=np.float64)
+    q10 = 3.0 ** ((celsius - 6.3) / 10.0)
+    am = 0.1 * vtrap(-(v + 40.0), 10.0)
+    bm = 4.0 * np.exp(-(v + 65.0) / 18.0)
+    ah = 0.07 * np.exp(-(v + 65.0) / 20.0)
+    bh = 1.0 / (np.exp(-(v + 35.0) / 10.0) + 1.0)
+    an = 0.01 * vtrap(-(v + 55.0), 10.0)
+    bn = 0.125 * np.exp(-(v + 65.0) / 80.0)
+    return {"m": (q10 * am, q10 * bm), "h": (q10 * ah, q10 * bh), "n": (q10 * an, q10 * bn)}
+
+
+def gate_inf(v, gate, celsius=6.3):
+    a, b = hh_alpha_beta(v, celsius)[gate]
+    return a / (a + b)
+
+
+def gate_exact(x, v, gate, dt, celsius=6.3):
+    """Exact linear-ODE update x' = (inf-x)/tau over dt.  solver.py:48-55 applied to hh.mod:55-60:
+    x(t+dt) = -c0/c1 + (c0 + c1 x) exp(c1 dt)/c1 with c1 = -1/tau, c0 = inf/tau."""
+    a, b = hh_alpha_beta(v, celsius)[gate]
+    s = a + b
+    tau = 1.0 / s
+    inf = a / s
+    c1 = -1.0 / tau
+    c0 = inf / tau
+    return -c0 / c1 + (c0 + c1 * x) * np.exp(c1 * dt) / c1
+
+
+# ---------------------------------------------------------------------------------------------
+# LTI kinetic tables and the table kernel
+# ---------------------------------------------------------------------------------------------
+def lti_matrix(derivative, inputs, num_states, time_step):
+    """Exact propagator of an LTI system at fixed inputs.  lti_sim/lti_model.py:51-65:
+    column j of A is derivative(inputs, e_j); P = expm(A dt); every column divided by its sum."""
+    A = np.empty((num_states, num_states))
+    for col in range(num_states):
+        state = [float(x == col) for x in range(num_states)]
+        A[:, col] = derivative(*inputs, *state)
+    M = scipy.linalg.expm(A * time_step)
+    for col in range(num_states):
+        M[:, col] *= 1.0 / sum(M[:, col].flat)
+    return M
+
+
+def lti_node_table(derivative, num_states, time_step, vmin=-100.0, vmax=100.0, num_buckets=2000):
+    """Propagator at every node of the voltage grid (num_buckets+1 nodes), shape [node][row][col]."""
+    nodes = vmin + np.arange(num_buckets + 1) * ((vmax - vmin) / num_buckets)
+    return np.stack([lti_matrix(derivative, [v], num_states, time_step) for v in nodes])
+
+
+def node_table_to_polynomial(node_table):
+    """Linear interpolation between grid nodes written in the reference's polynomial table layout
+    [bucket][col][row][term] with terms (1, frac): c0 = T[b], c1 = T[b+1] - T[b].
+    Layout: lti_sim/codegen.py:29-37 (transpose to column-major at 35)."""
+    T = np.asarray(node_table)
+    c0 = T[:-1]
+    c1 = T[1:] - T[:-1]
+    tbl = np.stack([c0, c1], axis=-1)          # [bucket][row][col][term]
+    return np.ascontiguousarray(tbl.transpose(0, 2, 1, 3))   # [bucket][col][row][term]
+
+
+def bucket_location(value, minimum, maximum, num_buckets):
+    """Reference kernel's bucket lookup for a linear input.  codegen.py:44-59:
+    input = (v - min) * nb/range; b = int(input) (truncates toward zero);
+    b > nb-1 -> (nb-1, 1.0); b < 0 -> (0, 0.0); else frac = input - b."""
+    bucket_frq = num_buckets / (maximum - minimum)
+    u = (value - minimum) * bucket_frq
+    b = int(u)
+    if b > num_buckets - 1:
+        return num_buckets - 1, 1.0
+    if b < 0:
+        return 0, 0.0
+    return b, u - b
+
+
+def lti_advance(states, v, poly_table, minimum=-100.0, maximum=100.0, conserve_sum=None):
+    """The generated per-instance LTI kernel.  codegen.py:39-106.
+
+    states: [S][M] array (SoA), v: [M] input voltages, poly_table: [bucket][col][row][term].
+    Returns the new states [S][M]."""
+    S = states.shape[0]
+    nb = poly_table.shape[0]
+    terms = poly_table.shape[3]
+    out = np.empty_like(states)
+    for inst in range(states.shape[1]):
+        b, frac = bucket_location(float(v[inst]), minimum, maximum, nb)
+        basis = [frac ** k for k in range(terms)]
+        scratch = np.zeros(S)
+        for col in range(S):
+            s = states[col, inst]
+            for row in range(S):
+                poly = 0.0
+                for k in range(terms):
+                    poly += basis[k] * poly_table[b, col, row, k]
+                scratch[row] += poly * s
+        if conserve_sum is not None:
+            scratch *= conserve_sum / scratch.sum()
+        out[:, inst] = scratch
+    return out
+
+
+def lti_advance_vec(states, v, poly_table, minimum=-100.0, maximum=100.0, conserve_sum=None):
+    """Vectorised form of ``lti_advance`` (same arithmetic per instance, summed in the same
+    column-then-row order); checked against the loop form in tests/test_oracle.py."""
+    S, M = states.shape
+    nb = poly_table.shape[0]
+    terms = poly_table.shape[3]
+    bucket_frq = nb / (maximum - minimum)
+    u = (np.asarray(v, dtype=np.float64) - minimum) * bucket_frq
+    b = np.trunc(u).astype(np.int64)
+    frac = u - b
+    hi = b > nb - 1
+    lo = b < 0
+    b = np.where(hi, nb - 1, np.where(lo, 0, b))
+    frac = np.where(hi, 1.0, np.where(lo, 0.0, frac))
+    basis = np.stack([frac ** k for k in range(terms)])          # [term][M]
+    scratch = np.zeros((S, M))
+    for col in range(S):
+        s = states[col]
+        for row in range(S):
+            poly = np.zeros(M)
+            for k in range(terms):
+                poly += basis[k] * poly_table[b, col, row, k]
+            scratch[row] += poly * s
+    if conserve_sum is not None:
+        scratch *= conserve_sum / scratch.sum(axis=0)
+    return scratch
+
+
+def two_state_derivative(rate_fn):
+    """Derivative of the scheme ``~ C <-> O (alpha, beta)`` with state order (C, O)."""
+    def deriv(v, C, O):
+        a, b = rate_fn(v)
+        return [-a * C + b * O, a * C - b * O]
+    return deriv
+
+
+def two_state_closed_form(alpha, beta, dt):
+    """Closed-form expm of [[-a, b], [a, -b]] dt (state order C, O) -> 2x2 matrix."""
+    s = alpha + beta
+    e = math.exp(-s * dt)
+    oinf = alpha / s
+    cinf = beta / s
+    return np.array([[cinf + oinf * e, cinf * (1 - e)],
+                     [oinf * (1 - e), oinf + cinf * e]])
+
+
+def hh_gate_rate(gate, celsius=6.3):
+    return lambda v: tuple(float(x) for x in hh_alpha_beta(v, celsius)[gate])
+
+
+# ---------------------------------------------------------------------------------------------
+# Accumulation
+# ---------------------------------------------------------------------------------------------
+def accumulate_scatter(num_segments, segment_of_instance, g_instance, reversal):
+    """Nonspecific conductance scatter: sum_g[seg] += g; d_num[seg] += g E.
+    species.py:256-260 (numpy fancy-index += keeps only one write per duplicate index; the
+    build uses true sums, as the numba pointer path in nmodl/__init__.py:396-406 does)."""
+    sum_g = np.zeros(num_segments)
+    d_num = np.zeros(num_segments)
+    np.add.at(sum_g, segment_of_instance, g_instance)
+    np.add.at(d_num, segment_of_instance, g_instance * reversal)
+    return sum_g, d_num
+
+
+def driving_voltage(sum_g, d_num):
+    """driving_voltage /= sum_conductance; NaN -> 0.  rxd_model.py:105-109."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.nan_to_num(d_num / sum_g)
+
+
+# ---------------------------------------------------------------------------------------------
+# Stimulus: counter-based Poisson pulses (device-generated in the build)
+# ---------------------------------------------------------------------------------------------
+def stim_hash(seed, cell, tick):
+    """32-bit hash of (seed, cell, tick); bit-exact restatement of the device generator."""
+    M = 0xFFFFFFFF
+    h = (np.uint64(seed) * np.uint64(0x9E3779B1)) & M
+    h ^= (np.uint64(cell) * np.uint64(0x85EBCA77)) & M
+    h ^= (np.uint64(tick) * np.uint64(0xC2B2AE3D)) & M
+    h = np.uint64(h)
+    h ^= h >> np.uint64(16)
+    h = (h * np.uint64(0x85EBCA6B)) & M
+    h ^= h >> np.uint64(13)
+    h = (h * np.uint64(0xC2B2AE35)) & M
+    h ^= h >> np.uint64(16)
+    return h
+
+
+def poisson_threshold(rate_hz, half_dt_ms):
+    return int(math.floor(rate_hz * half_dt_ms * 1e-3 * 4294967296.0))
+
+
+def poisson_pulses_active(seed, cell, tick, threshold, pulse_ticks):
+    """Number of pulses covering input tick ``tick``: onsets at ticks (tick-pulse_ticks, tick]."""
+    count = 0
+    for k in range(pulse_ticks):
+        t = tick - k
+        if t < 0:
+            break
+        if int(stim_hash(seed, cell, t)) < threshold:
+            count += 1
+    return count
+
+
+def pulse_ticks(duration_ms, half_dt_ms):
+    """Number of input ticks a constant_wave plays for.  time.py:271-273 with constant_wave 406-408."""
+    return math.ceil(duration_ms / half_dt_ms) - math.floor(0.0 / half_dt_ms)
+
+
+# ---------------------------------------------------------------------------------------------
+# Staggered advance for cells carrying HH channels
+# ---------------------------------------------------------------------------------------------
+class HHSim:
+    """Cells with leak + HH Na (m^3 h) + HH K (n^4) on every segment; the staggered step of
+    rxd_model.py:63-123.
+
+    gate_mode "exact"  : hh.mod DERIVATIVE block solved exactly (solver.py:48-55)
+    gate_mode "table"  : each gate a two-state kinetic scheme advanced by the LTI table kernel
+                         (codegen.py:39-106) with linear interpolation on a 0.1 mV grid.
+    g_bar: dict of per-segment Siemens (gbar * magnitude), E: reversal potentials (mV).
+    stim: callable(tick) -> per-segment current (A) for input tick ``tick`` (1-based, the value of
+          input_hook.ticks after its increment, time.py:114-117)."""
+
+    def __init__(self, P, C, g_bar, E=None, v0=-70.0, dt=0.1, celsius=6.3, gate_mode="exact",
+                 stim=None, ap_segments=(), ap_threshold=20.0, tables=None):
+        self.P, self.C = P, np.asarray(C, dtype=np.float64)
+        self.n = len(self.C)
+        self.g_bar = {k: np.asarray(v, dtype=np.float64) for k, v in g_bar.items()}
+        self.E = E or {"na": 60.0, "k": -88.0, "l": -54.3}
+        self.dt = float(dt)
+        self.half = 0.5 * self.dt
+        self.celsius = celsius
+        self.mode = gate_mode
+        self.v = np.full(self.n, float(v0))
+        self.gates = {g: np.full(self.n, float(gate_inf(v0, g, celsius))) for g in "mhn"}
+        self.sum_g = np.zeros(self.n)     # accumulators start at zero (data_components initial 0.0)
+        self.d_num = np.zeros(self.n)
+        self.stim = stim
+        self.input_ticks = 0
+        self.steps = 0
+        self.ap_segments = list(ap_segments)
+        self.ap_threshold = ap_threshold
+        self.ap_state = np.zeros(len(self.ap_segments), dtype=bool)
+        self.ap_detected = np.zeros(len(self.ap_segments), dtype=bool)
+        self.tables = tables
+        if gate_mode == "table" and tables is None:
+            self.tables = {}
+            for g in "mhn":
+                deriv = two_state_derivative(hh_gate_rate(g, celsius))
+                self.tables[g] = node_table_to_polynomial(lti_node_table(deriv, 2, self.dt))
+
+    def _half(self):
+        self.input_ticks += 1
+        I = np.zeros(self.n)
+        if self.stim is not None:
+            I = I + self.stim(self.input_ticks)
+        d = driving_voltage(self.sum_g, self.d_num)
+        self.v, _ = half_step(self.v, self.sum_g, d, I, self.C, self.P, self.half)
+
+    def _mechanisms(self):
+        v = self.v
+        for g in "mhn":
+            x = self.gates[g]
+            if self.mode == "exact":
+                self.gates[g] = gate_exact(x, v, g, self.dt, self.celsius)
+            else:
+                st = np.stack([1.0 - x, x])
+                self.gates[g] = lti_advance_vec(st, v, self.tables[g], conserve_sum=1.0)[1]
+        m, h, n_ = self.gates["m"], self.gates["h"], self.gates["n"]
+        gna = self.g_bar["na"] * m * m * m * h
+        gk = self.g_bar["k"] * n_ * n_ * n_ * n_
+        gl = self.g_bar["l"].copy()
+        self.sum_g = gna + gk + gl
+        self.d_num = gna * self.E["na"] + gk * self.E["k"] + gl * self.E["l"]
+
+    def advance(self):
+        self._half()
+        self._mechanisms()
+        self._half()
+        if self.ap_segments:
+            over = self.v[self.ap_segments] >= self.ap_threshold     # neuron.py:72-76
+            self.ap_detected = over & ~self.ap_state
+            self.ap_state = over
+        self.steps += 1
