@@ -133,7 +133,7 @@ def main():
     avg_launch_s = kern_ms / 1e3 / launches
     bytes_per_launch = b_step * N * (K / launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    stored_bytes_step = (st["nnz"] * 6.0 * 2) / N  # FP32 value + u16 column, two half steps
+    stored_bytes_step = (st["stored"] * 6.0 * 2) / N  # FP32 value + u16 column, two half steps
     out = {
         "metric": "compartment-steps/s @10M cmpts: FP32 batched vs FP64 plain; adjoint grads/s",
         "value": value,

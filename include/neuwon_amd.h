@@ -42,15 +42,16 @@ int nw_device_info(int device, char* name_out, int name_len, int* num_cus, int* 
  * R_host (Ohm) and C_host (F) are the per-segment axial resistance and capacitance.
  * For every cell the library builds A*dt exactly as electric.py:92-108, takes expm in FP64 on the
  * device, drops entries below `prune` (electric.py:114-115), and keeps per-cell CSR rows with FP32
- * values and 16-bit local column indices.  The diagonal is stored so that every FP32 row sums, in
- * FP64, to the FP64 row sum of the pruned matrix.
+ * values and 16-bit local column indices (rows padded with zeros to a multiple of 4 entries).  The
+ * diagonal is stored so that every FP32 row sums, in FP64, to the FP64 row sum of the pruned matrix.
  * ------------------------------------------------------------------------------------------ */
 typedef struct nw_propagator nw_propagator;
 
 typedef struct nw_propagator_stats {
     int64_t num_cells;
     int64_t num_rows;
-    int64_t nnz;             /* stored entries (FP32 value + u16 column each)                     */
+    int64_t nnz;             /* entries kept by the pruning                                       */
+    int64_t stored;          /* entries stored (FP32 value + u16 column), rows padded to 4        */
     int32_t max_cell_rows;
     int32_t max_row_nnz;
     double  build_seconds;
@@ -89,12 +90,19 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
  * 147-160, 252-260, and rxd_model.py:93-109): segmented reduction over instances sorted by
  * segment.  seg_inst_ptr[s]..seg_inst_ptr[s+1] are the instances on segment s (CSR offsets, so no
  * atomics and a deterministic order).  Every instance carries a conductance and a reversal
- * potential; nonspec_i (may be NULL) is the per-segment nonspecific current.
+ * potential; instance k's conductance is g_inst[g_index[k]] (g_index may be NULL: g_inst[k]), so
+ * sources written by different mechanisms into one buffer are read in segment order.
+ * nonspec_i (may be NULL) is the per-segment nonspecific current.  Sums are formed in FP64.
  *   sum_g[s] = sum g ; drive_v[s] = sum(g E) / sum_g (0 where sum_g == 0) ; sum_i[s] = nonspec_i
  * ------------------------------------------------------------------------------------------ */
 int nw_accumulate(int64_t num_segments, const int64_t* seg_inst_ptr, const float* g_inst,
-                  const float* e_inst, const float* nonspec_i, float* sum_g, float* drive_v,
-                  float* sum_i, void* stream);
+                  const int64_t* g_index, const float* e_inst, const float* nonspec_i,
+                  float* sum_g, float* drive_v, float* sum_i, void* stream);
+
+/* AP detector of every neuron (replaces Neuron._advance_AP_detector, neuron.py:72-76):
+ *   over = v[seg[i]] >= threshold; detected[i] = over && !state[i]; state[i] = over. */
+int nw_ap_detect(int64_t num, const int32_t* seg, const float* v, float threshold,
+                 uint8_t* state, uint8_t* detected, void* stream);
 
 /* ------------------------------------------------------------------------------------------
  * LTI kinetic advance (replaces the kernel generated by lti_sim/codegen.py:39-106)
@@ -112,6 +120,18 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
                    int64_t M, const float* voltage, const int32_t* seg_index, float* states,
                    int64_t state_stride, float conserve_sum, int32_t n_steps, const float* weights,
                    const float* gmax, float* cond_out, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * HH mechanism instances (replaces the advance generated for hh.mod, nmodl/__init__.py:408-432
+ * with the LTI or exact gate update, and its breakpoint, hh.mod:41-46): M instances, instance i on
+ * segment seg_index[i] (NULL: segment i).  gate_mode 0: two-state LTI tables (layout as nw_hh_step);
+ * 1: exact update with rate factor q10.  Per instance gbar*magnitude arrays g_na, g_k, g_l (S).
+ * g_out[3*i + {0,1,2}] receives the na, k and l conductances for nw_accumulate.
+ * ------------------------------------------------------------------------------------------ */
+int nw_hh_mechanism(int64_t M, int32_t gate_mode, float dt_ms, float q10, const float* gate_table,
+                    int32_t num_nodes, float vmin, float vmax, const float* voltage,
+                    const int32_t* seg_index, float* m, float* h, float* n, const float* g_na,
+                    const float* g_k, const float* g_l, float* g_out, void* stream);
 
 /* ------------------------------------------------------------------------------------------
  * Fused staggered step for cells carrying leak + HH Na (m^3 h) + HH K (n^4), every gate a two

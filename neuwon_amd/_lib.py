@@ -15,7 +15,7 @@ c_i32, c_i64, c_u32, c_f32, c_f64, c_vp = (ctypes.c_int32, ctypes.c_int64, ctype
 
 
 class NwPropagatorStats(ctypes.Structure):
-    _fields_ = [("num_cells", c_i64), ("num_rows", c_i64), ("nnz", c_i64),
+    _fields_ = [("num_cells", c_i64), ("num_rows", c_i64), ("nnz", c_i64), ("stored", c_i64),
                 ("max_cell_rows", c_i32), ("max_row_nnz", c_i32), ("build_seconds", c_f64),
                 ("device_bytes", c_i64)]
 
@@ -43,12 +43,15 @@ SIGNATURES = {
                                        ctypes.POINTER(c_i64), c_vp, c_vp, c_vp]),
     "nw_propagator_cell_export": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_vp]),
     "nw_electric_half_step": (c_i32, [c_vp, c_f32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
-    "nw_accumulate": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "nw_accumulate": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "nw_ap_detect": (c_i32, [c_i64, c_vp, c_vp, c_f32, c_vp, c_vp, c_vp]),
     "nw_lti_advance": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
                                c_f32, c_i32, c_vp, c_vp, c_vp, c_vp]),
     "nw_hh_step": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nw_stim_hash": (c_u32, [c_u32, c_u32, c_u32]),
+    "nw_hh_mechanism": (c_i32, [c_i64, c_i32, c_f32, c_f32, c_vp, c_i32, c_f32, c_f32, c_vp, c_vp,
+                                c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
 }
 
 _lib = None

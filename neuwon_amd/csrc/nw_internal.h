@@ -50,25 +50,82 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
-// v = P x for one CSR row held by one wavefront: x is stored in LDS as FP32 offsets from kVRef,
-// products of the FP32 values and offsets are accumulated in FP64, and the reference part is
-// restored with the exact FP64 sum of the stored row:  P x = kVRef * rowsum + P (x - kVRef).
-__device__ __forceinline__ double row_apply(const uint32_t* __restrict__ roff,
-                                            const float* __restrict__ vals,
-                                            const uint16_t* __restrict__ cols,
-                                            const float* __restrict__ xoff, double rowsum, int r,
-                                            int lane) {
-    const uint32_t beg = roff[r], end = roff[r + 1];
-    double acc0 = 0.0, acc1 = 0.0;
-    uint32_t k = beg + lane;
-    for (; k + kWave < end; k += 2 * kWave) {
-        const float a = vals[k], b = vals[k + kWave];
-        const uint16_t ca = cols[k], cb = cols[k + kWave];
-        acc0 = fma((double)a, (double)xoff[ca], acc0);
-        acc1 = fma((double)b, (double)xoff[cb], acc1);
+// Stored rows are padded to a multiple of kRowAlign entries (padding: value 0, column = the row),
+// so a lane can fetch kRowAlign consecutive values (16 B) and columns (8 B) with one vector load.
+constexpr int kRowAlign = 4;
+constexpr int kRowUnroll = 2;  // vector loads in flight per lane per round (512 entries per wave)
+
+struct RowChunk {
+    float4 a[kRowUnroll];
+    uint2 c[kRowUnroll];  // four u16 columns
+};
+
+__device__ __forceinline__ void row_load(RowChunk& ch, const float* __restrict__ vals,
+                                         const uint16_t* __restrict__ cols, uint32_t base,
+                                         uint32_t end, int lane) {
+#pragma unroll
+    for (int u = 0; u < kRowUnroll; ++u) {
+        const uint32_t k = base + (uint32_t)(u * kWave + lane) * kRowAlign;
+        if (k < end) {
+            ch.a[u] = *reinterpret_cast<const float4*>(vals + k);
+            ch.c[u] = *reinterpret_cast<const uint2*>(cols + k);
+        } else {
+            ch.a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            ch.c[u] = make_uint2(0u, 0u);
+        }
     }
-    if (k < end) acc0 = fma((double)vals[k], (double)xoff[cols[k]], acc0);
-    return fma(kVRef, rowsum, wave_sum(acc0 + acc1));
+}
+
+__device__ __forceinline__ double row_consume(const RowChunk& ch, const float* __restrict__ xoff,
+                                              double acc) {
+#pragma unroll
+    for (int u = 0; u < kRowUnroll; ++u) {
+        const uint32_t c01 = ch.c[u].x, c23 = ch.c[u].y;
+        acc = fma((double)ch.a[u].x, (double)xoff[c01 & 0xFFFFu], acc);
+        acc = fma((double)ch.a[u].y, (double)xoff[c01 >> 16], acc);
+        acc = fma((double)ch.a[u].z, (double)xoff[c23 & 0xFFFFu], acc);
+        acc = fma((double)ch.a[u].w, (double)xoff[c23 >> 16], acc);
+    }
+    return acc;
+}
+
+// out[r] = (P x)_r - kVRef for the rows r = first, first + stride, ... of one cell, one wavefront
+// per row.  x is stored in LDS as FP32 offsets from kVRef, products accumulate in FP64 and the
+// reference part is restored with the exact FP64 sum of the stored row:
+//     P x = kVRef * rowsum + P (x - kVRef).
+// The next row's offsets and first chunk are fetched before the current row is reduced.
+__device__ __forceinline__ void spmv_rows(const uint32_t* __restrict__ roff,
+                                          const float* __restrict__ vals,
+                                          const uint16_t* __restrict__ cols,
+                                          const double* __restrict__ rowsum,
+                                          const float* __restrict__ xoff, float* __restrict__ out,
+                                          int nrows, int first, int stride, int lane) {
+    int r = first;
+    if (r >= nrows) return;
+    uint32_t beg = roff[r], end = roff[r + 1];
+    RowChunk ch;
+    row_load(ch, vals, cols, beg, end, lane);
+    constexpr uint32_t kChunk = kWave * kRowAlign * kRowUnroll;
+    while (true) {
+        const int rn = r + stride;
+        uint32_t nbeg = 0, nend = 0;
+        if (rn < nrows) {
+            nbeg = roff[rn];
+            nend = roff[rn + 1];
+        }
+        double acc = row_consume(ch, xoff, 0.0);
+        for (uint32_t base = beg + kChunk; base < end; base += kChunk) {
+            row_load(ch, vals, cols, base, end, lane);
+            acc = row_consume(ch, xoff, acc);
+        }
+        if (rn < nrows) row_load(ch, vals, cols, nbeg, nend, lane);
+        const double s = fma(kVRef, rowsum[r], wave_sum(acc));
+        if (lane == 0) out[r] = (float)(s - kVRef);
+        if (rn >= nrows) break;
+        r = rn;
+        beg = nbeg;
+        end = nend;
+    }
 }
 
 // Location of `value` in a linear input table, reference semantics of the generated LTI kernel
@@ -108,7 +165,8 @@ struct nw_propagator {
     int device = 0;
     int64_t num_cells = 0;
     int64_t num_rows = 0;
-    int64_t nnz = 0;
+    int64_t nnz = 0;               // entries kept by the pruning
+    int64_t stored = 0;            // entries stored, rows padded to kRowAlign
     int32_t max_cell_rows = 0;
     int32_t max_row_nnz = 0;
     double build_seconds = 0.0;

@@ -111,8 +111,8 @@ __device__ __forceinline__ void hh_conductance(double gna_bar, double gk_bar, do
 }
 
 // Every per-segment quantity is computed in FP64 registers from FP32 storage; voltages live in LDS
-// as FP32 offsets from kVRef and the row products accumulate in FP64 (row_apply).
-template <int BLOCK>
+// as FP32 offsets from kVRef and the row products accumulate in FP64 (spmv_rows).
+template <int BLOCK, int GATE_MODE>
 __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
     extern __shared__ float lds[];
     float* xs = lds;            // x - kVRef of the current half step
@@ -168,17 +168,14 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         }
         __syncthreads();
         // ---- B: v = P x
-        for (int r = wave; r < nrows; r += NW) {
-            const double s = row_apply(roff, vals, cols, xs, rs[r], r, lane);
-            if (lane == 0) vs[r] = (float)(s - kVRef);
-        }
+        spmv_rows(roff, vals, cols, rs, xs, vs, nrows, wave, NW, lane);
         __syncthreads();
         // ---- C: mechanisms at v(t + dt/2), then relax for the second half step
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
             const double v = kVRef + (double)vs[i];
             double m = a.m[gi], h = a.h[gi], n = a.n[gi];
-            if (p.gate_mode == 1) {
+            if (GATE_MODE == 1) {
                 hh_exact(v, (double)p.q10, (double)p.dt_ms, m, h, n);
             } else {
                 int b;
@@ -202,10 +199,7 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         }
         __syncthreads();
         // ---- D: v = P x
-        for (int r = wave; r < nrows; r += NW) {
-            const double s = row_apply(roff, vals, cols, xs, rs[r], r, lane);
-            if (lane == 0) vs[r] = (float)(s - kVRef);
-        }
+        spmv_rows(roff, vals, cols, rs, xs, vs, nrows, wave, NW, lane);
         __syncthreads();
         if (threadIdx.x == 0 && ap_row >= 0) {
             const double vap = kVRef + (double)vs[ap_row];
@@ -220,6 +214,44 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         a.ap_state[cell] = ap_state;
         a.ap_count[cell] = ap_count;
     }
+}
+
+// ----------------------------------------------------------------------------------------------
+// HH mechanism instances (generic staggered path): thread per instance, voltage gathered through
+// the instance's segment; writes the three conductances interleaved (na, k, l).
+// ----------------------------------------------------------------------------------------------
+template <int GATE_MODE>
+__global__ void __launch_bounds__(256)
+k_hh_mechanism(int64_t M, double dt, double q10, const float* __restrict__ table, int num_nodes,
+               double vmin, double vmax, const float* __restrict__ voltage,
+               const int32_t* __restrict__ seg_index, float* m_, float* h_, float* n_,
+               const float* __restrict__ gna, const float* __restrict__ gk,
+               const float* __restrict__ gl, float* __restrict__ g_out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const double v = voltage[seg_index ? (int64_t)seg_index[i] : i];
+    double m = m_[i], h = h_[i], n = n_[i];
+    if (GATE_MODE == 1) {
+        hh_exact(v, q10, dt, m, h, n);
+    } else {
+        const int nb = num_nodes - 1;
+        const double frq = (double)nb / (vmax - vmin);
+        int b;
+        double f;
+        bucket_of(v, vmin, frq, nb, b, f);
+        const float2* tm = reinterpret_cast<const float2*>(table);
+        m = gate_advance(tm, b, f, m);
+        h = gate_advance(tm + num_nodes, b, f, h);
+        n = gate_advance(tm + 2 * num_nodes, b, f, n);
+    }
+    const float m32 = (float)m, h32 = (float)h, n32 = (float)n;
+    m_[i] = m32;
+    h_[i] = h32;
+    n_[i] = n32;
+    const double mm = m32, hh = h32, nn = n32;
+    g_out[3 * i + 0] = (float)((double)gna[i] * mm * mm * mm * hh);
+    g_out[3 * i + 1] = (float)((double)gk[i] * nn * nn * nn * nn);
+    g_out[3 * i + 2] = gl[i];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -282,20 +314,33 @@ k_lti(int num_nodes, float vmin, float vmax, const float* __restrict__ table, in
 // Segmented accumulation: thread per segment over its instances (sorted by segment).
 // ----------------------------------------------------------------------------------------------
 __global__ void k_accumulate(int64_t num_segments, const int64_t* __restrict__ ptr,
-                             const float* __restrict__ g_inst, const float* __restrict__ e_inst,
-                             const float* __restrict__ nonspec_i, float* __restrict__ sum_g,
-                             float* __restrict__ drive_v, float* __restrict__ sum_i) {
+                             const float* __restrict__ g_inst, const int64_t* __restrict__ g_index,
+                             const float* __restrict__ e_inst, const float* __restrict__ nonspec_i,
+                             float* __restrict__ sum_g, float* __restrict__ drive_v,
+                             float* __restrict__ sum_i) {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= num_segments) return;
-    float g = 0.0f, num = 0.0f;
+    double g = 0.0, num = 0.0;
     for (int64_t k = ptr[s]; k < ptr[s + 1]; ++k) {
-        const float gi = g_inst[k];
+        const double gi = g_inst[g_index ? g_index[k] : k];
         g += gi;
-        num = fmaf(gi, e_inst[k], num);
+        num = fma(gi, (double)e_inst[k], num);
     }
-    sum_g[s] = g;
-    drive_v[s] = (g != 0.0f) ? num / g : 0.0f;
+    sum_g[s] = (float)g;
+    drive_v[s] = (g != 0.0) ? (float)(num / g) : 0.0f;
     sum_i[s] = nonspec_i ? nonspec_i[s] : 0.0f;
+}
+
+// AP detector of every neuron (neuron.py:72-76): over = v[seg] >= threshold;
+// detected = over and not previously over; state = over.
+__global__ void k_ap_detect(int64_t num, const int32_t* __restrict__ seg,
+                            const float* __restrict__ v, float threshold, uint8_t* state,
+                            uint8_t* detected) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    const bool over = seg[i] >= 0 && v[seg[i]] >= threshold;
+    detected[i] = (over && !state[i]) ? 1 : 0;
+    state[i] = over ? 1 : 0;
 }
 
 }  // namespace nw
@@ -327,13 +372,37 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
     hipStream_t s = (hipStream_t)stream;
     const int bt = prm->block_threads ? prm->block_threads : 256;
     const unsigned grid = (unsigned)p->num_cells;
+    NW_REQUIRE(prm->gate_mode == 0 || prm->gate_mode == 1, "gate_mode must be 0 or 1");
+#define NW_HH_LAUNCH(B)                                                                   \
+    case B:                                                                               \
+        if (prm->gate_mode == 1) k_hh_step<B, 1><<<grid, B, lds, s>>>(a);                 \
+        else k_hh_step<B, 0><<<grid, B, lds, s>>>(a);                                     \
+        break;
     switch (bt) {
-        case 128: k_hh_step<128><<<grid, 128, lds, s>>>(a); break;
-        case 256: k_hh_step<256><<<grid, 256, lds, s>>>(a); break;
-        case 512: k_hh_step<512><<<grid, 512, lds, s>>>(a); break;
-        case 1024: k_hh_step<1024><<<grid, 1024, lds, s>>>(a); break;
+        NW_HH_LAUNCH(128) NW_HH_LAUNCH(256) NW_HH_LAUNCH(512) NW_HH_LAUNCH(1024)
         default: NW_REQUIRE(false, "block_threads must be 128, 256, 512 or 1024 (got %d)", bt);
     }
+#undef NW_HH_LAUNCH
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_hh_mechanism(int64_t M, int32_t gate_mode, float dt_ms, float q10, const float* gate_table,
+                    int32_t num_nodes, float vmin, float vmax, const float* voltage,
+                    const int32_t* seg_index, float* m, float* h, float* n, const float* g_na,
+                    const float* g_k, const float* g_l, float* g_out, void* stream) {
+    NW_REQUIRE(voltage && m && h && n && g_na && g_k && g_l && g_out, "nw_hh_mechanism: null argument");
+    NW_REQUIRE(gate_mode == 1 || (gate_table && num_nodes >= 2 && vmax > vmin), "bad gate table");
+    NW_REQUIRE(gate_mode == 0 || gate_mode == 1, "gate_mode must be 0 or 1");
+    if (M <= 0) return NW_OK;
+    const unsigned grid = (unsigned)((M + 255) / 256);
+    hipStream_t s = (hipStream_t)stream;
+    if (gate_mode == 1)
+        k_hh_mechanism<1><<<grid, 256, 0, s>>>(M, dt_ms, q10, gate_table, num_nodes, vmin, vmax,
+                                               voltage, seg_index, m, h, n, g_na, g_k, g_l, g_out);
+    else
+        k_hh_mechanism<0><<<grid, 256, 0, s>>>(M, dt_ms, q10, gate_table, num_nodes, vmin, vmax,
+                                               voltage, seg_index, m, h, n, g_na, g_k, g_l, g_out);
     NW_HIP(hipGetLastError());
     return NW_OK;
 }
@@ -367,14 +436,24 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
     return NW_OK;
 }
 
+int nw_ap_detect(int64_t num, const int32_t* seg, const float* v, float threshold,
+                 uint8_t* state, uint8_t* detected, void* stream) {
+    NW_REQUIRE(num >= 0 && (num == 0 || (seg && v && state && detected)), "nw_ap_detect: null argument");
+    if (num == 0) return NW_OK;
+    k_ap_detect<<<(unsigned)((num + 255) / 256), 256, 0, (hipStream_t)stream>>>(num, seg, v, threshold,
+                                                                              state, detected);
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
 int nw_accumulate(int64_t num_segments, const int64_t* seg_inst_ptr, const float* g_inst,
-                  const float* e_inst, const float* nonspec_i, float* sum_g, float* drive_v,
-                  float* sum_i, void* stream) {
+                  const int64_t* g_index, const float* e_inst, const float* nonspec_i, float* sum_g,
+                  float* drive_v, float* sum_i, void* stream) {
     NW_REQUIRE(num_segments >= 0 && seg_inst_ptr && sum_g && drive_v && sum_i,
                "nw_accumulate: null argument");
     if (num_segments == 0) return NW_OK;
     k_accumulate<<<(unsigned)((num_segments + 255) / 256), 256, 0, (hipStream_t)stream>>>(
-        num_segments, seg_inst_ptr, g_inst, e_inst, nonspec_i, sum_g, drive_v, sum_i);
+        num_segments, seg_inst_ptr, g_inst, g_index, e_inst, nonspec_i, sum_g, drive_v, sum_i);
     NW_HIP(hipGetLastError());
     return NW_OK;
 }

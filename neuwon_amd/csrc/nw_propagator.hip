@@ -138,6 +138,13 @@ __global__ void k_fill(const double* P, int n, double prune, const uint32_t* row
         }
         pos += (uint32_t)__popcll(mask);
     }
+    if (kF32) {  // padding up to the next row: value 0, column = the row itself
+        const uint32_t stop = row_off_local[row + 1];
+        for (uint32_t q = pos + lane; q < stop; q += kWave) {
+            vals32[q] = 0.0f;
+            cols16[q] = (uint16_t)row;
+        }
+    }
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -261,9 +268,9 @@ static int expm_cell(Workspace& w, int n, const int32_t* parent, const double* R
     return NW_OK;
 }
 
-// Row counts -> local CSR offsets on the host; returns entries of the cell.
-static int count_rows(Workspace& w, const double* P, int n, double prune,
-                      std::vector<uint32_t>& off, int32_t& max_row) {
+// Row counts -> local CSR offsets on the host (each row rounded up to `align` entries).
+static int count_rows(Workspace& w, const double* P, int n, double prune, int align,
+                      std::vector<uint32_t>& off, int32_t& max_row, int64_t& kept) {
     const int rows_per_block = 4;
     k_count<<<(n + rows_per_block - 1) / rows_per_block, rows_per_block * kWave, 0, w.stream>>>(
         P, n, prune, w.d_counts);
@@ -274,10 +281,12 @@ static int count_rows(Workspace& w, const double* P, int n, double prune,
     NW_HIP(hipStreamSynchronize(w.stream));
     off.assign(n + 1, 0);
     max_row = 0;
+    kept = 0;
     uint64_t acc = 0;
     for (int i = 0; i < n; ++i) {
         off[i] = (uint32_t)acc;
-        acc += (uint64_t)counts[i];
+        acc += (uint64_t)((counts[i] + align - 1) / align * align);
+        kept += counts[i];
         max_row = std::max(max_row, counts[i]);
     }
     NW_REQUIRE(acc < 0xFFFFFFFFull, "cell block too large for 32-bit row offsets");
@@ -296,9 +305,9 @@ static int grow(nw_propagator* p, int64_t need) {
         set_error("out of device memory growing the propagator to %lld entries", (long long)cap);
         return NW_ERR_ALLOC;
     }
-    if (p->nnz > 0) {
-        NW_HIP(hipMemcpy(nv, p->vals, p->nnz * sizeof(float), hipMemcpyDeviceToDevice));
-        NW_HIP(hipMemcpy(nc, p->cols, p->nnz * sizeof(uint16_t), hipMemcpyDeviceToDevice));
+    if (p->stored > 0) {
+        NW_HIP(hipMemcpy(nv, p->vals, p->stored * sizeof(float), hipMemcpyDeviceToDevice));
+        NW_HIP(hipMemcpy(nc, p->cols, p->stored * sizeof(uint16_t), hipMemcpyDeviceToDevice));
     }
     if (p->vals) (void)hipFree(p->vals);
     if (p->cols) (void)hipFree(p->cols);
@@ -361,20 +370,22 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
         double* P = nullptr;
         if ((rc = expm_cell(w, n, parent + r0, R + r0, C + r0, dt_ms, &P))) return rc;
         int32_t max_row = 0;
-        if ((rc = count_rows(w, P, n, prune, off, max_row))) return rc;
+        int64_t kept = 0;
+        if ((rc = count_rows(w, P, n, prune, kRowAlign, off, max_row, kept))) return rc;
         p->max_row_nnz = std::max(p->max_row_nnz, max_row);
         int64_t cnt = off[n];
-        if ((rc = grow(p, p->nnz + cnt))) return rc;
+        if ((rc = grow(p, p->stored + cnt))) return rc;
         uint32_t* dev_off = p->row_off + r0 + c;
         NW_HIP(hipMemcpyAsync(dev_off, off.data(), (n + 1) * sizeof(uint32_t),
                               hipMemcpyHostToDevice, w.stream));
-        k_fill<true><<<(n + 3) / 4, 4 * kWave, 0, w.stream>>>(P, n, prune, dev_off,
-                                                              p->vals + p->nnz, p->cols + p->nnz,
-                                                              p->rowsum + r0, nullptr, nullptr);
+        k_fill<true><<<(n + 3) / 4, 4 * kWave, 0, w.stream>>>(
+            P, n, prune, dev_off, p->vals + p->stored, p->cols + p->stored, p->rowsum + r0,
+            nullptr, nullptr);
         NW_HIP(hipGetLastError());
-        p->h_cell_nz0[c] = p->nnz;
+        p->h_cell_nz0[c] = p->stored;
         cell_nnz[c] = cnt;
-        p->nnz += cnt;
+        p->stored += cnt;
+        p->nnz += kept;
     }
     NW_HIP(hipStreamSynchronize(w.stream));
     NW_HIP(hipMalloc(&p->cell_row0, sizeof(int64_t) * (num_cells + 1)));
@@ -406,6 +417,7 @@ struct HalfArgs {
     const uint16_t* cols;
     const double* rowsum;
     const int32_t* order;
+    int32_t nmax;
     float half_dt_ms;
     float* v;
     const float* sum_g;
@@ -416,7 +428,7 @@ struct HalfArgs {
 };
 
 // FP32 storage, FP64 arithmetic: x is formed in FP64 and staged in LDS as an FP32 offset from
-// kVRef; the row products accumulate in FP64 (row_apply).
+// kVRef; the row products accumulate in FP64 (spmv_rows).
 __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
     extern __shared__ float xs[];
     const int cell = a.order[blockIdx.x];
@@ -444,11 +456,10 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
     const uint32_t* roff = a.row_off + row0 + cell;
     const float* vals = a.vals + a.cell_nz0[cell];
     const uint16_t* cols = a.cols + a.cell_nz0[cell];
-    const double* rs = a.rowsum + row0;
-    for (int r = wave; r < n; r += nwaves) {
-        const double s = row_apply(roff, vals, cols, xs, rs[r], r, lane);
-        if (lane == 0) a.v[row0 + r] = (float)s;
-    }
+    float* vout = xs + a.nmax;
+    spmv_rows(roff, vals, cols, a.rowsum + row0, xs, vout, n, wave, nwaves, lane);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a.v[row0 + i] = (float)(kVRef + (double)vout[i]);
 }
 
 }  // namespace nw
@@ -499,6 +510,7 @@ int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
     s->num_cells = p->num_cells;
     s->num_rows = p->num_rows;
     s->nnz = p->nnz;
+    s->stored = p->stored;
     s->max_cell_rows = p->max_cell_rows;
     s->max_row_nnz = p->max_row_nnz;
     s->build_seconds = p->build_seconds;
@@ -528,7 +540,8 @@ int nw_propagator_cell_f64(int device, int32_t n, const int32_t* parent_local_ho
     if ((rc = expm_cell(w, n, parent_local_host, R_host, C_host, time_step_ms, &P))) return rc;
     std::vector<uint32_t> off;
     int32_t max_row;
-    if ((rc = count_rows(w, P, n, prune, off, max_row))) return rc;
+    int64_t kept;
+    if ((rc = count_rows(w, P, n, prune, 1, off, max_row, kept))) return rc;
     *nnz_out = off[n];
     if (!val_host) return NW_OK;
     NW_REQUIRE(row_ptr_host && col_host, "null output array");
@@ -579,8 +592,8 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
                           float* integral_v, void* stream) {
     NW_REQUIRE(p && v && sum_g && drive_v && sum_i && inv_c, "nw_electric_half_step: null argument");
     HalfArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
-               half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v};
-    size_t lds = sizeof(float) * (size_t)p->max_cell_rows;
+               p->max_cell_rows, half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v};
+    size_t lds = 2 * sizeof(float) * (size_t)p->max_cell_rows;
     NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
     k_half_step<<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
     NW_HIP(hipGetLastError());

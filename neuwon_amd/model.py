@@ -1,0 +1,1185 @@
+"""Drop-in ``RxD_Model`` / ``Model`` whose ``advance()`` runs on the MI355X.
+
+Mirrors the public surface of neuwon/rxd/rxd_model.py:8-123 and neuwon/model.py:6-22:
+
+    m = RxD_Model(time_step=0.1, temperature=6.3, mechanisms=['hh'])
+    root = m.Neuron([0, 0, 0], 10).root
+    tip = root.add_segment([20, 0, 0], 1)
+    tip.insert('hh')
+    root.inject_current(0.5, 1.0)
+    m.advance()                          # one staggered step on the GPU
+    m.database.get_data('Segment.voltage')
+
+Storage follows the reference's structure-of-arrays Database (database.py:11-259): every class
+(Segment, Neuron, each mechanism) keeps one array per attribute.  While a model is being built the
+arrays live on the host; the first ``advance()`` sorts segments by cell (the reference's
+``sort_key='neuron'``, neuron.py:12), builds the per-cell propagators on the device and moves the
+state to device memory.  ``get_data`` in the default "host" memory space returns a host copy whose
+writes are uploaded before the next step (the reference returns a live view); inside
+``database.using_memory_space('cuda')`` (alias 'hip') it returns the live device tensor.
+
+Each half step runs nw_accumulate + nw_electric_half_step, the mechanisms run their own kernels
+(nw_hh_mechanism, nw_lti_advance) or, for Python plugin mechanisms, their ``advance()``; there is
+no CPU execution path.
+"""
+import contextlib
+import math
+import os
+
+import numpy as np
+
+from . import _lib, channels, geometry
+
+EPSILON = float(np.finfo(np.float64).eps)
+NULL = -1
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("neuwon_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+# ================================================================================================
+# Clock and TimeSeries (neuwon/database/time.py)
+# ================================================================================================
+class Clock:
+    """Tick counter with callbacks; time.py:54-121."""
+
+    def __init__(self, tick_period, units=""):
+        self.dt = self.time_step = self.tick_period = float(tick_period)
+        self.ticks = 0
+        self.units = str(units)
+        self._callbacks = []
+
+    def get_time(self):
+        return self.ticks * self.dt
+
+    __call__ = get_time
+
+    def get_ticks(self):
+        return self.ticks
+
+    def get_time_step(self):
+        return self.dt
+
+    get_tick_period = get_time_step
+
+    def get_units(self):
+        return self.units
+
+    def register_callback(self, function, period=1, weakref=False):
+        """Callbacks run after every tick, in registration order, and are removed when they
+        return True (time.py:90-103)."""
+        if period != 1:
+            raise NotImplementedError("period != 1")
+        self._callbacks.append(function)
+
+    def _run_callbacks(self):
+        current, self._callbacks = self._callbacks, []
+        keep = [fn for fn in current if not fn()]
+        self._callbacks = keep + self._callbacks   # plus any registered while running
+
+    def reset(self):
+        self.ticks = 0
+        self._run_callbacks()
+
+    def set_time(self, new_time):
+        self.ticks = round(float(new_time) / self.dt)
+        self._run_callbacks()
+
+    def tick(self):
+        self.ticks += 1
+        self._run_callbacks()
+
+    def is_now(self, time):
+        return round(float(time) / self.dt) == self.ticks
+
+
+class TimeSeries:
+    """Buffer of (timestamp, value) samples with play back and recording (time.py:123-408 subset)."""
+
+    def __init__(self, initial_data=None, timestamps=None):
+        self.data = list(initial_data) if initial_data is not None else []
+        self.timestamps = list(timestamps) if timestamps is not None else []
+        self._playing = None
+        self._recording = False
+
+    def get_data(self):
+        return list(self.data)
+
+    def get_timestamps(self):
+        return list(self.timestamps)
+
+    def __len__(self):
+        return len(self.data)
+
+    def set_data(self, data_samples, timestamps):
+        self.data = list(data_samples)
+        self.timestamps = list(timestamps)
+        return self
+
+    def constant_wave(self, value, duration):
+        """time.py:406-408."""
+        return self.set_data([value, value], [0, duration])
+
+    def is_stopped(self):
+        return self._playing is None and not self._recording
+
+    def stop(self):
+        self._playing = None
+        self._recording = False
+
+    def play(self, db_object, component, mode="+=", loop=False, clock=None, immediate=True):
+        """Write one value per clock tick into ``db_object.component`` (time.py:245-302)."""
+        assert self._playing is None
+        clock = clock or db_object._model.clock
+        dt = clock.get_tick_period()
+        start = math.floor(self.timestamps[0] / dt)
+        end = math.ceil(self.timestamps[-1] / dt)
+        ts = dt * np.arange(start, end)
+        values = np.interp(ts, self.timestamps, self.data, left=self.data[0], right=self.data[-1])
+        self._playing = dict(obj=db_object, component=component, data=values, index=0,
+                             loop=bool(loop), mode=mode)
+        clock.register_callback(self._play_tick)
+        if immediate:
+            self._play_tick()
+        return self
+
+    def _play_tick(self):
+        st = self._playing
+        if st is None:
+            return True
+        value = float(st["data"][st["index"]])
+        obj, name = st["obj"], st["component"]
+        if st["mode"] == "=":
+            setattr(obj, name, value)
+        elif st["mode"] == "+=":
+            setattr(obj, name, value + getattr(obj, name))
+        else:
+            raise NotImplementedError(st["mode"])
+        st["index"] += 1
+        if st["index"] >= len(st["data"]):
+            if st["loop"]:
+                st["index"] = 0
+            else:
+                self._playing = None
+                return True
+        return False
+
+    def record(self, db_object, component, record_duration=np.inf, discard_after=np.inf,
+               clock=None, immediate=True):
+        """Sample ``db_object.component`` after every clock tick (time.py:189-232)."""
+        self._clock = clock or db_object._model.clock
+        self._rec = (db_object, component)
+        self._recording = True
+        self._remaining = float(record_duration)
+        self._discard_after = float(discard_after)
+        self._clock.register_callback(self._record_tick)
+        if immediate:
+            self._record_tick()
+        return self
+
+    def _record_tick(self):
+        if not self._recording or self._remaining <= 0:
+            self._recording = False
+            return True
+        obj, name = self._rec
+        self.data.append(float(getattr(obj, name)))
+        self.timestamps.append(self._clock())
+        while self.timestamps[-1] - self.timestamps[0] > self._discard_after:
+            self.data.pop(0)
+            self.timestamps.pop(0)
+        self._remaining -= self._clock.dt
+        return False
+
+
+# ================================================================================================
+# Structure-of-arrays storage
+# ================================================================================================
+class Column:
+    """One attribute of one class: host list while building, device tensor once the model runs."""
+
+    def __init__(self, db_class, name, initial_value=None, dtype=np.float64, units=None,
+                 valid_range=None, device_dtype="float32", shape=(), device=True, on_write=None):
+        self.db_class, self.name = db_class, name
+        self.initial_value, self.dtype, self.units = initial_value, np.dtype(dtype), units
+        self.valid_range, self.device_dtype, self.shape = valid_range, device_dtype, tuple(shape)
+        self.device = bool(device)   # False: host-only (references between objects, geometry input)
+        self.on_write = on_write     # called after a host-only column changes
+        self.host = []
+        self.dev = None          # torch tensor, rows in device (sorted) order
+        self.checked_out = None  # host copy handed out by get_data('host'), uploaded on next use
+
+    def get_name(self):
+        return self.name
+
+    def get_units(self):
+        return self.units
+
+    def get_initial_value(self):
+        return self.initial_value
+
+    def append(self, value=None):
+        if value is None:
+            value = self.initial_value
+        self.host.append(value)
+
+    # ---- element access (rows in device order once finalized)
+    def get_item(self, row):
+        self._flush()
+        if self.dev is not None:
+            return self.dev[row].item()
+        return self.host[row]
+
+    def set_item(self, row, value):
+        self._flush()
+        if self.dev is not None:
+            self.dev[row] = value
+        else:
+            self.host[row] = value
+            if self.on_write is not None:
+                self.on_write()
+
+    # ---- bulk access
+    def host_array(self):
+        self._flush()
+        if self.dev is not None:
+            return self.dev.cpu().numpy()
+        return np.asarray(self.host, dtype=self.dtype)
+
+    def get_data(self, space):
+        """Host space: a copy whose changes are written back at the next use of the column.
+        Device space: the live device tensor."""
+        self._flush()
+        if space == "host":
+            if self.dev is not None:
+                arr = self.dev.cpu().numpy().copy()
+            else:
+                arr = np.array(self.host, dtype=self.dtype)
+            self.checked_out = arr
+            return arr
+        if self.dev is None:
+            raise RuntimeError(f"{self.db_class.name}.{self.name} is not on the device yet "
+                               "(call advance() first)")
+        return self.dev
+
+    def set_data(self, value):
+        value = np.asarray(value)
+        if self.dev is not None:
+            torch = _torch()
+            self.dev.copy_(torch.as_tensor(value.astype(self.dev.cpu().numpy().dtype), device=self.dev.device))
+        else:
+            assert len(value) == len(self.host)
+            self.host = list(value)
+
+    def _flush(self):
+        arr, self.checked_out = self.checked_out, None
+        if arr is None:
+            return
+        if self.dev is not None:
+            torch = _torch()
+            src = torch.as_tensor(np.ascontiguousarray(arr), device=self.dev.device)
+            self.dev.copy_(src.to(self.dev.dtype))
+        else:
+            self.host = list(arr)
+
+    def to_device(self, order, device):
+        if not self.device:
+            return
+        torch = _torch()
+        arr = np.asarray(self.host, dtype=self.dtype)[order] if len(self.host) else \
+            np.zeros((0,) + self.shape, dtype=self.dtype)
+        self.dev = torch.as_tensor(np.ascontiguousarray(arr.astype(self.device_dtype)), device=device)
+
+    def to_host(self, inverse_order):
+        if self.dev is not None:
+            arr = self.dev.cpu().numpy()
+            self.host = list(arr[inverse_order].astype(self.dtype))
+            self.dev = None
+            self.checked_out = None
+
+    def check(self):
+        arr = self.host_array().astype(np.float64)
+        if arr.size and np.isnan(arr).any():
+            raise AssertionError(f"{self.db_class.name}.{self.name} contains NaN")
+        if self.valid_range is not None and arr.size:
+            lo, hi = self.valid_range
+            if lo is not None and np.any(arr < lo):
+                raise AssertionError(f"{self.db_class.name}.{self.name} is less than {lo}")
+            if hi is not None and np.any(arr > hi):
+                raise AssertionError(f"{self.db_class.name}.{self.name} is greater than {hi}")
+
+
+class DBClass:
+    """A class of objects with structure-of-arrays attributes (database.py:261-655 subset)."""
+
+    def __init__(self, database, name):
+        self.database, self.name = database, name
+        self.columns = {}
+        self.class_attributes = {}
+        self.size = 0
+        self.row_of = np.zeros(0, dtype=np.int64)  # creation id -> row
+        database.classes[name] = self
+
+    def get_name(self):
+        return self.name
+
+    def __len__(self):
+        return self.size
+
+    def add_attribute(self, name, initial_value=None, **kw):
+        col = Column(self, name, initial_value, **kw)
+        for _ in range(self.size):
+            col.append()
+        self.columns[name] = col
+        return col
+
+    def add_class_attribute(self, name, initial_value, units=None, valid_range=None):
+        self.class_attributes[name] = initial_value
+        return initial_value
+
+    def get(self, name):
+        return self.columns[name]
+
+    def get_data(self, name):
+        return self.columns[name].get_data(self.database.memory_space)
+
+    def set_data(self, name, value):
+        self.columns[name].set_data(value)
+
+    def get_all_components(self):
+        return list(self.columns.values())
+
+    def new_row(self):
+        for col in self.columns.values():
+            col.append()
+        cid = self.size
+        self.size += 1
+        self.row_of = np.append(self.row_of, cid)
+        return cid
+
+
+class Database:
+    """get_data / set_data / using_memory_space / check (database.py:11-259 subset)."""
+
+    def __init__(self):
+        self.classes = {}
+        self.memory_space = "host"
+        self.clock = None
+
+    def get(self, name):
+        cls, _, attr = str(name).partition(".")
+        c = self.classes[cls]
+        return c.get(attr) if attr else c
+
+    def get_class(self, name):
+        return self.classes[str(name).partition(".")[0]]
+
+    def get_data(self, name):
+        cls, _, attr = str(name).partition(".")
+        if self.memory_space == "device" and getattr(self, "_model", None) is not None:
+            self._model._ensure_built()
+        return self.classes[cls].get_data(attr)
+
+    def set_data(self, name, value):
+        cls, _, attr = str(name).partition(".")
+        self.classes[cls].set_data(attr, value)
+
+    def get_memory_space(self):
+        return self.memory_space
+
+    @contextlib.contextmanager
+    def using_memory_space(self, memory_space):
+        space = {"cuda": "device", "hip": "device", "device": "device", "host": "host"}[memory_space]
+        prior = self.memory_space
+        self.memory_space = space
+        try:
+            yield
+        finally:
+            self.memory_space = prior
+
+    def add_clock(self, clock):
+        assert self.clock is None
+        self.clock = clock
+        return clock
+
+    def get_clock(self):
+        return self.clock
+
+    def check(self, name=None):
+        errors = []
+        classes = [self.get_class(name)] if name else list(self.classes.values())
+        for c in classes:
+            for col in c.columns.values():
+                try:
+                    col.check()
+                except AssertionError as e:
+                    errors.append(str(e))
+        if errors:
+            raise AssertionError(",\n".join(sorted(errors)) + ".")
+
+
+class DBObject:
+    """Handle to one row of a DBClass; attribute access reads/writes its columns."""
+    __slots__ = ("_cid",)
+    _db_class = None
+
+    def __getattr__(self, name):
+        cls = type(self)._db_class
+        col = cls.columns.get(name)
+        if col is None:
+            if name in cls.class_attributes:
+                return cls.class_attributes[name]
+            raise AttributeError(name)
+        cid = object.__getattribute__(self, "_cid")
+        return col.get_item(int(cls.row_of[cid]) if col.dev is not None else cid)
+
+    def __setattr__(self, name, value):
+        if name in DBObject.__slots__:
+            object.__setattr__(self, name, value)
+            return
+        cls = type(self)._db_class
+        col = cls.columns.get(name)
+        if col is not None:
+            col.set_item(int(cls.row_of[self._cid]) if col.dev is not None else self._cid, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def get_unstable_index(self):
+        return int(type(self)._db_class.row_of[self._cid])
+
+    def __eq__(self, other):
+        return type(self) is type(other) and self._cid == other._cid
+
+    def __hash__(self):
+        return hash((type(self).__name__, self._cid))
+
+
+# ================================================================================================
+# Mechanisms (neuwon/rxd/mechanisms.py:4-54)
+# ================================================================================================
+class Mechanism:
+    """Plugin ABI of the reference (mechanisms.py:4-54)."""
+    omnipresent = False
+
+    @classmethod
+    def initialize(cls, model):
+        pass
+
+    @classmethod
+    def get_name(cls):
+        raise NotImplementedError(cls)
+
+    @classmethod
+    def other_mechanisms(cls):
+        return []
+
+    def __init__(self, segment, outside, magnitude, *other_mechanisms):
+        raise NotImplementedError(type(self))
+
+    @classmethod
+    def advance(cls):
+        raise NotImplementedError(cls)
+
+
+class _DeviceMechanism(DBObject):
+    """Built-in mechanism instance: one row of its class; instance i sits on segment[i]."""
+    __slots__ = ()
+    _model = None
+    omnipresent = False
+
+    @classmethod
+    def get_name(cls):
+        return cls._name
+
+    @classmethod
+    def other_mechanisms(cls):
+        return []
+
+    @classmethod
+    def get_database_class(cls):
+        return cls._db_class
+
+
+def _make_hh_mechanism(model, name, gate_mode, gbar=None, reversal=None):
+    """hh.mod (nmodl_library/artificial/hh.mod): leak + Na (m^3 h) + K (n^4), the ions written as
+    nonspecific conductances (USEION ... WRITE gna, ena; nmodl/__init__.py:168-200).  gate_mode
+    "exact" is the DERIVATIVE block solved exactly (solver.py:48-55); "table" the same gates as
+    two-state kinetic schemes advanced by LTI tables (lti_sim/codegen.py:39-106)."""
+    db = model.database
+    cls_db = DBClass(db, name)
+    gbar = dict(channels.HH_GBAR, **(gbar or {}))
+    E = dict(channels.HH_REVERSAL, **(reversal or {}))
+    ss = channels.hh_steady_state(model.initial_voltage, model.temperature)
+    cls_db.add_attribute("segment", NULL, dtype=np.int64, device=False,
+                         on_write=model._invalidate_mechanisms)
+    cls_db.add_attribute("magnitude", 1.0, valid_range=(0.0, None), device=False,
+                         on_write=model._invalidate_mechanisms)
+    for g in "mhn":
+        cls_db.add_attribute(g, ss[g])
+    for ion in ("na", "k", "l"):
+        cls_db.add_class_attribute(f"{ion}_reversal_potential", E[ion])
+
+    def __init__(self, segment, outside=None, magnitude=1.0, *others):
+        cid = type(self)._db_class.new_row()
+        object.__setattr__(self, "_cid", cid)
+        model._invalidate_mechanisms()
+        self.segment = segment._cid
+        self.magnitude = float(magnitude) * segment.surface_area   # nmodl/__init__.py:499
+
+    mech = type(name, (_DeviceMechanism,), {"__slots__": (), "__init__": __init__, "_name": name,
+                                           "_db_class": cls_db, "_model": model,
+                                           "gate_mode": gate_mode, "gbar": gbar, "E": E})
+
+    def advance(cls):
+        model._run_hh_mechanism(cls)
+    mech.advance = classmethod(advance)
+    mech.conductance_sources = ("na", "k", "l")
+    return mech
+
+
+class LTIMechanism:
+    """Factory of a kinetic mechanism from an ``channels.LTIScheme`` whose conductance is
+    gmax * magnitude * sum_s weights[s] state_s with reversal potential ``reversal`` (mV); gmax is a
+    density in S/cm^2 (scaled by the segment's surface area, nmodl/__init__.py:97-107, 499)."""
+
+    def __init__(self, scheme, gmax_s_per_cm2, reversal, name=None):
+        self.scheme = scheme
+        self.gmax = float(gmax_s_per_cm2) * channels.CM2_TO_UM2
+        self.reversal = float(reversal)
+        self.name = name or scheme.name
+
+    def initialize(self, model):
+        db = model.database
+        cls_db = DBClass(db, self.name)
+        cls_db.add_attribute("segment", NULL, dtype=np.int64, device=False,
+                             on_write=model._invalidate_mechanisms)
+        cls_db.add_attribute("magnitude", 1.0, valid_range=(0.0, None), device=False,
+                             on_write=model._invalidate_mechanisms)
+        init = self.scheme.initial_state(model.initial_voltage)
+        for s, x0 in zip(self.scheme.state_names, init):
+            cls_db.add_attribute(s, float(x0))
+        spec = self
+
+        def __init__(self, segment, outside=None, magnitude=1.0, *others):
+            cid = type(self)._db_class.new_row()
+            object.__setattr__(self, "_cid", cid)
+            model._invalidate_mechanisms()
+            self.segment = segment._cid
+            self.magnitude = float(magnitude) * segment.surface_area
+
+        mech = type(self.name, (_DeviceMechanism,), {"__slots__": (), "__init__": __init__,
+                                                    "_name": self.name, "_db_class": cls_db,
+                                                    "_model": model, "spec": spec})
+        mech.advance = classmethod(lambda cls: model._run_lti_mechanism(cls))
+        mech.conductance_sources = ("g",)
+        return mech
+
+
+_BUILTIN = {
+    "hh": lambda model: _make_hh_mechanism(model, "hh", "exact"),
+    "hh_kinetic": lambda model: _make_hh_mechanism(model, "hh_kinetic", "table"),
+}
+
+
+# ================================================================================================
+# Neuron and Segment (neuwon/rxd/neuron/{neuron,segment,geometry,electric}.py)
+# ================================================================================================
+class _Segment(DBObject):
+    __slots__ = ()
+    _model = None
+
+    def __init__(self, parent, coordinates, diameter, *, segment_type=None):
+        model = type(self)._model
+        model._invalidate_structure()
+        cid = type(self)._db_class.new_row()
+        object.__setattr__(self, "_cid", cid)
+        self.parent = NULL if parent is None else parent._cid
+        self.coordinates = tuple(float(x) for x in coordinates)
+        self.diameter = float(diameter)
+        self.neuron = NULL if parent is None else parent.neuron
+        model._children.append([])
+        if parent is not None:
+            model._children[parent._cid].append(cid)
+
+    @classmethod
+    def get_database_class(cls):
+        return cls._db_class
+
+    def is_root(self):
+        return self.parent == NULL
+
+    is_sphere = is_root
+
+    def is_cylinder(self):
+        return not self.is_root()
+
+    @property
+    def children(self):
+        return [type(self)._from_cid(c) for c in type(self)._model._children[self._cid]]
+
+    @classmethod
+    def _from_cid(cls, cid):
+        obj = object.__new__(cls)
+        object.__setattr__(obj, "_cid", int(cid))
+        return obj
+
+    def get_parent(self):
+        return None if self.parent == NULL else type(self)._from_cid(self.parent)
+
+    def add_segment(self, coordinates, diameter, segment_type=None):
+        return type(self)(self, coordinates, diameter, segment_type=segment_type)
+
+    def add_section(self, coordinates, diameter, maximum_segment_length=np.inf, segment_type=None):
+        """segment.py:75-104."""
+        maximum_segment_length = float(maximum_segment_length)
+        assert maximum_segment_length > 0
+        coords = [float(x) for x in coordinates]
+        diameter = float(diameter)
+        displace = np.subtract(self.coordinates, coords)
+        length = float(np.linalg.norm(displace))
+        p_coords = np.array(self.coordinates, dtype=float)
+        if self.is_sphere() or len(type(self)._model._children[self._cid]) > 0:
+            self_r = 0.5 * self.diameter
+            if self_r < length - EPSILON:
+                p_coords -= displace * (self_r / length)
+                length -= self_r
+        divisions = max(1, int(np.ceil(length / maximum_segment_length)))
+        x = np.linspace(p_coords[0], coords[0], divisions + 1)
+        y = np.linspace(p_coords[1], coords[1], divisions + 1)
+        z = np.linspace(p_coords[2], coords[2], divisions + 1)
+        d = np.linspace(self.diameter, diameter, divisions + 1)
+        seg, section = self, []
+        for k in range(1, divisions + 1):
+            seg = seg.add_segment((x[k], y[k], z[k]), d[k], segment_type=segment_type)
+            section.append(seg)
+        return section
+
+    def insert(self, mechanisms, outside=None):
+        """segment.py:124-160."""
+        from collections.abc import Mapping, Iterable
+        if isinstance(mechanisms, Mapping):
+            pass
+        elif isinstance(mechanisms, str):
+            mechanisms = {mechanisms: 1.0}
+        elif isinstance(mechanisms, Iterable):
+            mechanisms = {name: 1.0 for name in mechanisms}
+        else:
+            raise ValueError(f'Expected dictionary, not "{type(mechanisms)}"')
+        all_mech = type(self)._model.mechanisms
+        out = {}
+        for name, magnitude in mechanisms.items():
+            out[name] = all_mech[name](self, outside, float(magnitude))
+        return out
+
+    def inject_current(self, current=1.0, duration=1.0):
+        """electric.py:119-125: current (nA) for duration (ms), from the next input tick."""
+        current = float(current) * 1e-9
+        duration = float(duration)
+        assert duration >= 0
+        TimeSeries().constant_wave(current, duration).play(
+            self, "sum_current", clock=type(self)._model.input_hook, immediate=False)
+
+    # derived geometry (computed when the model is built)
+    def _geo(self, key):
+        model = type(self)._model
+        model._ensure_geometry()
+        return float(model._geo[key][self._cid])
+
+    @property
+    def length(self):
+        return self._geo("length")
+
+    @property
+    def surface_area(self):
+        return self._geo("surface_area")
+
+    @property
+    def cross_sectional_area(self):
+        return self._geo("cross_sectional_area")
+
+    @property
+    def volume(self):
+        return self._geo("volume")
+
+    @property
+    def capacitance(self):
+        return self._geo("capacitance")
+
+    @property
+    def axial_resistance(self):
+        return self._geo("axial_resistance")
+
+    def get_time_constant(self):
+        return self.capacitance / self.sum_conductance
+
+    def get_length_constant(self):
+        rm = 1.0 / (self.sum_conductance / self.length)
+        ri = self.axial_resistance / self.length
+        return (rm / ri) ** 0.5
+
+    @classmethod
+    def _advance_electric(cls, time_step):
+        """One electric step of ``time_step`` ms with the current arrays (electric.py:57-76)."""
+        cls._model._electric_step(float(time_step))
+
+
+class _Neuron(DBObject):
+    __slots__ = ()
+    _model = None
+
+    def __init__(self, coordinates, diameter, *, neuron_type=None, segment_type=None):
+        model = type(self)._model
+        cid = type(self)._db_class.new_row()
+        object.__setattr__(self, "_cid", cid)
+        root = model.Segment(None, coordinates, diameter, segment_type=segment_type)
+        root.neuron = cid
+        self.root = root._cid
+        self.AP_detector_segment = root._cid
+        self.neuron_type = neuron_type
+
+    def __getattr__(self, name):
+        if name == "root":
+            cid = DBObject.__getattr__(self, "root")
+            return type(self)._model.Segment._from_cid(int(cid))
+        return DBObject.__getattr__(self, name)
+
+    def __setattr__(self, name, value):
+        if name == "neuron_type":
+            type(self)._model._neuron_types[self._cid] = value
+            return
+        DBObject.__setattr__(self, name, value)
+
+    @property
+    def neuron_type(self):
+        return type(self)._model._neuron_types.get(self._cid)
+
+    def set_AP_detector(self, segment=None, threshold=None):
+        if segment is not None:
+            self.AP_detector_segment = segment._cid
+        if threshold is not None:
+            type(self)._db_class.class_attributes["AP_detector_threshold"] = float(threshold)
+
+
+# ================================================================================================
+# The model
+# ================================================================================================
+class RxD_Model:
+    """rxd_model.py:8-123 on the MI355X (see module docstring)."""
+
+    def __init__(self, time_step=0.1, *, temperature=37, initial_voltage=-70,
+                 cytoplasmic_resistance=100, membrane_capacitance=1, extracellular_tortuosity=1.55,
+                 extracellular_max_distance=20e-6, species=(), mechanisms=(), device=0,
+                 prune=EPSILON):
+        self.time_step = float(time_step)
+        self.temperature = float(temperature)
+        self.initial_voltage = float(initial_voltage)
+        self.cytoplasmic_resistance = float(cytoplasmic_resistance)
+        self.membrane_capacitance = float(membrane_capacitance)
+        self.prune = float(prune)
+        self.device = int(device)
+        self.database = db = Database()
+        db._model = self
+        self.input_hook = Clock(0.5 * self.time_step, units="ms")
+        self.advance_hook = self.clock = Clock(self.time_step, units="ms")
+        db.add_clock(self.clock)
+        self._children = []
+        self._neuron_types = {}
+        self._built = False
+        self._geo = None
+        self._propagators = {}
+        # Segment class and attributes (segment.py:11-25, geometry.py:42-73, electric.py:10-43)
+        seg = DBClass(db, "Segment")
+        seg.add_attribute("parent", NULL, dtype=np.int64, device=False)
+        seg.add_attribute("neuron", NULL, dtype=np.int64, device=False)
+        seg.add_attribute("coordinates", (0.0, 0.0, 0.0), dtype=np.float64, shape=(3,),
+                          device=False, on_write=self._invalidate_structure)
+        seg.add_attribute("diameter", 1.0, valid_range=(EPSILON, None), device=False,
+                          on_write=self._invalidate_structure)
+        seg.add_attribute("voltage", self.initial_voltage)
+        seg.add_attribute("integral_voltage", 0.0)
+        seg.add_attribute("nonspecific_current", 0.0)
+        seg.add_attribute("sum_current", 0.0)
+        seg.add_attribute("sum_conductance", 0.0, valid_range=(0.0, None))
+        seg.add_attribute("driving_voltage", 0.0)
+        seg.add_class_attribute("cytoplasmic_resistance", self.cytoplasmic_resistance)
+        seg.add_class_attribute("membrane_capacitance", self.membrane_capacitance)
+        self.Segment = type("Segment", (_Segment,), {"__slots__": (), "_db_class": seg, "_model": self})
+        neu = DBClass(db, "Neuron")
+        neu.add_attribute("root", NULL, dtype=np.int64, device=False)
+        neu.add_attribute("AP_detector_segment", NULL, dtype=np.int64, device=False,
+                          on_write=self._invalidate_references)
+        neu.add_attribute("AP_detected", 0, dtype=np.uint8, device_dtype="uint8")
+        neu.add_attribute("_AP_true_state", 0, dtype=np.uint8, device_dtype="uint8")
+        neu.add_class_attribute("AP_detector_threshold", 20.0)
+        self.Neuron = type("Neuron", (_Neuron,), {"__slots__": (), "_db_class": neu, "_model": self})
+        # species: electric species with constant reversal potentials (species.py:85-142)
+        self.species = {}
+        for kw in species:
+            name = kw["name"]
+            rev = kw.get("reversal_potential")
+            if rev is not None:
+                seg.add_attribute(f"{name}_conductance", 0.0, valid_range=(0.0, None))
+                seg.add_class_attribute(f"{name}_reversal_potential", float(rev))
+            self.species[name] = dict(kw)
+        # mechanisms (mechanisms.py:56-99)
+        self.mechanisms = {}
+        for m in mechanisms:
+            self._load_mechanism(m)
+
+    # ------------------------------------------------------------------ accessors (rxd_model.py:37-45)
+    def get_temperature(self):
+        return self.temperature
+
+    def get_clock(self):
+        return self.clock
+
+    def get_database(self):
+        return self.database
+
+    def get_mechanisms(self):
+        return dict(self.mechanisms)
+
+    def get_Neuron(self):
+        return self.Neuron
+
+    def get_Segment(self):
+        return self.Segment
+
+    def get_species(self):
+        return dict(self.species)
+
+    def get_time_step(self):
+        return self.time_step
+
+    def register_input_callback(self, function):
+        self.input_hook.register_callback(function)
+
+    def register_advance_callback(self, function):
+        self.advance_hook.register_callback(function)
+
+    def check(self):
+        self._ensure_built()
+        self.database.check()
+
+    # ------------------------------------------------------------------ mechanisms
+    def _load_mechanism(self, m):
+        if isinstance(m, str):
+            key = os.path.splitext(os.path.basename(m))[0]
+            if key not in _BUILTIN:
+                raise ValueError(f"unsupported mechanism {m!r} (built-ins: {sorted(_BUILTIN)})")
+            mech = _BUILTIN[key](self)
+        elif isinstance(m, LTIMechanism):
+            mech = m.initialize(self)
+        else:
+            retval = m.initialize(self)
+            mech = retval if retval is not None else m
+        name = str(mech.get_name())
+        assert name not in self.mechanisms
+        self.mechanisms[name] = mech
+
+    # ------------------------------------------------------------------ structure changes
+    def _invalidate_structure(self):
+        if self._built:
+            self._to_host()
+        self._geo = None
+        self._propagators = {}
+
+    def _invalidate_mechanisms(self):
+        if self._built:
+            self._to_host()
+
+    def _invalidate_references(self):
+        self._refs_dirty = True
+
+    def _to_host(self):
+        for c in self.database.classes.values():
+            order = getattr(c, "_order", None)
+            if order is not None:
+                inv = np.empty(len(order), dtype=np.int64)
+                inv[order] = np.arange(len(order))
+                for col in c.columns.values():
+                    col.to_host(inv)
+            c.row_of = np.arange(c.size)
+            c._order = None
+        self._built = False
+
+    # ------------------------------------------------------------------ build
+    def _ensure_geometry(self):
+        if self._geo is not None:
+            return
+        seg = self.database.get_class("Segment")
+        parent = np.asarray(seg.columns["parent"].host, dtype=np.int64)
+        coords = np.asarray(seg.columns["coordinates"].host, dtype=np.float64).reshape(-1, 3)
+        diam = np.asarray(seg.columns["diameter"].host, dtype=np.float64)
+        geo = geometry.geometry(parent, coords, diam) if len(parent) else \
+            {k: np.zeros(0) for k in ("length", "surface_area", "area_at_creation",
+                                      "cross_sectional_area", "volume")}
+        R, C = geometry.passive(geo["length"], geo["cross_sectional_area"], geo["area_at_creation"],
+                                self.cytoplasmic_resistance, self.membrane_capacitance)
+        geo["axial_resistance"], geo["capacitance"] = R, C
+        self._geo = geo  # indexed by creation id
+
+    def _ensure_built(self):
+        if self._built:
+            return
+        torch = _torch()
+        dev = torch.device("cuda", self.device)
+        self._dev = dev
+        self._ensure_geometry()
+        db = self.database
+        seg = db.get_class("Segment")
+        N = seg.size
+        parent = np.asarray(seg.columns["parent"].host, dtype=np.int64).reshape(-1)
+        # cells: connected trees, ordered by the creation of their root (stable)
+        root = np.arange(N)
+        for i in range(N):
+            if parent[i] != NULL:
+                root[i] = root[parent[i]]
+        order = np.argsort(root, kind="stable")
+        row_of = np.empty(N, dtype=np.int64)
+        row_of[order] = np.arange(N)
+        sroot = root[order]
+        starts = np.nonzero(np.r_[True, sroot[1:] != sroot[:-1]])[0] if N else np.zeros(0, int)
+        cell_ptr = np.r_[starts, N].astype(np.int64)
+        parent_local = np.where(parent[order] < 0, -1, row_of[np.maximum(parent[order], 0)] -
+                                np.repeat(cell_ptr[:-1], np.diff(cell_ptr)))
+        self._cell_ptr = cell_ptr
+        self._parent_local = parent_local.astype(np.int32)
+        self._R = self._geo["axial_resistance"][order]
+        self._C = self._geo["capacitance"][order]
+        seg._order, seg.row_of = order, row_of
+        for col in seg.columns.values():
+            col.to_device(order, dev)
+        self._inv_c = torch.as_tensor((1.0 / self._C).astype(np.float32), device=dev)
+        # neurons keep creation order; their AP segments are device rows
+        neu = db.get_class("Neuron")
+        neu._order = np.arange(neu.size)
+        for col in neu.columns.values():
+            col.to_device(neu._order, dev)
+        self._upload_references()
+        # mechanisms: instances sorted by the row of their segment
+        self._sources = []
+        gbuf_len = 0
+        for mech in self.mechanisms.values():
+            if not isinstance(mech, type) or not issubclass(mech, _DeviceMechanism):
+                continue
+            mc = mech._db_class
+            segs = np.asarray(mc.columns["segment"].host, dtype=np.int64)
+            rows = row_of[segs] if len(segs) else segs
+            morder = np.argsort(rows, kind="stable")
+            mc._order = morder
+            mc.row_of = np.empty(mc.size, dtype=np.int64)
+            mc.row_of[morder] = np.arange(mc.size)
+            for col in mc.columns.values():
+                col.to_device(morder, dev)
+            mech._seg_dev = torch.as_tensor(rows[morder].astype(np.int32), device=dev)
+            mech._mag_sorted = np.asarray(mc.columns["magnitude"].host, dtype=np.float64)[morder]
+            k = len(mech.conductance_sources)
+            mech._g_offset = gbuf_len
+            for j, src in enumerate(mech.conductance_sources):
+                E = mech.E[src] if src in ("na", "k", "l") else mech.spec.reversal
+                self._sources.append((rows[morder], gbuf_len + j + k * np.arange(mc.size), E))
+            gbuf_len += k * mc.size
+            self._prepare_mechanism(mech, dev)
+        # species conductances: one source per segment each
+        self._species_cols = []
+        for name, kw in self.species.items():
+            if kw.get("reversal_potential") is None:
+                continue
+            self._species_cols.append((name, gbuf_len))
+            self._sources.append((np.arange(N), gbuf_len + np.arange(N),
+                                  float(kw["reversal_potential"])))
+            gbuf_len += N
+        self._gbuf = torch.zeros(max(gbuf_len, 1), dtype=torch.float32, device=dev)
+        if self._sources:
+            srow = np.concatenate([s[0] for s in self._sources])
+            sidx = np.concatenate([s[1] for s in self._sources])
+            sE = np.concatenate([np.full(len(s[0]), s[2]) for s in self._sources])
+        else:
+            srow = sidx = np.zeros(0, dtype=np.int64)
+            sE = np.zeros(0)
+        so = np.argsort(srow, kind="stable")
+        ptr = np.zeros(N + 1, dtype=np.int64)
+        np.cumsum(np.bincount(srow, minlength=N), out=ptr[1:])
+        self._acc_ptr = torch.as_tensor(ptr, device=dev)
+        self._acc_idx = torch.as_tensor(sidx[so].astype(np.int64), device=dev)
+        self._acc_E = torch.as_tensor(sE[so].astype(np.float32), device=dev)
+        self._built = True
+
+    def _upload_references(self):
+        torch = _torch()
+        neu = self.database.get_class("Neuron")
+        seg = self.database.get_class("Segment")
+        ap = np.asarray(neu.columns["AP_detector_segment"].host, dtype=np.int64)
+        rows = np.where(ap < 0, -1, seg.row_of[np.maximum(ap, 0)]) if len(ap) else ap
+        self._ap_seg = torch.as_tensor(rows.astype(np.int32), device=self._dev_or(torch))
+        self._refs_dirty = False
+
+    def _dev_or(self, torch):
+        return getattr(self, "_dev", None) or torch.device("cuda", self.device)
+
+    def _prepare_mechanism(self, mech, dev):
+        torch = _torch()
+        mc = mech._db_class
+        mag = torch.as_tensor(mech._mag_sorted.astype(np.float32), device=dev)
+        if hasattr(mech, "gate_mode"):
+            mech._gbar_dev = {ion: (mag * (mech.gbar[ion] * channels.CM2_TO_UM2)).contiguous()
+                              for ion in ("na", "k", "l")}
+            grid = channels.LinearInput(-100.0, 100.0, 2000)
+            mech._grid = grid
+            mech._table = torch.as_tensor(
+                channels.hh_gate_table(self.time_step, self.temperature, grid).astype(np.float32),
+                device=dev)
+        else:
+            spec = mech.spec
+            spec.scheme.time_step = self.time_step
+            mech._node_table = torch.as_tensor(spec.scheme.node_table().astype(np.float32), device=dev)
+            mech._weights = torch.as_tensor(np.asarray(spec.scheme.weights, dtype=np.float32), device=dev)
+            mech._gmax_dev = (mag * spec.gmax).contiguous()
+            S = spec.scheme.num_states
+            mech._states = torch.stack([mc.columns[s].dev for s in spec.scheme.state_names]).contiguous()
+            for k, s in enumerate(spec.scheme.state_names):
+                mc.columns[s].dev = mech._states[k]
+
+    # ------------------------------------------------------------------ kernels
+    def _stream(self):
+        return _torch().cuda.current_stream(self._dev).cuda_stream
+
+    def _col(self, cls, name):
+        return self.database.get_class(cls).columns[name].dev
+
+    def _run_hh_mechanism(self, mech):
+        mc = mech._db_class
+        M = mc.size
+        if M == 0:
+            return
+        P = _lib.ptr
+        out = self._gbuf[mech._g_offset:mech._g_offset + 3 * M]
+        g = mech._grid
+        _lib.check(_lib.load().nw_hh_mechanism(
+            M, 1 if mech.gate_mode == "exact" else 0, self.time_step,
+            channels.q10_factor(self.temperature), P(mech._table), g.num_buckets + 1, g.minimum,
+            g.maximum, P(self._col("Segment", "voltage")), P(mc.columns["segment"].dev),
+            P(mc.columns["m"].dev), P(mc.columns["h"].dev), P(mc.columns["n"].dev),
+            P(mech._gbar_dev["na"]), P(mech._gbar_dev["k"]), P(mech._gbar_dev["l"]), P(out),
+            self._stream()))
+
+    def _run_lti_mechanism(self, mech):
+        mc = mech._db_class
+        M = mc.size
+        if M == 0:
+            return
+        spec = mech.spec
+        P = _lib.ptr
+        out = self._gbuf[mech._g_offset:mech._g_offset + M]
+        grid = spec.scheme.grid
+        _lib.check(_lib.load().nw_lti_advance(
+            spec.scheme.num_states, grid.num_buckets + 1, grid.minimum, grid.maximum,
+            P(mech._node_table), M, P(self._col("Segment", "voltage")), P(mech._seg_dev),
+            P(mech._states), M, spec.scheme.conserve_sum or 0.0, 1, P(mech._weights),
+            P(mech._gmax_dev), P(out), self._stream()))
+
+    def _propagator(self, time_step):
+        from .population import Propagator
+        key = round(time_step, 12)
+        prop = self._propagators.get(key)
+        if prop is None:
+            prop = Propagator(self._cell_ptr, self._parent_local, self._R, self._C, time_step,
+                              self.prune, self.device)
+            self._propagators = {key: prop}
+        return prop
+
+    def _electric_step(self, time_step):
+        self._ensure_built()
+        if self.database.get_class("Segment").size == 0:
+            return
+        for col in self.database.get_class("Segment").columns.values():
+            col._flush()
+        prop = self._propagator(time_step)
+        P = _lib.ptr
+        _lib.check(_lib.load().nw_electric_half_step(
+            prop.handle, time_step, P(self._col("Segment", "voltage")),
+            P(self._col("Segment", "sum_conductance")), P(self._col("Segment", "driving_voltage")),
+            P(self._col("Segment", "sum_current")), P(self._inv_c),
+            P(self._col("Segment", "integral_voltage")), self._stream()))
+
+    # ------------------------------------------------------------------ the step (rxd_model.py:63-123)
+    def advance(self):
+        """Advance the model by one time step: half step, mechanisms, half step, AP detector."""
+        self._ensure_built()
+        if self._refs_dirty:
+            self._upload_references()
+        self._advance_species()
+        self._advance_mechanisms()
+        self._advance_species()
+        self._advance_ap_detector()
+        self.advance_hook.tick()
+
+    def _advance_species(self):
+        """Accumulate conductances and currents, run input callbacks, one dt/2 electric step
+        (rxd_model.py:91-113)."""
+        seg = self.database.get_class("Segment")
+        N = seg.size
+        if N == 0:
+            self.input_hook.tick()
+            return
+        P = _lib.ptr
+        for col in seg.columns.values():
+            col._flush()
+        _lib.check(_lib.load().nw_accumulate(
+            N, P(self._acc_ptr), P(self._gbuf), P(self._acc_idx), P(self._acc_E),
+            P(self._col("Segment", "nonspecific_current")), P(self._col("Segment", "sum_conductance")),
+            P(self._col("Segment", "driving_voltage")), P(self._col("Segment", "sum_current")),
+            self._stream()))
+        self.input_hook.tick()
+        self._electric_step(0.5 * self.time_step)
+
+    def _advance_mechanisms(self):
+        """Zero the accumulators and advance every mechanism (rxd_model.py:115-123)."""
+        self._gbuf.zero_()
+        for name, _ in self._species_cols:          # species._zero_accumulators (species.py:147-153)
+            self._col("Segment", f"{name}_conductance").zero_()
+        self._col("Segment", "nonspecific_current").zero_()
+        for name, m in self.mechanisms.items():
+            try:
+                m.advance()
+            except Exception:
+                print(f"ERROR in mechanism {name}.")
+                raise
+        for name, off in self._species_cols:
+            seg_col = self._col("Segment", f"{name}_conductance")
+            self._gbuf[off:off + len(seg_col)].copy_(seg_col)
+
+    def _advance_ap_detector(self):
+        neu = self.database.get_class("Neuron")
+        if neu.size == 0:
+            return
+        thr = neu.class_attributes["AP_detector_threshold"]
+        P = _lib.ptr
+        _lib.check(_lib.load().nw_ap_detect(
+            neu.size, P(self._ap_seg), P(self._col("Segment", "voltage")),
+            float(thr), P(self._col("Neuron", "_AP_true_state")), P(self._col("Neuron", "AP_detected")),
+            self._stream()))
+
+
+class Model(RxD_Model):
+    """neuwon/model.py:6-22: ``Model(simulation={...}, species=..., mechanisms=...)``."""
+
+    def __init__(self, simulation=None, *, species=(), mechanisms=(), regions=None, segments=None,
+                 neurons=None, synapses=None):
+        self.parameters = dict(simulation=simulation or {}, species=species, mechanisms=mechanisms,
+                               regions=regions or {}, segments=segments or {},
+                               neurons=neurons or {}, synapses=synapses or {})
+        if regions or neurons or synapses:
+            raise NotImplementedError("regions, neuron growth and synapses are outside the advance path")
+        if isinstance(species, dict):
+            species = [dict(v, name=k) for k, v in species.items()]
+        if isinstance(mechanisms, dict):
+            mechanisms = list(mechanisms.values())
+        super().__init__(species=species, mechanisms=mechanisms, **(simulation or {}))
+
+    def get_parameters(self):
+        return self.parameters

@@ -56,8 +56,11 @@ class Propagator:
         vals = np.empty(int(off[-1]), dtype=np.float32)
         _lib.check(_lib.load().nw_propagator_cell_export(self.handle, cell, None, _lib.ptr(cols),
                                                          _lib.ptr(vals)))
-        return scipy.sparse.csr_matrix((vals.astype(np.float64), cols.astype(np.int64),
-                                        off.astype(np.int64)), shape=(n, n))
+        m = scipy.sparse.csr_matrix((vals.astype(np.float64), cols.astype(np.int64),
+                                     off.astype(np.int64)), shape=(n, n))
+        m.sum_duplicates()      # row padding: zero entries on the diagonal column
+        m.eliminate_zeros()
+        return m
 
     def close(self):
         if getattr(self, "handle", None):

@@ -463,3 +463,86 @@ class HHSim:
             self.ap_detected = over & ~self.ap_state
             self.ap_state = over
         self.steps += 1
+
+
+# ---------------------------------------------------------------------------------------------
+# C restatement (oracle/neuwon_oracle_c.c) -- same step, FP64, OpenMP over cells
+# ---------------------------------------------------------------------------------------------
+def _c_lib():
+    import ctypes
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `make -C oracle`")
+    lib = ctypes.CDLL(path)
+
+    class Params(ctypes.Structure):
+        _fields_ = [("dt", ctypes.c_double), ("vmin", ctypes.c_double), ("vmax", ctypes.c_double),
+                    ("num_buckets", ctypes.c_int32), ("e_na", ctypes.c_double),
+                    ("e_k", ctypes.c_double), ("e_l", ctypes.c_double),
+                    ("stim_mode", ctypes.c_int32), ("stim_amp", ctypes.c_double),
+                    ("seed", ctypes.c_uint32), ("threshold", ctypes.c_uint32),
+                    ("pulse_ticks", ctypes.c_int32), ("gate_mode", ctypes.c_int32),
+                    ("celsius", ctypes.c_double)]
+    lib.oracle_hh_run.restype = ctypes.c_int
+    lib.oracle_stim_hash.restype = ctypes.c_uint32
+    lib.oracle_stim_hash.argtypes = [ctypes.c_uint32] * 3
+    lib.Params = Params
+    return lib
+
+
+class CHHPopulation:
+    """Cells with leak + HH Na/K advanced by the C restatement.
+
+    cell_ptr: int64 [cells+1]; P: scipy CSR over all segments (block diagonal, global columns);
+    C, g_bar (dict na/k/l): per segment; stim: None | ('constant', amp_A) |
+    ('poisson', amp_A, seed, threshold, pulse_ticks), applied to segment 0 of every cell."""
+
+    def __init__(self, cell_ptr, P, C, g_bar, v0=-70.0, dt=0.1, celsius=6.3, gate_mode="table",
+                 stim=None, E=None, tables=None):
+        import ctypes
+        self.lib = _c_lib()
+        self.cell_ptr = np.ascontiguousarray(cell_ptr, dtype=np.int64)
+        P = P.tocsr()
+        self.rowptr = np.ascontiguousarray(P.indptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(P.indices, dtype=np.int32)
+        self.val = np.ascontiguousarray(P.data, dtype=np.float64)
+        self.C = np.ascontiguousarray(C, dtype=np.float64)
+        self.g = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in g_bar.items()}
+        N = len(self.C)
+        self.v = np.full(N, float(v0))
+        self.gates = {g: np.full(N, float(gate_inf(v0, g, celsius))) for g in "mhn"}
+        self.sum_g = np.zeros(N)
+        self.d_num = np.zeros(N)
+        self.steps = 0
+        E = E or {"na": 60.0, "k": -88.0, "l": -54.3}
+        p = self.lib.Params()
+        p.dt, p.vmin, p.vmax, p.num_buckets = dt, -100.0, 100.0, 2000
+        p.e_na, p.e_k, p.e_l = E["na"], E["k"], E["l"]
+        p.gate_mode = {"table": 0, "exact": 1}[gate_mode]
+        p.celsius = celsius
+        if stim is None:
+            p.stim_mode = 0
+        elif stim[0] == "constant":
+            p.stim_mode, p.stim_amp = 1, stim[1]
+        else:
+            p.stim_mode, p.stim_amp, p.seed, p.threshold, p.pulse_ticks = 2, stim[1], stim[2], stim[3], stim[4]
+        self.p = p
+        if tables is None:
+            tables = {}
+            for g in "mhn":
+                deriv = two_state_derivative(hh_gate_rate(g, celsius))
+                tables[g] = node_table_to_polynomial(lti_node_table(deriv, 2, dt))
+        self.poly = np.ascontiguousarray(np.stack([tables[g] for g in "mhn"]), dtype=np.float64)
+
+    def advance(self, n_steps=1, threads=0):
+        import ctypes
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        rc = self.lib.oracle_hh_run(
+            ctypes.byref(self.p), ctypes.c_int64(len(self.cell_ptr) - 1), P(self.cell_ptr),
+            P(self.rowptr), P(self.col), P(self.val), P(self.C), P(self.g["na"]), P(self.g["k"]),
+            P(self.g["l"]), P(self.v), P(self.gates["m"]), P(self.gates["h"]), P(self.gates["n"]),
+            P(self.sum_g), P(self.d_num), P(self.poly), ctypes.c_int64(self.steps),
+            ctypes.c_int32(n_steps), ctypes.c_int32(threads))
+        assert rc == 0
+        self.steps += n_steps

@@ -1,0 +1,190 @@
+"""The drop-in RxD_Model API on the GPU, following the reference's own tests
+(neuwon/rxd/tests/test_electric.py, test_model.py, test_segment.py)."""
+import bisect
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import neuwon_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(msg):
+    print(f"[gpu-model] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_time_constant(torch_gpu):
+    """test_electric.py:28-46 through Segment._advance_electric."""
+    from neuwon_amd.model import RxD_Model
+    m = RxD_Model()
+    root = m.Neuron([0, 0, 0], 20).root
+    m.advance()   # builds the model on the device
+    root.voltage = 0
+    root.driving_voltage = 1
+    root.sum_conductance = 1e-10
+    root.sum_current = 0
+    tau = root.get_time_constant()
+    nstep = 1000
+    dt = tau / nstep * 1e3
+    for _ in range(nstep):
+        m.Segment._advance_electric(dt)
+    _say(f"time constant: v(tau) = {root.voltage!r}")
+    assert root.voltage == pytest.approx(1 - 1 / np.e, rel=2e-6)
+    m.check()
+
+
+def test_inject_current(torch_gpu):
+    """test_electric.py:132-150."""
+    from neuwon_amd.model import RxD_Model
+    m = RxD_Model(time_step=.01)
+    root = m.Neuron([0, 0, 0], 10).root
+    init_v = root.voltage
+    while m.clock() < 3:
+        m.advance()
+    assert init_v == pytest.approx(root.voltage)
+    init_Q = 1e-3 * root.voltage * root.capacitance
+    root.inject_current(1, 2)
+    delta_Q = 1e-9 * 2e-3
+    while m.clock() < 6:
+        m.advance()
+    m.check()
+    new_Q = 1e-3 * root.voltage * root.capacitance
+    _say(f"inject: dQ/expected - 1 = {(new_Q - init_Q) / delta_Q - 1:.2e}")
+    assert init_Q + delta_Q == pytest.approx(new_Q, rel=1e-5)
+
+
+def test_length_constant(torch_gpu):
+    """test_electric.py:48-94 (case 1): attenuation 1/e at one length constant."""
+    from neuwon_amd.model import RxD_Model
+    diam, rm, max_len, dt, ptol = 1, 4e11, 2, .01, .05
+    m = RxD_Model()
+    root = m.Neuron([0, 0, 0], diam).root
+    section = [root] + root.add_section([3000, 0, 0], diam, max_len)
+    m.advance()
+    for seg in section:
+        seg.voltage = 0
+        seg.driving_voltage = 0
+        seg.sum_conductance = seg.surface_area / rm
+        seg.sum_current = 0
+    probe = section[len(section) // 2]
+    probe_x = probe.coordinates[0]
+    lam = probe.get_length_constant()
+    for _ in range(round(10 / dt)):
+        probe.voltage = 1
+        m.Segment._advance_electric(dt)
+    xs = [seg.coordinates[0] for seg in section]
+    test = section[bisect.bisect_left(xs, probe_x + lam)]
+    ratio = test.voltage / probe.voltage
+    _say(f"length constant {lam:.1f} um, attenuation {ratio:.4f}")
+    assert ratio == pytest.approx(1 / np.e, ptol)
+
+
+def _golden_model(mechanism):
+    from neuwon_amd.model import RxD_Model, TimeSeries
+    m = RxD_Model(time_step=.1, temperature=6.3, mechanisms=[mechanism])
+    hh = m.mechanisms[mechanism]
+    root = tip = m.Neuron([-1, 0, 7], 5.7).root
+    hh(root)
+    for x in range(10):
+        tip = m.Segment(tip, [x, 0, 7], 1)
+        hh(tip)
+    x = TimeSeries().record(tip, 'voltage')
+    m.advance()
+    m.check()
+    while m.clock() < 20:
+        for t in [5, 15]:
+            if m.clock.is_now(t):
+                root.inject_current(.1, 1)
+        m.advance()
+    m.check()
+    return np.array(x.get_data())
+
+
+@pytest.mark.parametrize("mechanism", ["hh", "hh_kinetic"])
+def test_model_hh_golden(torch_gpu, mechanism):
+    """test_model.py:35-128 through the drop-in API, against the golden trace and the oracle."""
+    golden = np.loadtxt(os.path.join(GOLDEN, "hh_trace_test_model.txt"))
+    v = _golden_model(mechanism)
+    assert len(v) == 201
+    err = np.abs(v - golden)
+    _say(f"{mechanism}: max |v - golden| = {err.max():.3f} mV")
+    assert err.max() < 30.0
+    assert err[:51].max() < 0.2
+    # oracle of the same scheme
+    parents = [-1] + list(range(10))
+    coords = [np.array([-1.0, 0, 7])] + [np.array([float(x), 0, 7]) for x in range(10)]
+    g = O.geometry(np.array(parents), np.array(coords), np.array([5.7] + [1.0] * 10))
+    R, C = O.passive(g["length"], g["cross_sectional_area"], g["area_at_creation"])
+    P = O.propagator(np.array(parents), R, C, 0.05)
+    mag = g["area_at_creation"]
+    gbar = {"na": 0.12e-8 * mag, "k": 0.036e-8 * mag, "l": 0.0003e-8 * mag}
+    pulses = []
+
+    def stim(tick):
+        I = np.zeros(11)
+        for t0, nt in pulses:
+            if t0 <= tick < t0 + nt:
+                I[0] += 0.1e-9
+        return I
+    sim = O.HHSim(P, C, gbar, gate_mode="exact" if mechanism == "hh" else "table", stim=stim)
+    ref = [sim.v[-1]]
+    sim.advance()
+    ref.append(sim.v[-1])
+    ticks = 1
+    while ticks * 0.1 < 20:
+        for t in (5, 15):
+            if round(t / 0.1) == ticks:
+                pulses.append((sim.input_ticks + 1, O.pulse_ticks(1.0, 0.05)))
+        sim.advance()
+        ticks += 1
+        ref.append(sim.v[-1])
+    ref = np.array(ref)
+    d = np.abs(v - ref)
+    _say(f"{mechanism}: max |v - oracle| = {d.max():.2e} mV")
+    assert d.max() < 5e-3
+
+
+def test_insert_api(torch_gpu):
+    """test_segment.py:51-69: insert by dict, name and list; duplicates allowed."""
+    from neuwon_amd.model import RxD_Model
+    m = RxD_Model(mechanisms=['hh'])
+    root = m.Neuron([0, 0, 0], 10).root
+    segs = root.add_section([100, 0, 0], 3, maximum_segment_length=10)
+    root.insert({})
+    inst = root.insert({'hh': 1})
+    segs[2].insert('hh')
+    segs[3].insert(['hh'])
+    segs[4].insert({'hh': 1})
+    segs[4].insert({'hh': 1})
+    assert inst['hh'].get_database_class().get_name() == 'hh'
+    for _ in range(20):
+        m.advance()
+    m.check()
+    g = m.database.get_data("Segment.sum_conductance")
+    assert g[segs[4].get_unstable_index()] > 1.9 * g[segs[3].get_unstable_index()] * \
+        segs[3].surface_area / segs[4].surface_area
+
+
+def test_lti_mechanism_api(torch_gpu):
+    """A kinetic mechanism given as an LTI scheme runs through nw_lti_advance."""
+    from neuwon_amd import channels
+    from neuwon_amd.model import RxD_Model, LTIMechanism
+    scheme = channels.random_ampa13_scheme(seed=2)
+    mech = LTIMechanism(scheme, gmax_s_per_cm2=0.002, reversal=0.0, name="markov13")
+    m = RxD_Model(time_step=0.1, temperature=6.3, mechanisms=[mech, 'hh'])
+    root = m.Neuron([0, 0, 0], 10).root
+    segs = root.add_section([200, 0, 0], 2, maximum_segment_length=20)
+    for s in [root] + segs:
+        s.insert({'hh': 1, 'markov13': 1})
+    for _ in range(50):
+        m.advance()
+    m.check()
+    states = np.stack([m.database.get_data(f"markov13.s{k}") for k in range(13)])
+    assert np.abs(states.sum(axis=0) - 1).max() < 1e-5
+    v = m.database.get_data("Segment.voltage")
+    assert np.all(np.isfinite(v)) and v.max() < 60

@@ -214,7 +214,7 @@ def test_accumulate(torch_gpu):
     t = lambda a, dt=np.float32: torch.as_tensor(np.asarray(a, dtype=dt), device="cuda")
     d_ptr, d_g, d_e, d_i = t(ptr, np.int64), t(g), t(E), t(Inon)
     sg, dv, si = (torch.zeros(nseg, device="cuda") for _ in range(3))
-    _lib.check(_lib.load().nw_accumulate(nseg, _lib.ptr(d_ptr), _lib.ptr(d_g), _lib.ptr(d_e),
+    _lib.check(_lib.load().nw_accumulate(nseg, _lib.ptr(d_ptr), _lib.ptr(d_g), None, _lib.ptr(d_e),
                                          _lib.ptr(d_i), _lib.ptr(sg), _lib.ptr(dv), _lib.ptr(si),
                                          None))
     torch.cuda.synchronize()
@@ -224,10 +224,3 @@ def test_accumulate(torch_gpu):
     assert np.allclose(dv.cpu().numpy(), ref_d, rtol=1e-5, atol=1e-5)
     assert np.allclose(si.cpu().numpy(), Inon, rtol=1e-7, atol=0)
     assert np.all(dv.cpu().numpy()[per == 0] == 0)
-
-
-def test_stim_hash_matches_oracle():
-    from neuwon_amd import _lib
-    lib = _lib.load()
-    for seed, cell, tick in [(0, 0, 0), (1, 2, 3), (123456, 9999, 4000000), (2**32 - 1, 7, 11)]:
-        assert lib.nw_stim_hash(seed, cell, tick) == int(O.stim_hash(seed, cell, tick))

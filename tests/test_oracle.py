@@ -1,0 +1,366 @@
+"""Pins of the FP64 oracle: reference closed-form tests, library routines, known answers.
+
+Each check is chosen so that a plausible slip (dt vs dt/2, a unit factor, a transposed matrix,
+a wrong sign in the relaxation) fails it.
+"""
+import bisect
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse.linalg
+
+from oracle import neuwon_oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------------------------------------------
+# helpers: reference-style segment construction (segment.py:27-104)
+# ------------------------------------------------------------------------------------------------
+def add_section(parent_idx, coords, diam, parents, target, target_diam, max_len):
+    """Segment.add_section (segment.py:75-104): indices of the new segments."""
+    p_coords = np.array(coords[parent_idx], dtype=float)
+    displace = p_coords - np.asarray(target, dtype=float)
+    length = float(np.linalg.norm(displace))
+    is_sphere = parents[parent_idx] == -1
+    has_children = any(p == parent_idx for p in parents)
+    if is_sphere or has_children:
+        r = 0.5 * diam[parent_idx]
+        if r < length - O.EPSILON:
+            p_coords -= displace * (r / length)
+            length -= r
+    div = max(1, int(np.ceil(length / max_len)))
+    xs = [np.linspace(p_coords[k], target[k], div + 1) for k in range(3)]
+    ds = np.linspace(diam[parent_idx], target_diam, div + 1)
+    out = []
+    prev = parent_idx
+    for i in range(1, div + 1):
+        parents.append(prev)
+        coords.append(np.array([xs[0][i], xs[1][i], xs[2][i]]))
+        diam.append(ds[i])
+        prev = len(parents) - 1
+        out.append(prev)
+    return out
+
+
+def build(parents, coords, diam, Ra=100.0, Cm=1.0):
+    parent = np.array(parents, dtype=np.int64)
+    g = O.geometry(parent, np.array(coords), np.array(diam))
+    R, C = O.passive(g["length"], g["cross_sectional_area"], g["area_at_creation"], Ra, Cm)
+    return parent, g, R, C
+
+
+# ------------------------------------------------------------------------------------------------
+# electric closed forms (neuwon/rxd/tests/test_electric.py)
+# ------------------------------------------------------------------------------------------------
+def test_time_constant():
+    """test_electric.py:28-46: one sphere relaxes to 1 - 1/e after one time constant."""
+    parent, g, R, C = build([-1], [np.zeros(3)], [20.0])
+    v = np.array([0.0])
+    gsum = np.array([1e-10])
+    d = np.array([1.0])
+    tau = C[0] / gsum[0]
+    nstep = 1000
+    dt = tau / nstep * 1e3
+    P = O.propagator(parent, R, C, dt)
+    for _ in range(nstep):
+        v, _ = O.half_step(v, gsum, d, np.zeros(1), C, P, dt)
+    assert v[0] == pytest.approx(1 - 1 / np.e)
+
+
+def test_time_constant_catches_unit_slip():
+    """The same closed form with the exponent in ms instead of s misses by far."""
+    parent, g, R, C = build([-1], [np.zeros(3)], [20.0])
+    tau = C[0] / 1e-10
+    dt = tau / 1000 * 1e3
+    v = 0.0
+    for _ in range(1000):
+        v = 1.0 - (1.0 - v) * math.exp(-dt * 1e-10 / C[0])   # dt left in ms
+    assert abs(v - (1 - 1 / np.e)) > 0.3
+
+
+def _measure_length_constant(diam, rm, max_len, dt, ptol):
+    """test_electric.py:48-85."""
+    parents, coords, diams = [-1], [np.zeros(3)], [diam]
+    section = [0] + add_section(0, coords, diams, parents, [3000, 0, 0], diam, max_len)
+    parent, g, R, C = build(parents, coords, diams)
+    n = len(parent)
+    v = np.zeros(n)
+    gsum = g["surface_area"] / rm
+    d = np.zeros(n)
+    probe = section[len(section) // 2]
+    probe_x = coords[probe][0]
+    rm_l = 1.0 / (gsum[probe] / g["length"][probe])
+    ri = R[probe] / g["length"][probe]
+    lam = (rm_l / ri) ** 0.5
+    P = O.propagator(parent, R, C, dt)
+    for _ in range(round(10 / dt)):
+        v[probe] = 1.0
+        v, _ = O.half_step(v, gsum, d, np.zeros(n), C, P, dt)
+    xcoords = [coords[s][0] for s in section]
+    test = section[bisect.bisect_left(xcoords, probe_x + lam)]
+    ratio = v[test] / v[probe]
+    assert ratio == pytest.approx(1 / np.e, ptol)
+    assert coords[test][0] - probe_x == pytest.approx(lam, 0.1)
+    assert np.sum(P.data) / P.shape[0] == pytest.approx(1)   # charge conserved (test_electric.py:84-85)
+
+
+def test_length_constant_1():
+    _measure_length_constant(diam=1, rm=4e11, max_len=2, dt=0.01, ptol=0.05)
+
+
+def test_length_constant_2():
+    _measure_length_constant(diam=0.9, rm=2e11, max_len=10, dt=0.025, ptol=0.15)
+
+
+def test_length_constant_5():
+    _measure_length_constant(diam=2, rm=3e11, max_len=20, dt=0.1, ptol=0.25)
+
+
+def test_inject_current_charge():
+    """test_electric.py:132-150: 1 nA for 2 ms adds exactly 2 pC to an isolated sphere."""
+    parent, g, R, C = build([-1], [np.zeros(3)], [10.0])
+    dt = 0.01
+    half = 0.5 * dt
+    P = O.propagator(parent, R, C, half)
+    v = np.array([-70.0])
+    zero = np.zeros(1)
+    # no conductance: the state holds
+    for _ in range(300):
+        v, _ = O.half_step(v, zero, zero, zero, C, P, half)
+        v, _ = O.half_step(v, zero, zero, zero, C, P, half)
+    assert v[0] == pytest.approx(-70.0)
+    q0 = 1e-3 * v[0] * C[0]
+    ticks = O.pulse_ticks(2.0, half)
+    I = np.array([1e-9])
+    for t in range(1, 601):
+        v, _ = O.half_step(v, zero, zero, I if t <= ticks else zero, C, P, half)
+    q1 = 1e-3 * v[0] * C[0]
+    assert q0 + 1e-9 * 2e-3 == pytest.approx(q1)
+
+
+def test_section_counts():
+    """test_segment.py:6-25: section lengths after subtracting the root radius."""
+    parents, coords, diams = [-1], [np.zeros(3)], [4.0]
+    s1 = add_section(0, coords, diams, parents, [10, 0, 0], 1, 3)
+    s2 = add_section(0, coords, diams, parents, [10, 10, 0], 1, 3)
+    assert len(s1) == 3 and len(s2) == 5
+
+
+# ------------------------------------------------------------------------------------------------
+# propagator invariants and the reference's own expm routine
+# ------------------------------------------------------------------------------------------------
+def _random_tree(n, seed):
+    rng = np.random.default_rng(seed)
+    parent = np.full(n, -1)
+    for i in range(1, n):
+        parent[i] = rng.integers(0, i)
+    coords = np.zeros((n, 3))
+    for i in range(1, n):
+        coords[i] = coords[parent[i]] + rng.normal(size=3) * 8
+    diam = np.exp(rng.uniform(np.log(0.5), np.log(4), n))
+    return parent, coords, diam
+
+
+def test_propagator_matches_scipy_sparse_expm():
+    parent, coords, diam = _random_tree(60, 1)
+    g = O.geometry(parent, coords, diam)
+    R, C = O.passive(g["length"], g["cross_sectional_area"], g["area_at_creation"])
+    Pd = O.propagator(parent, R, C, 0.05, prune=0.0, method="dense").toarray()
+    Ps = O.propagator(parent, R, C, 0.05, prune=0.0, method="sparse").toarray()
+    assert np.abs(Pd - Ps).max() < 1e-12
+    assert np.abs(Pd.sum(axis=1) - 1).max() < 1e-12           # zero row-sum generator
+    assert np.abs(C @ Pd - C).max() / C.max() < 1e-12         # c^T P = c^T (charge)
+    assert Pd.min() >= 0.0                                     # exp of a Metzler matrix
+    # the generator is the transpose-sensitive part: A^T gives a different propagator
+    A = O.coefficient_matrix(parent, R, C, 0.05).toarray()
+    assert np.abs(scipy.linalg.expm(A.T) - Pd).max() > 1e-3
+    # semigroup: P(dt)^2 = P(2 dt)
+    P2 = O.propagator(parent, R, C, 0.1, prune=0.0).toarray()
+    assert np.abs(Pd @ Pd - P2).max() < 1e-12
+
+
+def test_propagator_pruning_is_reference_rule():
+    parent, coords, diam = _random_tree(80, 2)
+    g = O.geometry(parent, coords, diam)
+    R, C = O.passive(g["length"], g["cross_sectional_area"], g["area_at_creation"])
+    full = O.propagator(parent, R, C, 0.05, prune=0.0).toarray()
+    pr = O.propagator(parent, R, C, 0.05, prune=1e-9).toarray()
+    keep = np.abs(full) >= 1e-9
+    assert np.array_equal(pr != 0, keep)
+    assert np.array_equal(pr[keep], full[keep])
+
+
+# ------------------------------------------------------------------------------------------------
+# HH and LTI
+# ------------------------------------------------------------------------------------------------
+def test_vtrap_continuous_and_rates_positive():
+    v = np.linspace(-100, 100, 20001)
+    for a, b in O.hh_alpha_beta(v).values():
+        assert np.all(a > 0) and np.all(b > 0)
+    x = np.array([-1e-5, -1e-7, 0.0, 1e-7, 1e-5])
+    vals = O.vtrap(x, 10.0)
+    assert np.abs(np.diff(vals)).max() < 1e-5
+
+
+def test_gate_exact_matches_closed_form_expm():
+    for gate in "mhn":
+        for v in (-90.0, -65.0, -40.0, 0.0, 30.0):
+            a, b = (float(z) for z in O.hh_alpha_beta(v)[gate])
+            P = scipy.linalg.expm(np.array([[-a, b], [a, -b]]) * 0.1)
+            for x in (0.0, 0.3, 1.0):
+                assert O.gate_exact(x, v, gate, 0.1) == pytest.approx(P[1, 0] * (1 - x) + P[1, 1] * x,
+                                                                     rel=1e-12, abs=1e-15)
+
+
+def test_two_state_table_is_closed_form():
+    deriv = O.two_state_derivative(O.hh_gate_rate("n"))
+    v = 12.3
+    M = O.lti_matrix(deriv, [v], 2, 0.1)
+    a, b = (float(z) for z in O.hh_alpha_beta(v)["n"])
+    assert np.abs(M - O.two_state_closed_form(a, b, 0.1)).max() < 1e-14
+    assert np.abs(M.sum(axis=0) - 1).max() < 1e-15
+
+
+def test_bucket_location_known_answers():
+    """lti_sim/tests/test_inputs.py:5-19."""
+    b, f = O.bucket_location(4.0, 4.0, 7.0, 17)
+    assert (b, f) == (0, 0.0)
+    b, f = O.bucket_location(7 - 1e-12, 4.0, 7.0, 17)
+    assert b == 16 and f == pytest.approx(1.0)
+    b, f = O.bucket_location(7.5, 4.0, 7.0, 17)
+    assert (b, f) == (16, 1.0)
+    b, f = O.bucket_location(3.0, 4.0, 7.0, 17)     # more than one bucket below: clamped
+    assert (b, f) == (0, 0.0)
+    b, f = O.bucket_location(4.0 - 0.5 * 3 / 17, 4.0, 7.0, 17)   # truncation toward zero
+    assert b == 0 and f == pytest.approx(-0.5)
+
+
+def test_lti_polynomial_table_is_linear_interpolation():
+    scheme_rates = [(0, 1, 0.3, 0.02), (1, 0, 0.1, -0.03), (1, 2, 0.5, 0.01), (2, 1, 0.2, 0.0)]
+
+    def deriv(v, *s):
+        d = [0.0, 0.0, 0.0]
+        for i, j, a, bb in scheme_rates:
+            f = a * math.exp(bb * v) * s[i]
+            d[i] -= f
+            d[j] += f
+        return d
+    node = O.lti_node_table(deriv, 3, 0.1, num_buckets=200)
+    poly = O.node_table_to_polynomial(node)
+    rng = np.random.default_rng(0)
+    v = rng.uniform(-99, 99, 50)
+    st = rng.dirichlet(np.ones(3), 50).T
+    out_loop = O.lti_advance(st, v, poly, conserve_sum=1.0)
+    out_vec = O.lti_advance_vec(st, v, poly, conserve_sum=1.0)
+    assert np.abs(out_loop - out_vec).max() < 1e-15
+    for k in range(50):
+        b, f = O.bucket_location(v[k], -100, 100, 200)
+        M = (1 - f) * node[b] + f * node[b + 1]
+        ref = M @ st[:, k]
+        assert np.abs(ref / ref.sum() - out_loop[:, k]).max() < 1e-14
+    # at a node the table reproduces the exact propagator
+    vn = np.array([-100 + 37 * 1.0])
+    exact = O.lti_matrix(deriv, [vn[0]], 3, 0.1)
+    assert np.abs(O.lti_advance_vec(st[:, :1], vn, poly) [:, 0] - exact @ st[:, 0]).max() < 1e-14
+
+
+def test_golden_hh_trace():
+    """neuwon/rxd/tests/test_model.py:35-128 through the oracle (exact gates)."""
+    golden = np.loadtxt(os.path.join(GOLDEN, "hh_trace_test_model.txt"))
+    parents = [-1] + list(range(10))
+    coords = [np.array([-1.0, 0, 7])] + [np.array([float(x), 0, 7]) for x in range(10)]
+    parent, g, R, C = build(parents, coords, [5.7] + [1.0] * 10)
+    P = O.propagator(parent, R, C, 0.05)
+    mag = g["area_at_creation"]          # every hh instance inserted at creation
+    gbar = {"na": 0.12e-8 * mag, "k": 0.036e-8 * mag, "l": 0.0003e-8 * mag}
+    pulses = []
+
+    def stim(tick):
+        I = np.zeros(11)
+        for t0, nt in pulses:
+            if t0 <= tick < t0 + nt:
+                I[0] += 0.1e-9
+        return I
+    sim = O.HHSim(P, C, gbar, v0=-70.0, dt=0.1, celsius=6.3, stim=stim)
+    trace = [sim.v[-1]]
+    sim.advance()
+    trace.append(sim.v[-1])
+    ticks = 1
+    while ticks * 0.1 < 20:
+        for t in (5, 15):
+            if round(t / 0.1) == ticks:
+                pulses.append((sim.input_ticks + 1, O.pulse_ticks(1.0, 0.05)))
+        sim.advance()
+        ticks += 1
+        trace.append(sim.v[-1])
+    err = np.abs(np.array(trace) - golden)
+    assert len(trace) == 201
+    assert err.max() < 30.0              # the reference test's bar
+    assert err.max() < 3.5               # what the oracle achieves (spike timing shift < 0.02 ms)
+    assert err[:51].max() < 0.2          # relaxation before the first stimulus
+
+
+def test_accumulate_scatter_true_sums():
+    rng = np.random.default_rng(3)
+    seg = rng.integers(0, 20, 200)
+    gi = rng.uniform(0, 1e-9, 200)
+    E = rng.uniform(-90, 60, 200)
+    sg, num = O.accumulate_scatter(20, seg, gi, E)
+    for s in range(20):
+        m = seg == s
+        assert sg[s] == pytest.approx(gi[m].sum(), rel=1e-13, abs=0)
+        assert num[s] == pytest.approx((gi[m] * E[m]).sum(), rel=1e-12, abs=1e-25)
+    d = O.driving_voltage(np.array([0.0, 2.0]), np.array([0.0, -130.0]))
+    assert d[0] == 0.0 and d[1] == -65.0
+
+
+def test_stim_hash_bit_identities():
+    # avalanche: one flipped input bit changes about half of the output bits
+    flips = [bin(int(O.stim_hash(5, 7, t)) ^ int(O.stim_hash(5, 7, t ^ 1))).count("1")
+             for t in range(0, 2000, 2)]
+    assert 14 < np.mean(flips) < 18
+    # onsets are Poisson at the requested rate
+    thr = O.poisson_threshold(5.0, 0.05)
+    hits = sum(int(O.stim_hash(1, c, t)) < thr for c in range(20) for t in range(20000))
+    expect = 20 * 20000 * thr / 2 ** 32
+    assert abs(hits - expect) < 4 * math.sqrt(expect)
+    assert O.pulse_ticks(1.0, 0.05) == 20
+
+
+def test_c_restatement_matches_numpy_oracle():
+    """The C oracle (CPU baseline) and the numpy oracle run the same step to rounding."""
+    import scipy.sparse
+    from neuwon_amd import synth, geometry, channels
+    cells = synth.population(total=800, seed=2, size_range=(100, 300))
+    g = synth.concat(cells)
+    geo = geometry.geometry(g["parent"], g["length"], g["diam"])
+    R, C = geometry.passive(geo["length"], geo["cross_sectional_area"], geo["area_at_creation"])
+    blocks = []
+    for c in range(len(cells)):
+        r0, r1 = g["cell_ptr"][c], g["cell_ptr"][c + 1]
+        blocks.append(O.propagator(cells[c].parent.astype(np.int64), R[r0:r1], C[r0:r1], 0.05,
+                                   prune=1e-9))
+    P = scipy.sparse.block_diag(blocks, format="csr")
+    gbar = {k: channels.HH_GBAR[k] * 1e-8 * geo["surface_area"] for k in ("na", "k", "l")}
+    thr = O.poisson_threshold(200.0, 0.05)
+    for mode in ("table", "exact"):
+        co = O.CHHPopulation(g["cell_ptr"], P, C, gbar, gate_mode=mode,
+                             stim=("poisson", 0.5e-9, 3, thr, 20))
+        co.advance(60, threads=2)
+        for c in range(len(cells)):
+            r0, r1 = g["cell_ptr"][c], g["cell_ptr"][c + 1]
+
+            def stim(t, c=c, n=r1 - r0):
+                I = np.zeros(n)
+                I[0] = 0.5e-9 * O.poisson_pulses_active(3, c, t, thr, 20)
+                return I
+            sim = O.HHSim(blocks[c], C[r0:r1], {k: v[r0:r1] for k, v in gbar.items()},
+                          gate_mode=mode, stim=stim)
+            for _ in range(60):
+                sim.advance()
+            assert np.abs(sim.v - co.v[r0:r1]).max() < 1e-11
+            assert np.abs(sim.gates["n"] - co.gates["n"][r0:r1]).max() < 1e-13
