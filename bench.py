@@ -21,6 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per compartment-step of the step kernel on this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counter runs scaled by what the same counters read for a plain 1 GiB device copy
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
+MEASURED_TRAFFIC_PER_CMPT_STEP = 6890.6
 
 
 def parse():
@@ -33,9 +37,11 @@ def parse():
     ap.add_argument("--launch-steps", type=int, default=0,
                     help="steps per kernel launch (0: all timed steps in one launch)")
     ap.add_argument("--block-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample-cells", type=int, default=12)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-sample-cells", type=int, default=48)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -43,44 +49,59 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pop, cells, args):
-    """FP64 oracle step (scipy CSR SpMV + numpy, one thread) on a bounded sample of the same
-    cells, fed the same pruned FP64 propagator the device built."""
-    import threadpoolctl
+def cpu_baseline(pop, args):
+    """The oracle's C FP64 restatement of the same step (CSR SpMV, table gates, Poisson stimulus),
+    fed the same pruned FP64 propagator the device built, on a bounded sample of the population's
+    cells: one thread (the reference's serial path) and every host core."""
+    import scipy.sparse
     from oracle import neuwon_oracle as O
     from neuwon_amd import population
-    sample = list(range(min(args.cpu_sample_cells, pop.num_cells)))
-    sims = []
-    n_total = 0
-    tables = {}
-    for g in "mhn":
-        deriv = O.two_state_derivative(O.hh_gate_rate(g, pop.celsius))
-        tables[g] = O.node_table_to_polynomial(O.lti_node_table(deriv, 2, pop.dt))
-    for c in sample:
+    cores = len(os.sched_getaffinity(0))
+    sample = min(args.cpu_sample_cells, pop.num_cells)
+    r_end = int(pop.cell_ptr[sample])
+    blocks = []
+    for c in range(sample):
         r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
-        P = population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1], pop.C[r0:r1],
-                                           0.5 * pop.dt, pop.propagator.prune)
-        gbar = {k: pop.g_bar[k][r0:r1] for k in ("na", "k", "l")}
-        sim = O.HHSim(P, pop.C[r0:r1], gbar, v0=-70.0, dt=pop.dt, celsius=pop.celsius,
-                      gate_mode="table", tables=tables)
-        sims.append(sim)
-        n_total += r1 - r0
-    with threadpoolctl.threadpool_limits(1):
-        for s in sims:
-            s.advance()  # warm-up
-        steps = 0
-        t0 = time.perf_counter()
+        blocks.append(population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1],
+                                                      pop.C[r0:r1], 0.5 * pop.dt,
+                                                      pop.propagator.prune))
+    P = scipy.sparse.block_diag(blocks, format="csr")
+    p = pop.params
+    stim = ("poisson", float(p.stim_amp_a), int(p.stim_seed), int(p.poisson_threshold),
+            int(p.pulse_ticks))
+    gbar = {k: pop.g_bar[k][:r_end] for k in ("na", "k", "l")}
+    rates = {}
+    for threads in (1, cores):
+        co = O.CHHPopulation(pop.cell_ptr[:sample + 1], P, pop.C[:r_end], gbar, dt=pop.dt,
+                             celsius=pop.celsius, gate_mode="table", stim=stim)
+        co.advance(1, threads=threads)   # warm-up
+        steps, t0 = 0, time.perf_counter()
         while True:
-            for s in sims:
-                s.advance()
+            co.advance(1, threads=threads)
             steps += 1
             el = time.perf_counter() - t0
-            if el >= args.cpu_seconds or steps >= 1000:
+            if el >= args.cpu_seconds / 2 or steps >= 2000:
                 break
-    rate = n_total * steps / el
-    return {"value": rate, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{len(sample)} cells / {n_total} compartments of the same population, "
-                      f"{steps} steps, FP64 scipy CSR SpMV + numpy (oracle), 1 thread"}
+        rates[threads] = (r_end * steps / el, steps)
+        log(f"cpu baseline {threads} thread(s): {rates[threads][0] / 1e6:.2f} M cmpt-steps/s")
+    return {"value": rates[cores][0], "unit": "compartment-steps/s", "cores": cores, "kind": "port",
+            "value_1_thread": rates[1][0],
+            "sample": f"first {sample} cells ({r_end} compartments, {P.nnz} FP64 propagator entries) "
+                      f"of the same population; {rates[1][1]} steps on 1 thread, {rates[cores][1]} "
+                      f"steps on {cores} threads; oracle C FP64 restatement (OpenMP over cells)"}
+
+
+def copy_probe(args):
+    """A plain device-to-device copy of known size: the calibration run for rocprof's
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = int(args.copy_probe_mib) * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 3}), flush=True)
 
 
 def main():
@@ -90,6 +111,9 @@ def main():
               "independent, a multi-GPU run would shard whole cells with no per-step exchange)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mib:
+        copy_probe(args)
+        return
     import torch
     from neuwon_amd import population, synth
     t0 = time.time()
@@ -152,14 +176,15 @@ def main():
                    "nnz_per_row": round(nnz_bar, 2), "steps_per_launch": L,
                    "propagator_build_s": round(st["build_seconds"], 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC_PER_CMPT_STEP * N * (K / launches),
                      "kernel": "nw::k_hh_step", "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_cmpt_step": b_step,
                      "stored_propagator_bytes_per_cmpt_step": stored_bytes_step},
     }
     log(f"timed {K} steps: {wall:.3f}s wall, kernel {kern_ms:.1f} ms, {value / 1e9:.3f} G cmpt-steps/s")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pop, cells, args)
+        out["cpu_baseline"] = cpu_baseline(pop, args)
     print(json.dumps(out), flush=True)
 
 

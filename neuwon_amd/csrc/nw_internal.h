@@ -34,9 +34,9 @@ void set_error(const char* fmt, ...);
 
 constexpr int kWave = 64;
 
-// Voltages are held in LDS as FP32 offsets from this reference (mV): the offset keeps the FP32
-// rounding proportional to the distance from rest instead of to the full membrane potential.
-constexpr double kVRef = -65.0;
+// Voltages are held in LDS as FP32 offsets from a per-cell reference voltage (the first segment's
+// value when the launch starts): the FP32 rounding is then proportional to the spread of voltages
+// within the cell instead of to the membrane potential itself.
 
 // Sum across the 64 lanes of a wavefront; every lane receives the total.
 __device__ __forceinline__ float wave_sum(float x) {
@@ -89,17 +89,17 @@ __device__ __forceinline__ double row_consume(const RowChunk& ch, const float* _
     return acc;
 }
 
-// out[r] = (P x)_r - kVRef for the rows r = first, first + stride, ... of one cell, one wavefront
-// per row.  x is stored in LDS as FP32 offsets from kVRef, products accumulate in FP64 and the
+// out[r] = (P x)_r - vref for the rows r = first, first + stride, ... of one cell, one wavefront
+// per row.  x is stored in LDS as FP32 offsets from vref, products accumulate in FP64 and the
 // reference part is restored with the exact FP64 sum of the stored row:
-//     P x = kVRef * rowsum + P (x - kVRef).
+//     P x = vref * rowsum + P (x - vref).
 // The next row's offsets and first chunk are fetched before the current row is reduced.
 __device__ __forceinline__ void spmv_rows(const uint32_t* __restrict__ roff,
                                           const float* __restrict__ vals,
                                           const uint16_t* __restrict__ cols,
                                           const double* __restrict__ rowsum,
                                           const float* __restrict__ xoff, float* __restrict__ out,
-                                          int nrows, int first, int stride, int lane) {
+                                          double vref, int nrows, int first, int stride, int lane) {
     int r = first;
     if (r >= nrows) return;
     uint32_t beg = roff[r], end = roff[r + 1];
@@ -119,8 +119,8 @@ __device__ __forceinline__ void spmv_rows(const uint32_t* __restrict__ roff,
             acc = row_consume(ch, xoff, acc);
         }
         if (rn < nrows) row_load(ch, vals, cols, nbeg, nend, lane);
-        const double s = fma(kVRef, rowsum[r], wave_sum(acc));
-        if (lane == 0) out[r] = (float)(s - kVRef);
+        const double s = fma(vref, rowsum[r], wave_sum(acc));
+        if (lane == 0) out[r] = (float)(s - vref);
         if (rn >= nrows) break;
         r = rn;
         beg = nbeg;

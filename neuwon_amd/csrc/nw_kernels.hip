@@ -111,12 +111,12 @@ __device__ __forceinline__ void hh_conductance(double gna_bar, double gk_bar, do
 }
 
 // Every per-segment quantity is computed in FP64 registers from FP32 storage; voltages live in LDS
-// as FP32 offsets from kVRef and the row products accumulate in FP64 (spmv_rows).
+// as FP32 offsets from the cell's reference voltage and the row products accumulate in FP64 (spmv_rows).
 template <int BLOCK, int GATE_MODE>
 __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
     extern __shared__ float lds[];
-    float* xs = lds;            // x - kVRef of the current half step
-    float* vs = lds + a.nmax;   // v - kVRef
+    float* xs = lds;            // x - vref of the current half step
+    float* vs = lds + a.nmax;   // v - vref
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -146,7 +146,8 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         ap_count = a.ap_count[cell];
     }
 
-    for (int i = threadIdx.x; i < nrows; i += BLOCK) vs[i] = (float)((double)a.v[row0 + i] - kVRef);
+    const double vref = (double)a.v[row0];  // the cell's reference voltage for this launch
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) vs[i] = (float)((double)a.v[row0 + i] - vref);
     __syncthreads();
 
     for (int step = 0; step < a.n_steps; ++step) {
@@ -157,23 +158,23 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
             const double invc = a.inv_c[gi];
-            const double v = kVRef + (double)vs[i];
+            const double v = vref + (double)vs[i];
             double g = 0.0, d = 0.0;
             if (g_valid)
                 hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], a.m[gi], a.h[gi], a.n[gi], e_na,
                                e_k, e_l, g, d);
             const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick_a) : 0.0;
             const double alpha = exp(-hdt_s * g * invc);
-            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - kVRef);
+            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - vref);
         }
         __syncthreads();
         // ---- B: v = P x
-        spmv_rows(roff, vals, cols, rs, xs, vs, nrows, wave, NW, lane);
+        spmv_rows(roff, vals, cols, rs, xs, vs, vref, nrows, wave, NW, lane);
         __syncthreads();
         // ---- C: mechanisms at v(t + dt/2), then relax for the second half step
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
-            const double v = kVRef + (double)vs[i];
+            const double v = vref + (double)vs[i];
             double m = a.m[gi], h = a.h[gi], n = a.n[gi];
             if (GATE_MODE == 1) {
                 hh_exact(v, (double)p.q10, (double)p.dt_ms, m, h, n);
@@ -195,21 +196,21 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
             const double I =
                 (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick_a + 1) : 0.0;
             const double alpha = exp(-hdt_s * g * invc);
-            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - kVRef);
+            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - vref);
         }
         __syncthreads();
         // ---- D: v = P x
-        spmv_rows(roff, vals, cols, rs, xs, vs, nrows, wave, NW, lane);
+        spmv_rows(roff, vals, cols, rs, xs, vs, vref, nrows, wave, NW, lane);
         __syncthreads();
         if (threadIdx.x == 0 && ap_row >= 0) {
-            const double vap = kVRef + (double)vs[ap_row];
+            const double vap = vref + (double)vs[ap_row];
             const bool over = vap >= (double)p.ap_threshold;
             if (over && !ap_state) ++ap_count;
             ap_state = over ? 1 : 0;
             if (a.trace) a.trace[(int64_t)cell * a.trace_stride + step] = (float)vap;
         }
     }
-    for (int i = threadIdx.x; i < nrows; i += BLOCK) a.v[row0 + i] = (float)(kVRef + (double)vs[i]);
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) a.v[row0 + i] = (float)(vref + (double)vs[i]);
     if (threadIdx.x == 0 && ap_row >= 0) {
         a.ap_state[cell] = ap_state;
         a.ap_count[cell] = ap_count;

@@ -427,28 +427,36 @@ struct HalfArgs {
     float* integral_v;
 };
 
-// FP32 storage, FP64 arithmetic: x is formed in FP64 and staged in LDS as an FP32 offset from
-// kVRef; the row products accumulate in FP64 (spmv_rows).
+// FP32 storage, FP64 arithmetic: x is formed in FP64 and staged in LDS as an FP32 offset from the
+// cell's reference voltage (x of its first segment); the row products accumulate in FP64
+// (spmv_rows).
+__device__ __forceinline__ double half_x(const HalfArgs& a, int64_t gi, double dt_ms, double dt_s,
+                                         bool write_integral) {
+    const double invc = a.inv_c[gi];
+    const double d = a.drive_v[gi];
+    const double v = a.v[gi];
+    const double dv = dt_ms * (double)a.sum_i[gi] * invc;
+    const double expo = -dt_s * (double)a.sum_g[gi] * invc;
+    const double alpha = exp(expo);
+    const double diff = d - v;
+    if (write_integral && a.integral_v)
+        a.integral_v[gi] = (float)(dt_s * d - expo * diff * alpha + 0.5 * dt_s * dv);
+    return d - diff * alpha + dv;
+}
+
 __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
     extern __shared__ float xs[];
+    __shared__ double s_vref;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int n = (int)(a.cell_row0[cell + 1] - row0);
     const double dt_ms = a.half_dt_ms;
     const double dt_s = dt_ms * 1e-3;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int64_t gi = row0 + i;
-        const double invc = a.inv_c[gi];
-        const double d = a.drive_v[gi];
-        const double v = a.v[gi];
-        const double dv = dt_ms * (double)a.sum_i[gi] * invc;
-        const double expo = -dt_s * (double)a.sum_g[gi] * invc;
-        const double alpha = exp(expo);
-        const double diff = d - v;
-        xs[i] = (float)((d - diff * alpha + dv) - kVRef);
-        if (a.integral_v)
-            a.integral_v[gi] = (float)(dt_s * d - expo * diff * alpha + 0.5 * dt_s * dv);
-    }
+    if (threadIdx.x == 0) s_vref = half_x(a, row0, dt_ms, dt_s, false);
+    __syncthreads();
+    const double vref = s_vref;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        xs[i] = (float)(half_x(a, row0 + i, dt_ms, dt_s, true) - vref);
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -457,9 +465,9 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
     const float* vals = a.vals + a.cell_nz0[cell];
     const uint16_t* cols = a.cols + a.cell_nz0[cell];
     float* vout = xs + a.nmax;
-    spmv_rows(roff, vals, cols, a.rowsum + row0, xs, vout, n, wave, nwaves, lane);
+    spmv_rows(roff, vals, cols, a.rowsum + row0, xs, vout, vref, n, wave, nwaves, lane);
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) a.v[row0 + i] = (float)(kVRef + (double)vout[i]);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a.v[row0 + i] = (float)(vref + (double)vout[i]);
 }
 
 }  // namespace nw

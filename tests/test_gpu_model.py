@@ -34,7 +34,7 @@ def test_time_constant(torch_gpu):
     for _ in range(nstep):
         m.Segment._advance_electric(dt)
     _say(f"time constant: v(tau) = {root.voltage!r}")
-    assert root.voltage == pytest.approx(1 - 1 / np.e, rel=2e-6)
+    assert root.voltage == pytest.approx(1 - 1 / np.e, rel=1e-6)
     m.check()
 
 
@@ -166,8 +166,10 @@ def test_insert_api(torch_gpu):
         m.advance()
     m.check()
     g = m.database.get_data("Segment.sum_conductance")
-    assert g[segs[4].get_unstable_index()] > 1.9 * g[segs[3].get_unstable_index()] * \
-        segs[3].surface_area / segs[4].surface_area
+    # two hh instances on segs[4]: twice the conductance per unit area of segs[3]
+    ratio = (g[segs[4].get_unstable_index()] / segs[4].surface_area) / \
+        (g[segs[3].get_unstable_index()] / segs[3].surface_area)
+    assert 1.8 < ratio < 2.2, ratio
 
 
 def test_lti_mechanism_api(torch_gpu):
