@@ -178,6 +178,35 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
                const int32_t* ap_local, uint8_t* ap_state, uint32_t* ap_count, float* trace,
                int64_t trace_stride, void* stream);
 
+/* ------------------------------------------------------------------------------------------
+ * Intracellular Ca2+ with a first-order buffer and diffusion (replaces _SpeciesInstance._advance,
+ * species.py:54-70, whose diffusion matrix is a stub at species.py:74-75, together with a buffer
+ * mechanism writing the Ca derivative accumulator, nmodl/__init__.py:204-207).  `p` is the
+ * diffusion propagator of the half step (built by nw_propagator_build with R = 1e-3/k,
+ * k = D*xarea/L per edge, and C = volume).  Per step and cell:
+ *   c = P max(0, c + dcdt dt/2);  CaB' = inf + (CaB - inf) exp(-(kon c + koff) dt),
+ *   inf = btot kon c/(kon c + koff);  dcdt = -(CaB' - CaB)/dt;  c = P max(0, c + dcdt dt/2).
+ * Arrays FP32 per compartment (any concentration unit consistent with kon); `cab` holds the bound
+ * buffer minus cab_ref (so its small per-step changes keep FP32 precision next to free Ca).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nw_ca_params {
+    float   dt_ms;
+    float   kon;      /* per concentration unit per ms */
+    float   koff;     /* per ms */
+    float   btot;     /* total buffer, concentration unit */
+    double  cab_ref;  /* reference level of the bound buffer */
+    int32_t block_threads;
+    int32_t pad_;
+} nw_ca_params;
+
+int nw_ca_step(const nw_propagator* p, const nw_ca_params* prm, int32_t n_steps, float* ca,
+               float* cab, float* dcdt, void* stream);
+
+/* partials[b] = FP64 sum over block b's share of w[i] * (x[i] + y[i] + z_scale * z[i]) (y, z may
+ * be NULL); the caller sums the num_partials values (mass conservation checks). */
+int nw_weighted_sum(int64_t n, const double* w, const float* x, const float* y, const float* z,
+                    double z_scale, double* partials, int32_t num_partials, void* stream);
+
 /* Stimulus generator, the same hash the step kernel evaluates (exposed for checks). */
 uint32_t nw_stim_hash(uint32_t seed, uint32_t cell, uint32_t tick);
 

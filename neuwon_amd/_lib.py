@@ -29,6 +29,11 @@ class NwHHParams(ctypes.Structure):
                 ("pad_", c_i32)]
 
 
+class NwCaParams(ctypes.Structure):
+    _fields_ = [("dt_ms", c_f32), ("kon", c_f32), ("koff", c_f32), ("btot", c_f32),
+                ("cab_ref", c_f64), ("block_threads", c_i32), ("pad_", c_i32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "nw_last_error": (ctypes.c_char_p, []),
@@ -50,6 +55,8 @@ SIGNATURES = {
     "nw_hh_step": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nw_stim_hash": (c_u32, [c_u32, c_u32, c_u32]),
+    "nw_ca_step": (c_i32, [c_vp, ctypes.POINTER(NwCaParams), c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "nw_weighted_sum": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_f64, c_vp, c_i32, c_vp]),
     "nw_hh_mechanism": (c_i32, [c_i64, c_i32, c_f32, c_f32, c_vp, c_i32, c_f32, c_f32, c_vp, c_vp,
                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
 }

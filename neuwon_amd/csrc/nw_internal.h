@@ -128,6 +128,49 @@ __device__ __forceinline__ void spmv_rows(const uint32_t* __restrict__ roff,
     }
 }
 
+// Same product with one thread per row, for propagators whose rows hold a few entries (the
+// diffusion propagator): rows are padded to kRowAlign, so every load is a 16-B / 8-B vector load.
+__device__ __forceinline__ void spmv_rows_lane(const uint32_t* __restrict__ roff,
+                                               const float* __restrict__ vals,
+                                               const uint16_t* __restrict__ cols,
+                                               const double* __restrict__ rowsum,
+                                               const float* __restrict__ xoff,
+                                               float* __restrict__ out, double vref, int nrows,
+                                               int tid, int nthreads) {
+    for (int r = tid; r < nrows; r += nthreads) {
+        const uint32_t beg = roff[r], end = roff[r + 1];
+        double acc = 0.0;
+        for (uint32_t k = beg; k < end; k += kRowAlign) {
+            const float4 a = *reinterpret_cast<const float4*>(vals + k);
+            const uint2 c = *reinterpret_cast<const uint2*>(cols + k);
+            acc = fma((double)a.x, (double)xoff[c.x & 0xFFFFu], acc);
+            acc = fma((double)a.y, (double)xoff[c.x >> 16], acc);
+            acc = fma((double)a.z, (double)xoff[c.y & 0xFFFFu], acc);
+            acc = fma((double)a.w, (double)xoff[c.y >> 16], acc);
+        }
+        out[r] = (float)(fma(vref, rowsum[r], acc) - vref);
+    }
+}
+
+// Rows of at most this many stored entries use one thread per row.
+constexpr int kLaneRowMax = 32;
+
+template <int ROWMODE>
+__device__ __forceinline__ void spmv(const uint32_t* __restrict__ roff,
+                                     const float* __restrict__ vals,
+                                     const uint16_t* __restrict__ cols,
+                                     const double* __restrict__ rowsum,
+                                     const float* __restrict__ xoff, float* __restrict__ out,
+                                     double vref, int nrows) {
+    if (ROWMODE == 1) {
+        spmv_rows_lane(roff, vals, cols, rowsum, xoff, out, vref, nrows, threadIdx.x, blockDim.x);
+    } else {
+        const int lane = threadIdx.x & (kWave - 1);
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+        spmv_rows(roff, vals, cols, rowsum, xoff, out, vref, nrows, wave, blockDim.x / kWave, lane);
+    }
+}
+
 // Location of `value` in a linear input table, reference semantics of the generated LTI kernel
 // (lti_sim/codegen.py:44-59): u = (value - vmin) * nb / (vmax - vmin); b = trunc(u);
 // b > nb-1 -> (nb-1, 1); b < 0 -> (0, 0); otherwise frac = u - b.

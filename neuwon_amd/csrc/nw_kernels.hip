@@ -332,6 +332,93 @@ __global__ void k_accumulate(int64_t num_segments, const int64_t* __restrict__ p
     sum_i[s] = nonspec_i ? nonspec_i[s] : 0.0f;
 }
 
+// ----------------------------------------------------------------------------------------------
+// Intracellular Ca2+ with a first-order buffer and diffusion (species.py:54-70 with a diffusion
+// propagator; buffer B <-> CaB, forward kon*Ca, backward koff, CONSERVE B + CaB = btot).
+// Per step: c = P_D max(0, c + dcdt dt/2); buffer over dt at c(t+dt/2), dcdt = -(CaB' - CaB)/dt;
+// c = P_D max(0, c + dcdt dt/2).  One workgroup per cell, concentrations stay in LDS.  The bound
+// buffer is stored as an FP32 offset from the scalar cab_ref: it is large next to free Ca, and its
+// per-step change must keep the precision of the free concentration.
+// ----------------------------------------------------------------------------------------------
+struct CaArgs {
+    const int64_t* cell_row0;
+    const int64_t* cell_nz0;
+    const uint32_t* row_off;
+    const float* vals;
+    const uint16_t* cols;
+    const double* rowsum;
+    const int32_t* order;
+    int32_t nmax;
+    int32_t n_steps;
+    nw_ca_params prm;
+    float* ca;
+    float* cab;
+    float* dcdt;
+};
+
+template <int BLOCK, int ROWMODE>
+__global__ void __launch_bounds__(BLOCK) k_ca_step(CaArgs a) {
+    extern __shared__ float lds[];
+    float* xs = lds;
+    float* cs = lds + a.nmax;
+    const int cell = a.order[blockIdx.x];
+    const int64_t row0 = a.cell_row0[cell];
+    const int nrows = (int)(a.cell_row0[cell + 1] - row0);
+    const uint32_t* roff = a.row_off + row0 + cell;
+    const float* vals = a.vals + a.cell_nz0[cell];
+    const uint16_t* cols = a.cols + a.cell_nz0[cell];
+    const double* rs = a.rowsum + row0;
+    const double dt = a.prm.dt_ms, hdt = 0.5 * dt;
+    const double kon = a.prm.kon, koff = a.prm.koff, btot = a.prm.btot;
+    const double cab_ref = a.prm.cab_ref;
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) cs[i] = a.ca[row0 + i];
+    __syncthreads();
+    for (int step = 0; step < a.n_steps; ++step) {
+        for (int i = threadIdx.x; i < nrows; i += BLOCK)
+            xs[i] = (float)fmax(0.0, (double)cs[i] + (double)a.dcdt[row0 + i] * hdt);
+        __syncthreads();
+        spmv<ROWMODE>(roff, vals, cols, rs, xs, cs, 0.0, nrows);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+            const int64_t gi = row0 + i;
+            const double c = cs[i];
+            const float stored = a.cab[gi];
+            const double cab = cab_ref + (double)stored;
+            const double fw = kon * c, s = fw + koff;
+            const double inf = btot * fw / s;
+            const float stored_new = (float)(inf + (cab - inf) * exp(-s * dt) - cab_ref);
+            // the Ca change is the change of the stored bound amount: mass stays exact
+            const double d = -((double)stored_new - (double)stored) / dt;
+            a.cab[gi] = stored_new;
+            a.dcdt[gi] = (float)d;
+            xs[i] = (float)fmax(0.0, c + d * hdt);
+        }
+        __syncthreads();
+        spmv<ROWMODE>(roff, vals, cols, rs, xs, cs, 0.0, nrows);
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) a.ca[row0 + i] = cs[i];
+}
+
+// out[b] = sum over this block's slice of w * (x + y + z_scale * z), FP64 (mass checks).
+__global__ void k_weighted_sum(int64_t n, const double* __restrict__ w, const float* __restrict__ x,
+                               const float* __restrict__ y, const float* __restrict__ z,
+                               double z_scale, double* __restrict__ out) {
+    __shared__ double part[256 / kWave];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        acc += w[i] * ((double)x[i] + (y ? (double)y[i] : 0.0) + (z ? z_scale * (double)z[i] : 0.0));
+    acc = wave_sum(acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < (int)(blockDim.x / kWave); ++k) t += part[k];
+        out[blockIdx.x] = t;
+    }
+}
+
 // AP detector of every neuron (neuron.py:72-76): over = v[seg] >= threshold;
 // detected = over and not previously over; state = over.
 __global__ void k_ap_detect(int64_t num, const int32_t* __restrict__ seg,
@@ -433,6 +520,43 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
         default: NW_REQUIRE(false, "state count %d not supported (1..16)", S);
     }
 #undef NW_LTI_CASE
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_ca_step(const nw_propagator* p, const nw_ca_params* prm, int32_t n_steps, float* ca,
+               float* cab, float* dcdt, void* stream) {
+    NW_REQUIRE(p && prm && ca && cab && dcdt, "nw_ca_step: null argument");
+    NW_REQUIRE(n_steps >= 0 && prm->dt_ms > 0.0f, "bad step count or time step");
+    if (n_steps == 0) return NW_OK;
+    CaArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
+             p->max_cell_rows, n_steps, *prm, ca, cab, dcdt};
+    const size_t lds = 2 * sizeof(float) * (size_t)p->max_cell_rows;
+    NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
+    const int bt = prm->block_threads ? prm->block_threads : 256;
+    const unsigned grid = (unsigned)p->num_cells;
+    hipStream_t s = (hipStream_t)stream;
+    const bool lane_rows = p->max_row_nnz <= kLaneRowMax;
+#define NW_CA_LAUNCH(B)                                                                   \
+    case B:                                                                               \
+        if (lane_rows) k_ca_step<B, 1><<<grid, B, lds, s>>>(a);                           \
+        else k_ca_step<B, 0><<<grid, B, lds, s>>>(a);                                     \
+        break;
+    switch (bt) {
+        NW_CA_LAUNCH(64) NW_CA_LAUNCH(128) NW_CA_LAUNCH(256) NW_CA_LAUNCH(512)
+        default: NW_REQUIRE(false, "block_threads must be 64, 128, 256 or 512 (got %d)", bt);
+    }
+#undef NW_CA_LAUNCH
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_weighted_sum(int64_t n, const double* w, const float* x, const float* y, const float* z,
+                    double z_scale, double* partials, int32_t num_partials, void* stream) {
+    NW_REQUIRE(n >= 0 && w && x && partials && num_partials >= 1 && num_partials <= 65536,
+               "nw_weighted_sum: bad argument");
+    k_weighted_sum<<<(unsigned)num_partials, 256, 0, (hipStream_t)stream>>>(n, w, x, y, z, z_scale,
+                                                                            partials);
     NW_HIP(hipGetLastError());
     return NW_OK;
 }

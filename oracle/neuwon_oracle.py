@@ -546,3 +546,84 @@ class CHHPopulation:
             ctypes.c_int32(n_steps), ctypes.c_int32(threads))
         assert rc == 0
         self.steps += n_steps
+
+
+# ---------------------------------------------------------------------------------------------
+# Intracellular species: concentration update, diffusion, first-order buffer
+# ---------------------------------------------------------------------------------------------
+def diffusion_coupling(parent, length, xarea, D):
+    """Per edge (child c, parent p) the diffusive conductance k = D xarea_c / L_c (um^3/ms): the flux
+    through the child's cross-section over its length.  The reference's matrix is a stub
+    (species.py:74-75, extracellular.py:52-53); the layout follows the electric matrix
+    (electric.py:92-108) with 1/(r C) replaced by k / vol."""
+    parent = np.asarray(parent)
+    k = np.zeros(len(parent))
+    child = parent != NULL
+    k[child] = D * np.asarray(xarea)[child] / np.asarray(length)[child]
+    return k
+
+
+def diffusion_matrix(parent, k, vol, time_step_ms):
+    """Sparse A_D dt with A[c,p] = k/vol_c, A[c,c] = -k/vol_c, A[p,c] = k/vol_p, A[p,p] -= k/vol_p."""
+    src, dst, coef = [], [], []
+    for c, p in enumerate(parent):
+        if p == NULL:
+            continue
+        src += [c, c, p, p]
+        dst += [p, c, c, p]
+        coef += [time_step_ms * k[c] / vol[p], -time_step_ms * k[c] / vol[c],
+                 time_step_ms * k[c] / vol[c], -time_step_ms * k[c] / vol[p]]
+    n = len(parent)
+    return scipy.sparse.csc_matrix((coef, (dst, src)), shape=(n, n), dtype=np.float64)
+
+
+def diffusion_propagator(parent, k, vol, time_step_ms, prune=EPSILON):
+    A = diffusion_matrix(parent, k, vol, time_step_ms)
+    P = scipy.sparse.csr_matrix(scipy.linalg.expm(A.toarray()))
+    P.data[np.abs(P.data) < prune] = 0.0
+    P.eliminate_zeros()
+    return P
+
+
+def buffer_exact(cab, ca, dt, kon, koff, btot):
+    """Two-state buffer B <-> CaB with forward rate kon*Ca (Ca held over the step) and backward rate
+    koff, CONSERVE B + CaB = btot, solved exactly (solver.py:48-55 applied to
+    dCaB/dt = kon Ca (btot - CaB) - koff CaB)."""
+    a = kon * ca
+    s = a + koff
+    inf = btot * a / s
+    return inf + (cab - inf) * np.exp(-s * dt)
+
+
+class CaSim:
+    """Ca2+ with a first-order buffer in every compartment and intracellular diffusion.
+
+    The step follows the staggered order of rxd_model.py:63-123 with the species update of
+    species.py:54-70 in each half step: c = max(0, c + dc/dt * dt/2); c = P_D c.  The buffer is a
+    mechanism: it reads Ca at t + dt/2, advances CaB over dt and writes dc/dt = -(CaB' - CaB)/dt
+    (the accumulator ``{ion}_derivative``, nmodl/__init__.py:204-207), zeroed before mechanisms run
+    (species.py:49-52)."""
+
+    def __init__(self, P_half, vol, ca0, cab0, dt=0.1, kon=0.1, koff=0.01, btot=100.0):
+        self.P = P_half
+        self.vol = np.asarray(vol, dtype=np.float64)
+        self.ca = np.asarray(ca0, dtype=np.float64).copy()
+        self.cab = np.asarray(cab0, dtype=np.float64).copy()
+        self.dcdt = np.zeros_like(self.ca)
+        self.dt, self.kon, self.koff, self.btot = dt, kon, koff, btot
+
+    def _half(self):
+        c = np.maximum(0.0, self.ca + self.dcdt * (0.5 * self.dt))
+        self.ca = self.P.dot(c)
+
+    def advance(self):
+        self._half()
+        new = buffer_exact(self.cab, self.ca, self.dt, self.kon, self.koff, self.btot)
+        self.dcdt = -(new - self.cab) / self.dt
+        self.cab = new
+        self._half()
+
+    def mass(self):
+        """Conserved total: the last buffer change is paid back in Ca over two half steps, the
+        second of them in the next step, hence the pending dcdt dt/2."""
+        return float(np.sum(self.vol * (self.ca + self.cab + 0.5 * self.dt * self.dcdt)))

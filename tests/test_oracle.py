@@ -364,3 +364,53 @@ def test_c_restatement_matches_numpy_oracle():
                 sim.advance()
             assert np.abs(sim.v - co.v[r0:r1]).max() < 1e-11
             assert np.abs(sim.gates["n"] - co.gates["n"][r0:r1]).max() < 1e-13
+
+
+# ------------------------------------------------------------------------------------------------
+# species diffusion and buffer (the reference's diffusion is a stub, species.py:74-75)
+# ------------------------------------------------------------------------------------------------
+def test_diffusion_gaussian_spread():
+    """A point release on a long uniform cable spreads with variance 2 D t."""
+    n, L, d, D = 801, 1.0, 1.0, 0.6
+    parent = np.arange(-1, n - 1)
+    length = np.full(n, L)
+    xarea = np.full(n, 0.25 * np.pi * d * d)
+    vol = xarea * L
+    k = O.diffusion_coupling(parent, length, xarea, D)
+    P = O.diffusion_propagator(parent, k, vol, 0.05, prune=0.0)
+    c = np.zeros(n)
+    c[n // 2] = 1.0
+    steps = 400                      # 20 ms
+    for _ in range(steps):
+        c = P.dot(c)
+    x = (np.arange(n) - n // 2) * L
+    var = np.sum(c * x * x) / np.sum(c)
+    assert var == pytest.approx(2 * D * steps * 0.05, rel=0.02)
+    assert np.sum(c) == pytest.approx(1.0, rel=1e-12)
+
+
+def test_diffusion_conserves_mass_and_uniform():
+    parent, coords, diam = _random_tree(70, 4)
+    g = O.geometry(parent, coords, diam)
+    k = O.diffusion_coupling(parent, g["length"], g["cross_sectional_area"], 0.6)
+    P = O.diffusion_propagator(parent, k, g["volume"], 0.05, prune=0.0).toarray()
+    assert np.abs(P.sum(axis=1) - 1).max() < 1e-13
+    assert np.abs(g["volume"] @ P - g["volume"]).max() / g["volume"].max() < 1e-13
+    assert P.min() >= 0
+
+
+def test_buffer_equilibrium_and_mass():
+    kon, koff, btot = 0.1, 0.01, 100.0
+    ca, cab = np.array([1.0]), np.array([0.0])
+    vol = np.array([3.0])
+    m0 = np.sum(vol * (ca + cab))
+    sim = O.CaSim(scipy.sparse.identity(1, format="csr"), vol, ca, cab, kon=kon, koff=koff, btot=btot)
+    for _ in range(20000):
+        sim.advance()
+    assert sim.mass() == pytest.approx(m0, rel=1e-12)
+    assert sim.cab[0] == pytest.approx(btot * sim.ca[0] / (sim.ca[0] + koff / kon), rel=1e-9)
+    # the exact two-state update equals the closed-form expm of its rate matrix
+    a, b = kon * 0.7, koff
+    P = scipy.linalg.expm(np.array([[-a, b], [a, -b]]) * 0.1)
+    got = O.buffer_exact(np.array([30.0]), 0.7, 0.1, kon, koff, btot)[0]
+    assert got == pytest.approx(P[1, 0] * (btot - 30.0) + P[1, 1] * 30.0, rel=1e-13)
