@@ -152,6 +152,7 @@ int nw_hh_mechanism(int64_t M, int32_t gate_mode, float dt_ms, float q10, const 
  * AP detector (neuron.py:72-76) on segment ap_local of each cell: ap_state (u8 per cell) holds
  *   the over-threshold flag, ap_count (u32 per cell) counts detections.
  * trace (may be NULL): trace[cell * trace_stride + step] = v of segment ap_local after the step.
+ * record (may be NULL): see nw_hh_adjoint; (5 n_steps + 4) arrays of num_rows floats.
  * ------------------------------------------------------------------------------------------ */
 typedef struct nw_hh_params {
     float   dt_ms;
@@ -176,7 +177,25 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
                const float* gate_table, float* v, float* m, float* h, float* n, const float* g_na,
                const float* g_k, const float* g_l, const float* inv_c, const int32_t* stim_local,
                const int32_t* ap_local, uint8_t* ap_state, uint32_t* ap_count, float* trace,
-               int64_t trace_stride, void* stream);
+               int64_t trace_stride, float* record, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Reverse pass of the fused step (no counterpart in the reference, which has no differentiation:
+ * it is the adjoint of nw_hh_step with table gates and a constant stimulus).  `record` is the
+ * record nw_hh_step wrote for the same n_steps (num_rows-long arrays v, m, h, n at the start of
+ * every step and v after its first half step, 5 per step, then v, m, h, n after the last step).
+ * Loss: sum over the steps of (v[soma_local[cell]] after the step - target[cell*stride + step])^2.
+ * lam_v/lam_m/lam_h/lam_n (FP32, per segment) carry the adjoint of the state from the following
+ * launch (zeros before the last one); grad_na/k/l (FP64, per segment) and grad_theta (FP64,
+ * [cell][6]: d/dt_alpha, d/dt_beta for m, h, n) and loss (FP64 per cell) are accumulated.
+ * table6: [3 gates][num_nodes][6] = a, b, da/dt_alpha, db/dt_alpha, da/dt_beta, db/dt_beta.
+ * ------------------------------------------------------------------------------------------ */
+int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
+                  const float* record, const float* table6, const float* g_na, const float* g_k,
+                  const float* g_l, const float* inv_c, const int32_t* soma_local,
+                  const float* target, int64_t target_stride, float* lam_v, float* lam_m,
+                  float* lam_h, float* lam_n, double* grad_na, double* grad_k, double* grad_l,
+                  double* grad_theta, double* loss, void* stream);
 
 /* ------------------------------------------------------------------------------------------
  * Intracellular Ca2+ with a first-order buffer and diffusion (replaces _SpeciesInstance._advance,

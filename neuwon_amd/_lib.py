@@ -53,7 +53,10 @@ SIGNATURES = {
     "nw_lti_advance": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
                                c_f32, c_i32, c_vp, c_vp, c_vp, c_vp]),
     "nw_hh_step": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
-                           c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+                           c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "nw_hh_adjoint": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp,
+                              c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                              c_vp, c_vp, c_vp, c_vp]),
     "nw_stim_hash": (c_u32, [c_u32, c_u32, c_u32]),
     "nw_ca_step": (c_i32, [c_vp, ctypes.POINTER(NwCaParams), c_i32, c_vp, c_vp, c_vp, c_vp]),
     "nw_weighted_sum": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_f64, c_vp, c_i32, c_vp]),

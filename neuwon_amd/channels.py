@@ -220,3 +220,42 @@ def random_ampa13_scheme(seed, time_step=0.1):
     b = rng.uniform(-0.05, 0.05, len(edges))
     return voltage_markov_scheme(edges, a, b, time_step=time_step, name="markov13",
                                  conserve_sum=1.0, num_states=13, weights=AMPA13_WEIGHTS)
+
+
+def hh_gate_table_theta(dt, celsius=6.3, theta=None, grid=None):
+    """Gate tables for rates scaled by theta = [ta_m, tb_m, ta_h, tb_h, ta_n, tb_n] (multipliers of
+    alpha and beta): FP64 [3 gates][node][6] = (a, b, da/dta, db/dta, da/dtb, db/dtb) with
+    a = P_OC, b = P_OO - P_OC = exp(-(alpha + beta) dt) of the two-state scheme."""
+    grid = grid or LinearInput(-100.0, 100.0, 2000)
+    theta = np.ones(6) if theta is None else np.asarray(theta, dtype=np.float64)
+    v = grid.nodes()
+    rates = hh_rates(v, celsius)
+    out = np.empty((3, len(v), 6))
+    for gi, g in enumerate("mhn"):
+        a0, b0 = rates[g]
+        ta, tb = theta[2 * gi], theta[2 * gi + 1]
+        al, be = ta * a0, tb * b0
+        s = al + be
+        e = np.exp(-s * dt)
+        oinf = al / s
+        out[gi, :, 0] = oinf * (1.0 - e)
+        out[gi, :, 1] = e
+        out[gi, :, 2] = a0 * (be / s ** 2 * (1.0 - e) + oinf * dt * e)
+        out[gi, :, 3] = a0 * (-dt * e)
+        out[gi, :, 4] = b0 * (-al / s ** 2 * (1.0 - e) + oinf * dt * e)
+        out[gi, :, 5] = b0 * (-dt * e)
+    return out
+
+
+def hh_rest_theta(v, celsius=6.3, theta=None):
+    """Open fractions at rest for scaled rates, and their derivatives d/dta, d/dtb per gate."""
+    theta = np.ones(6) if theta is None else np.asarray(theta, dtype=np.float64)
+    rates = hh_rates(v, celsius)
+    rest, drest = [], []
+    for gi, g in enumerate("mhn"):
+        a0, b0 = (float(x[0]) for x in rates[g])
+        ta, tb = theta[2 * gi], theta[2 * gi + 1]
+        s = ta * a0 + tb * b0
+        rest.append(ta * a0 / s)
+        drest.append((a0 * tb * b0 / s ** 2, -ta * a0 * b0 / s ** 2))
+    return rest, drest

@@ -44,6 +44,8 @@ struct HHArgs {
     uint32_t* ap_count;
     float* trace;
     int64_t trace_stride;
+    float* record;       // optional: per step 5 arrays (v, m, h, n at the start, v after half A)
+    int64_t num_rows;
 };
 
 // Stimulus current (A) of the cell's stimulated segment on input tick `tick` (1-based).
@@ -166,6 +168,13 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
             const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick_a) : 0.0;
             const double alpha = exp(-hdt_s * g * invc);
             xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - vref);
+            if (a.record) {
+                float* rec = a.record + (int64_t)step * 5 * a.num_rows + gi;
+                rec[0] = (float)v;
+                rec[a.num_rows] = a.m[gi];
+                rec[2 * a.num_rows] = a.h[gi];
+                rec[3 * a.num_rows] = a.n[gi];
+            }
         }
         __syncthreads();
         // ---- B: v = P x
@@ -175,6 +184,7 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
             const double v = vref + (double)vs[i];
+            if (a.record) a.record[((int64_t)step * 5 + 4) * a.num_rows + gi] = (float)v;
             double m = a.m[gi], h = a.h[gi], n = a.n[gi];
             if (GATE_MODE == 1) {
                 hh_exact(v, (double)p.q10, (double)p.dt_ms, m, h, n);
@@ -210,10 +220,229 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
             if (a.trace) a.trace[(int64_t)cell * a.trace_stride + step] = (float)vap;
         }
     }
-    for (int i = threadIdx.x; i < nrows; i += BLOCK) a.v[row0 + i] = (float)(vref + (double)vs[i]);
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+        const float vf = (float)(vref + (double)vs[i]);
+        a.v[row0 + i] = vf;
+        if (a.record) {  // the state after the last step closes the record
+            float* rec = a.record + (int64_t)a.n_steps * 5 * a.num_rows + row0 + i;
+            rec[0] = vf;
+            rec[a.num_rows] = a.m[row0 + i];
+            rec[2 * a.num_rows] = a.h[row0 + i];
+            rec[3 * a.num_rows] = a.n[row0 + i];
+        }
+    }
     if (threadIdx.x == 0 && ap_row >= 0) {
         a.ap_state[cell] = ap_state;
         a.ap_count[cell] = ap_count;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Reverse pass of n_steps fused steps (table gates), from a record written by k_hh_step.
+// Loss: sum over steps of (v[soma] after the step - target)^2.  Adjoints of the voltage and the
+// three gates are carried across launches; parameter gradients accumulate in FP64.
+//   P^T y = C P (C^-1 y): the transpose product is a forward product of y/C scaled by C.
+// ----------------------------------------------------------------------------------------------
+struct AdjArgs {
+    const int64_t* cell_row0;
+    const int64_t* cell_nz0;
+    const uint32_t* row_off;
+    const float* vals;
+    const uint16_t* cols;
+    const double* rowsum;
+    const int32_t* order;
+    int32_t nmax;
+    int32_t n_steps;
+    nw_hh_params prm;
+    const float* record;
+    int64_t num_rows;
+    const float* table6;   // [3][num_nodes][6]: a, b, da/dta, db/dta, da/dtb, db/dtb
+    const float* g_na;
+    const float* g_k;
+    const float* g_l;
+    const float* inv_c;
+    const int32_t* soma_local;
+    const float* target;
+    int64_t target_stride;
+    float* lam_v;
+    float* lam_m;
+    float* lam_h;
+    float* lam_n;
+    double* grad_na;
+    double* grad_k;
+    double* grad_l;
+    double* grad_theta;    // [cell][6]
+    double* loss;          // [cell]
+};
+
+struct GateLookup {
+    double val[6];
+    double slope[6];
+};
+
+__device__ __forceinline__ void lookup6(const float* tbl, int num_nodes, double v, double vmin,
+                                        double frq, GateLookup& o) {
+    const int nb = num_nodes - 1;
+    const double u = (v - vmin) * frq;
+    int b = (int)u;
+    double f;
+    bool inside = true;
+    if (b > nb - 1) { b = nb - 1; f = 1.0; inside = false; }
+    else if (b < 0) { b = 0; f = 0.0; inside = false; }
+    else f = u - (double)b;
+    const float* t0 = tbl + (int64_t)b * 6;
+    const float* t1 = t0 + 6;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const double d = (double)t1[q] - (double)t0[q];
+        o.val[q] = fma(f, d, (double)t0[q]);
+        o.slope[q] = inside ? d * frq : 0.0;
+    }
+}
+
+// Adjoint of x = D(1-al) + v al + b through G, N of the given gates; returns lambda_v and adds the
+// gate adjoints and the conductance gradients.
+__device__ __forceinline__ double half_adjoint(double lx, double v_in, double m, double h, double n,
+                                               double gna_bar, double gk_bar, double gl,
+                                               double e_na, double e_k, double e_l, double hs,
+                                               double invc, double& lm, double& lh, double& ln,
+                                               double& gr_na, double& gr_k, double& gr_l) {
+    const double m3h = m * m * m * h, n4 = n * n * n * n;
+    const double gna = gna_bar * m3h, gk = gk_bar * n4;
+    const double G = gna + gk + gl;
+    const double N = gna * e_na + gk * e_k + gl * e_l;
+    const double D = N / G;
+    const double al = exp(-hs * G * invc);
+    const double lD = (1.0 - al) * lx;
+    const double lal = (v_in - D) * lx;
+    const double lG = -lal * al * hs * invc - lD * D / G;
+    const double lN = lD / G;
+    const double l_gna = lG + e_na * lN, l_gk = lG + e_k * lN, l_gl = lG + e_l * lN;
+    gr_na += l_gna * m3h;
+    gr_k += l_gk * n4;
+    gr_l += l_gl;
+    lm += l_gna * gna_bar * 3.0 * m * m * h;
+    lh += l_gna * gna_bar * m * m * m;
+    ln += l_gk * gk_bar * 4.0 * n * n * n;
+    return al * lx;
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
+    extern __shared__ float lds[];
+    float* ys = lds;            // adjoint / C, input of the product
+    float* zs = lds + a.nmax;   // product output
+    float* lv = zs + a.nmax;    // adjoint of the voltage
+    __shared__ double red[6][BLOCK / kWave];
+    __shared__ double red_loss[BLOCK / kWave];
+    const nw_hh_params& p = a.prm;
+    const int cell = a.order[blockIdx.x];
+    const int64_t row0 = a.cell_row0[cell];
+    const int nrows = (int)(a.cell_row0[cell + 1] - row0);
+    const uint32_t* roff = a.row_off + row0 + cell;
+    const float* vals = a.vals + a.cell_nz0[cell];
+    const uint16_t* cols = a.cols + a.cell_nz0[cell];
+    const double* rs = a.rowsum + row0;
+    const int64_t NR = a.num_rows;
+    const double hs = 0.5 * (double)p.dt_ms * 1e-3;
+    const double vmin = p.vmin;
+    const double frq = (double)(p.num_nodes - 1) / ((double)p.vmax - (double)p.vmin);
+    const double e_na = p.e_na, e_k = p.e_k, e_l = p.e_l;
+    const int soma = a.soma_local[cell];
+    double th[6] = {0, 0, 0, 0, 0, 0};
+    double loss = 0.0;
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) lv[i] = a.lam_v[row0 + i];
+    __syncthreads();
+    for (int k = a.n_steps - 1; k >= 0; --k) {
+        const int64_t gstep = p.step0 + k;
+        const float* rec = a.record + (int64_t)k * 5 * NR;
+        const float* nxt = rec + 5 * NR;  // state at the start of the following step
+        if (threadIdx.x == 0) {
+            const double vs = nxt[row0 + soma];
+            const double r = vs - (double)a.target[(int64_t)cell * a.target_stride + gstep];
+            loss += r * r;
+            lv[soma] += (float)(2.0 * r);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) ys[i] = lv[i] * a.inv_c[row0 + i];
+        __syncthreads();
+        spmv<0>(roff, vals, cols, rs, ys, zs, 0.0, nrows);
+        __syncthreads();
+        // half B and the mechanism
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+            const int64_t gi = row0 + i;
+            const double invc = a.inv_c[gi];
+            const double lxB = (double)zs[i] / invc;
+            const double v1 = rec[4 * NR + gi];
+            const double mN = nxt[NR + gi], hN = nxt[2 * NR + gi], nN = nxt[3 * NR + gi];
+            double lm = a.lam_m[gi], lh = a.lam_h[gi], ln = a.lam_n[gi];
+            double gr_na = 0, gr_k = 0, gr_l = 0;
+            double lv1 = half_adjoint(lxB, v1, mN, hN, nN, a.g_na[gi], a.g_k[gi], a.g_l[gi], e_na,
+                                      e_k, e_l, hs, invc, lm, lh, ln, gr_na, gr_k, gr_l);
+            const double g0[3] = {rec[NR + gi], rec[2 * NR + gi], rec[3 * NR + gi]};
+            double lg[3] = {lm, lh, ln};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                GateLookup L;
+                lookup6(a.table6 + (int64_t)q * p.num_nodes * 6, p.num_nodes, v1, vmin, frq, L);
+                lv1 += (L.slope[0] + L.slope[1] * g0[q]) * lg[q];
+                th[2 * q] += (L.val[2] + L.val[3] * g0[q]) * lg[q];
+                th[2 * q + 1] += (L.val[4] + L.val[5] * g0[q]) * lg[q];
+                lg[q] *= L.val[1];
+            }
+            a.lam_m[gi] = (float)lg[0];
+            a.lam_h[gi] = (float)lg[1];
+            a.lam_n[gi] = (float)lg[2];
+            a.grad_na[gi] += gr_na;
+            a.grad_k[gi] += gr_k;
+            a.grad_l[gi] += gr_l;
+            ys[i] = (float)(lv1 * invc);
+        }
+        __syncthreads();
+        spmv<0>(roff, vals, cols, rs, ys, zs, 0.0, nrows);
+        __syncthreads();
+        // half A with the gates at the start of the step
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+            const int64_t gi = row0 + i;
+            const double invc = a.inv_c[gi];
+            const double lxA = (double)zs[i] / invc;
+            double lvk = lxA;
+            if (gstep > 0 || p.conductances_valid) {
+                double lm = a.lam_m[gi], lh = a.lam_h[gi], ln = a.lam_n[gi];
+                double gr_na = 0, gr_k = 0, gr_l = 0;
+                lvk = half_adjoint(lxA, rec[gi], rec[NR + gi], rec[2 * NR + gi], rec[3 * NR + gi],
+                                   a.g_na[gi], a.g_k[gi], a.g_l[gi], e_na, e_k, e_l, hs, invc, lm,
+                                   lh, ln, gr_na, gr_k, gr_l);
+                a.lam_m[gi] = (float)lm;
+                a.lam_h[gi] = (float)lh;
+                a.lam_n[gi] = (float)ln;
+                a.grad_na[gi] += gr_na;
+                a.grad_k[gi] += gr_k;
+                a.grad_l[gi] += gr_l;
+            }
+            lv[i] = (float)lvk;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) a.lam_v[row0 + i] = lv[i];
+    // reduce the rate-parameter gradients of the block (fixed order: deterministic)
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const double s = wave_sum(th[q]);
+        if (lane == 0) red[q][wave] = s;
+    }
+    if (lane == 0) red_loss[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 6; ++q) {
+            double t = 0.0;
+            for (int w = 0; w < BLOCK / kWave; ++w) t += red[q][w];
+            a.grad_theta[(int64_t)cell * 6 + q] += t;
+        }
+        double t = 0.0;
+        for (int w = 0; w < BLOCK / kWave; ++w) t += red_loss[w];
+        a.loss[cell] += t;
     }
 }
 
@@ -441,7 +670,7 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
                const float* gate_table, float* v, float* m, float* h, float* n, const float* g_na,
                const float* g_k, const float* g_l, const float* inv_c, const int32_t* stim_local,
                const int32_t* ap_local, uint8_t* ap_state, uint32_t* ap_count, float* trace,
-               int64_t trace_stride, void* stream) {
+               int64_t trace_stride, float* record, void* stream) {
     NW_REQUIRE(p && prm && gate_table && v && m && h && n && g_na && g_k && g_l && inv_c,
                "nw_hh_step: null argument");
     NW_REQUIRE(n_steps >= 0, "n_steps must be non-negative");
@@ -454,7 +683,7 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
     HHArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
              p->max_cell_rows,
              n_steps, *prm, gate_table, v, m, h, n, g_na, g_k, g_l, inv_c, stim_local, ap_local,
-             ap_state, ap_count, trace, trace_stride};
+             ap_state, ap_count, trace, trace_stride, record, p->num_rows};
     const size_t lds = 2 * sizeof(float) * (size_t)p->max_cell_rows;
     NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
     hipStream_t s = (hipStream_t)stream;
@@ -471,6 +700,31 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
         default: NW_REQUIRE(false, "block_threads must be 128, 256, 512 or 1024 (got %d)", bt);
     }
 #undef NW_HH_LAUNCH
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
+                  const float* record, const float* table6, const float* g_na, const float* g_k,
+                  const float* g_l, const float* inv_c, const int32_t* soma_local,
+                  const float* target, int64_t target_stride, float* lam_v, float* lam_m,
+                  float* lam_h, float* lam_n, double* grad_na, double* grad_k, double* grad_l,
+                  double* grad_theta, double* loss, void* stream) {
+    NW_REQUIRE(p && prm && record && table6 && g_na && g_k && g_l && inv_c && soma_local &&
+                   target && lam_v && lam_m && lam_h && lam_n && grad_na && grad_k && grad_l &&
+                   grad_theta && loss,
+               "nw_hh_adjoint: null argument");
+    NW_REQUIRE(prm->gate_mode == 0, "the adjoint is written for the table gate mode");
+    NW_REQUIRE(prm->stim_mode == 0 || prm->stim_mode == 1, "adjoint supports constant stimulus only");
+    NW_REQUIRE(n_steps >= 0 && prm->num_nodes >= 2, "bad arguments");
+    if (n_steps == 0) return NW_OK;
+    AdjArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
+              p->max_cell_rows, n_steps, *prm, record, p->num_rows, table6, g_na, g_k, g_l, inv_c,
+              soma_local, target, target_stride, lam_v, lam_m, lam_h, lam_n, grad_na, grad_k,
+              grad_l, grad_theta, loss};
+    const size_t lds = 3 * sizeof(float) * (size_t)p->max_cell_rows;
+    NW_REQUIRE(lds <= 150 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
+    k_hh_adjoint<256><<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
     NW_HIP(hipGetLastError());
     return NW_OK;
 }

@@ -183,9 +183,11 @@ class HHPopulation:
         else:
             p.stim_mode = STIM_NONE
 
-    def advance(self, n_steps=1, trace=None, stream=None):
+    def advance(self, n_steps=1, trace=None, stream=None, record=None):
         """Advance every cell by n_steps time steps (one launch).  trace: optional float32 device
-        tensor [num_cells, >= n_steps] receiving the root voltage after each step."""
+        tensor [num_cells, >= n_steps] receiving the AP-segment voltage after each step.
+        record: optional float32 device tensor of (5 n_steps + 4) * num_segments values receiving
+        the state of every step for the reverse pass."""
         torch = self.torch
         p = self.params
         p.step0 = self.steps
@@ -202,7 +204,7 @@ class HHPopulation:
                 self.propagator.handle, ctypes.byref(p), int(n_steps), P(self.table), P(self.v),
                 P(self.m), P(self.h), P(self.n), P(self.g_na), P(self.g_k), P(self.g_l),
                 P(self.inv_c), P(self.stim_local), P(self.ap_local), P(self.ap_state),
-                P(self.ap_count), tr_ptr, tr_stride, ctypes.c_void_p(s.cuda_stream)))
+                P(self.ap_count), tr_ptr, tr_stride, P(record), ctypes.c_void_p(s.cuda_stream)))
         self.steps += n_steps
 
     def time(self):

@@ -499,7 +499,7 @@ class CHHPopulation:
     ('poisson', amp_A, seed, threshold, pulse_ticks), applied to segment 0 of every cell."""
 
     def __init__(self, cell_ptr, P, C, g_bar, v0=-70.0, dt=0.1, celsius=6.3, gate_mode="table",
-                 stim=None, E=None, tables=None):
+                 stim=None, E=None, tables=None, gates0=None):
         import ctypes
         self.lib = _c_lib()
         self.cell_ptr = np.ascontiguousarray(cell_ptr, dtype=np.int64)
@@ -511,7 +511,8 @@ class CHHPopulation:
         self.g = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in g_bar.items()}
         N = len(self.C)
         self.v = np.full(N, float(v0))
-        self.gates = {g: np.full(N, float(gate_inf(v0, g, celsius))) for g in "mhn"}
+        gates0 = gates0 or {g: float(gate_inf(v0, g, celsius)) for g in "mhn"}
+        self.gates = {g: np.full(N, float(gates0[g])) for g in "mhn"}
         self.sum_g = np.zeros(N)
         self.d_num = np.zeros(N)
         self.steps = 0
@@ -627,3 +628,211 @@ class CaSim:
         """Conserved total: the last buffer change is paid back in Ca over two half steps, the
         second of them in the next step, hence the pending dcdt dt/2."""
         return float(np.sum(self.vol * (self.ca + self.cab + 0.5 * self.dt * self.dcdt)))
+
+
+# ---------------------------------------------------------------------------------------------
+# Adjoint of the staggered HH step (table gates): FP64 forward, reverse pass, finite differences
+# ---------------------------------------------------------------------------------------------
+def hh_theta_tables(dt, celsius=6.3, theta=None, vmin=-100.0, vmax=100.0, num_buckets=2000):
+    """Node tables of every gate for rates scaled by theta[gate] = (t_alpha, t_beta):
+    open' = a(v) + b(v) open with a = P_OC, b = P_OO - P_OC = exp(-(alpha+beta) dt), and the
+    derivatives of a and b with respect to t_alpha and t_beta (closed form of the 2x2 expm,
+    lti_model.py:51-65).  Returns dict gate -> array [node][6] = (a, b, da/dta, db/dta, da/dtb, db/dtb)."""
+    theta = theta or {}
+    v = vmin + np.arange(num_buckets + 1) * ((vmax - vmin) / num_buckets)
+    rates = hh_alpha_beta(v, celsius)
+    out = {}
+    for g in "mhn":
+        ta, tb = theta.get(g, (1.0, 1.0))
+        a0, b0 = rates[g]
+        al, be = ta * a0, tb * b0
+        s = al + be
+        e = np.exp(-s * dt)
+        oinf = al / s
+        a = oinf * (1.0 - e)
+        da_dal = be / s ** 2 * (1.0 - e) + oinf * dt * e
+        da_dbe = -al / s ** 2 * (1.0 - e) + oinf * dt * e
+        db_dal = -dt * e
+        db_dbe = -dt * e
+        out[g] = np.stack([a, e, a0 * da_dal, a0 * db_dal, b0 * da_dbe, b0 * db_dbe], axis=1)
+    return out
+
+
+class HHAdjointProblem:
+    """Cells with HH on every segment, constant current at segment 0, loss
+    L = sum_steps (v_soma - target)^2 with v_soma the voltage of segment 0 after each step.
+
+    Parameters: per-segment conductances g_na, g_k, g_l (S) and six rate multipliers
+    theta[gate] = (t_alpha, t_beta).  Same step as HHSim in table mode, with the table
+    interpolation written on node values (identical to the polynomial form)."""
+
+    def __init__(self, P, C, stim_amp, dt=0.05, celsius=6.3, v0=-70.0, E=None):
+        self.P = P.tocsr()
+        self.PT = self.P.T.tocsr()
+        self.C = np.asarray(C, dtype=np.float64)
+        self.n = len(self.C)
+        self.stim = np.zeros(self.n)
+        self.stim[0] = stim_amp
+        self.dt, self.celsius, self.v0 = dt, celsius, v0
+        self.E = E or {"na": 60.0, "k": -88.0, "l": -54.3}
+        self.nb = 2000
+        self.frq = self.nb / 200.0
+
+    def _lookup(self, tbl, v):
+        u = (v + 100.0) * self.frq
+        b = np.trunc(u).astype(np.int64)
+        f = u - b
+        hi, lo = b > self.nb - 1, b < 0
+        inside = ~(hi | lo)
+        b = np.where(hi, self.nb - 1, np.where(lo, 0, b))
+        f = np.where(hi, 1.0, np.where(lo, 0.0, f))
+        t0, t1 = tbl[b], tbl[b + 1]
+        val = t0 + f[:, None] * (t1 - t0)
+        slope = np.where(inside[:, None], (t1 - t0) * self.frq, 0.0)
+        return val, slope
+
+    def _g(self, gb, m, h, n):
+        gna = gb["na"] * m ** 3 * h
+        gk = gb["k"] * n ** 4
+        gl = gb["l"]
+        G = gna + gk + gl
+        N = gna * self.E["na"] + gk * self.E["k"] + gl * self.E["l"]
+        return gna, gk, G, N
+
+    def forward(self, gb, theta, steps, target=None, record=False):
+        tables = hh_theta_tables(self.dt, self.celsius, theta)
+        hs = 0.5 * self.dt * 1e-3
+        hdt = 0.5 * self.dt
+        bI = hdt * self.stim / self.C
+        v = np.full(self.n, self.v0)
+        ss = {g: float(gate_inf(self.v0, g, self.celsius)) for g in "mhn"}
+        # rest state of the scaled rates
+        gates = {}
+        for g in "mhn":
+            a0, b0 = (float(x) for x in hh_alpha_beta(self.v0, self.celsius)[g])
+            ta, tb = theta.get(g, (1.0, 1.0))
+            gates[g] = np.full(self.n, ta * a0 / (ta * a0 + tb * b0))
+        soma, rec = [], []
+        for k in range(steps):
+            m, h, n_ = gates["m"], gates["h"], gates["n"]
+            if k == 0:
+                x = v + bI                       # accumulators start at zero
+            else:
+                _, _, G, N = self._g(gb, m, h, n_)
+                D = N / G
+                al = np.exp(-hs * G / self.C)
+                x = D * (1 - al) + v * al + bI
+            v1 = self.P.dot(x)
+            new = {}
+            for g in "mhn":
+                val, _ = self._lookup(tables[g], v1)
+                new[g] = val[:, 0] + val[:, 1] * gates[g]
+            if record:
+                rec.append((v.copy(), dict(gates), v1.copy()))
+            _, _, G, N = self._g(gb, new["m"], new["h"], new["n"])
+            D = N / G
+            al = np.exp(-hs * G / self.C)
+            v = self.P.dot(D * (1 - al) + v1 * al + bI)
+            gates = new
+            soma.append(v[0])
+        soma = np.array(soma)
+        loss = None if target is None else float(np.sum((soma - target) ** 2))
+        if record:
+            self._rec, self._final_gates, self._tables = rec, gates, tables
+        return soma, loss
+
+    def gradient(self, gb, theta, target):
+        """Reverse pass over the recorded forward; returns (loss, grads) with grads
+        {'na','k','l': per segment, 'theta': {gate: (dL/dt_alpha, dL/dt_beta)}}."""
+        steps = len(target)
+        soma, loss = self.forward(gb, theta, steps, target, record=True)
+        hs = 0.5 * self.dt * 1e-3
+        E = self.E
+        lv = np.zeros(self.n)
+        lg = {g: np.zeros(self.n) for g in "mhn"}
+        grad = {"na": np.zeros(self.n), "k": np.zeros(self.n), "l": np.zeros(self.n)}
+        gth = {g: [0.0, 0.0] for g in "mhn"}
+        tables = self._tables
+
+        def g_adjoint(gates, lG, lN):
+            """Adjoint through G, N of the given gates: gate adjoints and parameter grads."""
+            m, h, n_ = gates["m"], gates["h"], gates["n"]
+            l_gna = lG + E["na"] * lN
+            l_gk = lG + E["k"] * lN
+            l_gl = lG + E["l"] * lN
+            grad["na"] += l_gna * m ** 3 * h
+            grad["k"] += l_gk * n_ ** 4
+            grad["l"] += l_gl
+            return {"m": l_gna * gb["na"] * 3 * m ** 2 * h, "h": l_gna * gb["na"] * m ** 3,
+                    "n": l_gk * gb["k"] * 4 * n_ ** 3}
+
+        def half_adjoint(v_in, gates, lx, first):
+            """x = D(1-al) + v al + b: returns lambda_v_in and gate adjoints of the half step."""
+            if first:
+                return lx, {g: np.zeros(self.n) for g in "mhn"}
+            _, _, G, N = self._g(gb, gates["m"], gates["h"], gates["n"])
+            D = N / G
+            al = np.exp(-hs * G / self.C)
+            lD = (1 - al) * lx
+            lal = (v_in - D) * lx
+            lG = lal * al * (-hs / self.C) - lD * D / G
+            lN = lD / G
+            return al * lx, g_adjoint(gates, lG, lN)
+
+        next_gates = self._final_gates
+        for k in range(steps - 1, -1, -1):
+            v_k, gates_k, v1 = self._rec[k]
+            lv[0] += 2.0 * (soma[k] - target[k])
+            # half B: v2 = P x_B, gates after the mechanism = next_gates
+            lxB = self.PT.dot(lv)
+            lv1, lgB = half_adjoint(v1, next_gates, lxB, False)
+            for g in "mhn":
+                lg[g] = lg[g] + lgB[g]
+            # mechanism: g' = a(v1) + b(v1) g
+            for g in "mhn":
+                val, slope = self._lookup(tables[g], v1)
+                lv1 = lv1 + (slope[:, 0] + slope[:, 1] * gates_k[g]) * lg[g]
+                gth[g][0] += float(np.sum((val[:, 2] + val[:, 3] * gates_k[g]) * lg[g]))
+                gth[g][1] += float(np.sum((val[:, 4] + val[:, 5] * gates_k[g]) * lg[g]))
+                lg[g] = val[:, 1] * lg[g]
+            # half A: v1 = P x_A with the gates at the start of the step
+            lxA = self.PT.dot(lv1)
+            lv, lgA = half_adjoint(v_k, gates_k, lxA, k == 0)
+            for g in "mhn":
+                lg[g] = lg[g] + lgA[g]
+            next_gates = gates_k
+        # initial gates depend on theta through the rest state
+        for g in "mhn":
+            a0, b0 = (float(x) for x in hh_alpha_beta(self.v0, self.celsius)[g])
+            ta, tb = theta.get(g, (1.0, 1.0))
+            s = ta * a0 + tb * b0
+            d_ta = a0 * (tb * b0) / s ** 2
+            d_tb = -(ta * a0) * b0 / s ** 2
+            gth[g][0] += float(np.sum(lg[g])) * d_ta
+            gth[g][1] += float(np.sum(lg[g])) * d_tb
+        grad["theta"] = {g: tuple(gth[g]) for g in "mhn"}
+        return loss, grad
+
+
+def theta_poly_tables(dt, celsius=6.3, theta=None):
+    """Polynomial LTI tables (codegen layout) of the three gates for rates scaled by theta, and the
+    rest state of the scaled rates at v0 = -70 mV."""
+    t = hh_theta_tables(dt, celsius, theta)
+    out = {}
+    for g in "mhn":
+        a, b = t[g][:, 0], t[g][:, 1]
+        node = np.empty((len(a), 2, 2))
+        node[:, 1, 0], node[:, 1, 1] = a, a + b
+        node[:, 0, 0], node[:, 0, 1] = 1.0 - a, 1.0 - (a + b)
+        out[g] = node_table_to_polynomial(node)
+    return out
+
+
+def rest_gates(v0, celsius=6.3, theta=None):
+    theta = theta or {}
+    out = {}
+    for g in "mhn":
+        a0, b0 = (float(x) for x in hh_alpha_beta(v0, celsius)[g])
+        ta, tb = theta.get(g, (1.0, 1.0))
+        out[g] = ta * a0 / (ta * a0 + tb * b0)
+    return out

@@ -1,0 +1,186 @@
+"""The hand-written reverse pass (nw_hh_adjoint) against the FP64 oracle adjoint."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import neuwon_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-adjoint] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_problem(prob, c, stim_amp, dt):
+    pop = prob.pop
+    r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
+    P = O.propagator(pop.parent_local[r0:r1].astype(np.int64), pop.R[r0:r1], pop.C[r0:r1],
+                     0.5 * dt, prune=pop.propagator.prune)
+    return r0, r1, O.HHAdjointProblem(P, pop.C[r0:r1], stim_amp, dt=dt)
+
+
+def _hidden(gb, rng):
+    return {k: gb[k] * np.exp(rng.uniform(np.log(0.5), np.log(2.0), len(gb[k]))) for k in gb}
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_adjoint_matches_oracle(torch_gpu):
+    from neuwon_amd import adjoint, synth
+    torch = torch_gpu
+    dt, T, amp = 0.05, 300, 0.1e-9
+    cells = synth.uniform_population(num_cells=4, n=90, seed=6)
+    prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100)
+    pop = prob.pop
+    gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
+    rng = np.random.default_rng(7)
+    targets = np.zeros((pop.num_cells, T))
+    oracles = []
+    for c in range(pop.num_cells):
+        r0, r1, op = _oracle_problem(prob, c, amp, dt)
+        hid = _hidden({k: gb[k][r0:r1] for k in gb}, rng)
+        targets[c], _ = op.forward(hid, {}, T)
+        oracles.append((r0, r1, op))
+    theta = np.ones(6)
+    tgt = torch.as_tensor(targets.astype(np.float32), device="cuda")
+    f32 = lambda a: torch.as_tensor(a.astype(np.float32), device="cuda")
+    loss, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), theta, tgt)
+    g_na, g_k, g_l = (gr[k].cpu().numpy() for k in ("na", "k", "l"))
+    th_ref = np.zeros(6)
+    loss_ref = 0.0
+    for c, (r0, r1, op) in enumerate(oracles):
+        lr, ref = op.gradient({k: gb[k][r0:r1] for k in gb}, {}, targets[c])
+        loss_ref += lr
+        errs = [_rel(g_na[r0:r1], ref["na"]), _rel(g_k[r0:r1], ref["k"]), _rel(g_l[r0:r1], ref["l"])]
+        _say(f"cell {c}: per-segment gradient rel err {['%.1e' % e for e in errs]}, "
+             f"v_soma max {targets[c].max():.1f}")
+        assert max(errs) < 1e-3
+        th_ref += np.array([ref["theta"][g][j] for g in "mhn" for j in (0, 1)])
+    th_err = np.max(np.abs(gr["theta"] - th_ref) / np.abs(th_ref))
+    _say(f"loss {loss:.6e} vs {loss_ref:.6e}; theta gradient rel err {th_err:.2e}")
+    assert loss == pytest.approx(loss_ref, rel=1e-4)
+    assert th_err < 1e-3
+
+
+def test_autograd_function(torch_gpu):
+    from neuwon_amd import adjoint, synth
+    torch = torch_gpu
+    cells = synth.uniform_population(num_cells=3, n=60, seed=8)
+    prob = adjoint.HHAdjoint(cells, dt=0.05, stim_amp_na=0.1, checkpoint_every=50)
+    pop = prob.pop
+    T = 120
+    tgt = torch.full((pop.num_cells, T), -65.0, device="cuda")
+    params = [pop.g_na.clone().requires_grad_(), pop.g_k.clone().requires_grad_(),
+              pop.g_l.clone().requires_grad_(),
+              torch.ones(6, dtype=torch.float64, device="cuda", requires_grad=True)]
+    loss = adjoint.trajectory_loss(prob, *params, tgt)
+    loss.backward()
+    direct_loss, gr = prob.loss_and_grad(pop.g_na.clone(), pop.g_k.clone(), pop.g_l.clone(),
+                                         np.ones(6), tgt)
+    assert float(loss) == pytest.approx(direct_loss, rel=1e-12)
+    assert torch.allclose(params[0].grad.double(), gr["na"].double(), rtol=1e-6)
+    assert np.allclose(params[3].grad.cpu().numpy(), gr["theta"], rtol=1e-9)
+
+
+def test_adjoint_workload_1024x500(torch_gpu):
+    """1024 cells x 500 compartments, 2000 steps at dt 0.05 ms, checkpoints every 100 steps:
+    gradients per second and a per-cell check against the FP64 oracle adjoint."""
+    from neuwon_amd import adjoint, synth
+    torch = torch_gpu
+    dt, T, amp = 0.05, 2000, 0.5e-9
+    t0 = time.time()
+    cells = synth.uniform_population(num_cells=1024, n=500, seed=4)
+    prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100)
+    pop = prob.pop
+    _say(f"setup {time.time() - t0:.1f}s, nnz/row {pop.propagator.mean_nnz_per_row:.1f}")
+    rng = np.random.default_rng(4)
+    gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
+    hidden = _hidden(gb, rng)
+    f32 = lambda a: torch.as_tensor(a.astype(np.float32), device="cuda")
+    dummy = torch.zeros((pop.num_cells, T), device="cuda")
+    prob.forward(f32(hidden["na"]), f32(hidden["k"]), f32(hidden["l"]), np.ones(6), dummy)
+    target = prob.trace.clone()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), target)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    _say(f"one gradient of {pop.num_segments} compartments x {T} steps: {el:.2f}s "
+         f"({1.0 / el:.3f} gradients/s), loss {loss:.4e}, "
+         f"target v_soma max {float(target.max()):.1f} mV")
+    # check two cells against the FP64 oracle adjoint fed the same target traces
+    tg = target.cpu().numpy().astype(np.float64)
+    for c in (0, 511):
+        r0, r1, op = _oracle_problem(prob, c, amp, dt)
+        _, ref = op.gradient({k: gb[k][r0:r1] for k in gb}, {}, tg[c])
+        errs = [_rel(gr[k].cpu().numpy()[r0:r1], ref[k]) for k in ("na", "k", "l")]
+        _say(f"cell {c}: rel err vs oracle adjoint {['%.1e' % e for e in errs]}")
+        assert max(errs) < 1e-3
+
+
+def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
+    """Central finite differences of the FP64 C oracle (h = 1e-6 relative) on 32 cells of
+    500 compartments, 2000 steps at dt 0.05 ms, against the GPU reverse pass."""
+    import os
+    import scipy.sparse
+    from neuwon_amd import adjoint, synth, population
+    torch = torch_gpu
+    dt, T, amp, ncell = 0.05, 2000, 0.5e-9, 32
+    cells = synth.uniform_population(num_cells=1024, n=500, seed=4)[:ncell]
+    prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100)
+    pop = prob.pop
+    gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
+    hidden = _hidden(gb, np.random.default_rng(4))
+    f32 = lambda a: torch.as_tensor(a.astype(np.float32), device="cuda")
+    prob.forward(f32(hidden["na"]), f32(hidden["k"]), f32(hidden["l"]), np.ones(6),
+                 torch.zeros((ncell, T), device="cuda"))
+    target = prob.trace.clone()
+    _, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), target)
+    tg = target.cpu().numpy().astype(np.float64)
+    blocks = []
+    for c in range(ncell):
+        r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
+        blocks.append(population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1],
+                                                     pop.C[r0:r1], 0.5 * dt, 1e-9))
+    P = scipy.sparse.block_diag(blocks, format="csr")
+    roots = pop.cell_ptr[:-1]
+    threads = len(os.sched_getaffinity(0))
+
+    def loss64(g, theta):
+        co = O.CHHPopulation(pop.cell_ptr, P, pop.C, g, dt=dt, gate_mode="table",
+                             stim=("constant", amp), tables=O.theta_poly_tables(dt, 6.3, theta),
+                             gates0=O.rest_gates(-70.0, 6.3, theta))
+        L = 0.0
+        for k in range(T):
+            co.advance(1, threads=threads)
+            L += float(np.sum((co.v[roots] - tg[:, k]) ** 2))
+        return L
+
+    h = 1e-6
+    t0 = time.time()
+    checks = []
+    for gi, g in enumerate("mhn"):
+        for j in range(2):
+            tp = {g: (1 + h, 1.0) if j == 0 else (1.0, 1 + h)}
+            tm = {g: (1 - h, 1.0) if j == 0 else (1.0, 1 - h)}
+            fd = (loss64(gb, tp) - loss64(gb, tm)) / (2 * h)
+            checks.append((f"theta_{g}{'ab'[j]}", gr["theta"][2 * gi + j], fd))
+    for k, i in (("na", 0), ("k", 0), ("l", 3)):
+        gp = {kk: vv.copy() for kk, vv in gb.items()}
+        gm = {kk: vv.copy() for kk, vv in gb.items()}
+        d = h * gb[k][i]
+        gp[k][i] += d
+        gm[k][i] -= d
+        fd = (loss64(gp, {}) - loss64(gm, {})) / (2 * d)
+        checks.append((f"g_{k}[{i}]", float(gr[k][i]), fd))
+    for name, adj, fd in checks:
+        _say(f"{name}: adjoint {adj:.6e} FD64 {fd:.6e} rel {abs(adj - fd) / abs(fd):.1e}")
+    _say(f"finite differences took {time.time() - t0:.0f}s on {threads} threads")
+    for name, adj, fd in checks:
+        assert adj == pytest.approx(fd, rel=1e-4), name

@@ -414,3 +414,53 @@ def test_buffer_equilibrium_and_mass():
     P = scipy.linalg.expm(np.array([[-a, b], [a, -b]]) * 0.1)
     got = O.buffer_exact(np.array([30.0]), 0.7, 0.1, kon, koff, btot)[0]
     assert got == pytest.approx(P[1, 0] * (btot - 30.0) + P[1, 1] * 30.0, rel=1e-13)
+
+
+# ------------------------------------------------------------------------------------------------
+# adjoint of the staggered step: the FP64 reverse pass against central finite differences
+# ------------------------------------------------------------------------------------------------
+def test_oracle_adjoint_matches_finite_differences():
+    from neuwon_amd import synth, geometry, channels
+    cell = synth.single_cell(40, seed=4)[0]
+    parent = cell.parent.astype(np.int64)
+    geo = geometry.geometry(parent, cell.length, cell.diam)
+    R, C = geometry.passive(geo["length"], geo["cross_sectional_area"], geo["area_at_creation"])
+    P = O.propagator(parent, R, C, 0.025, prune=1e-9)
+    gb = {k: channels.HH_GBAR[k] * 1e-8 * geo["surface_area"] for k in ("na", "k", "l")}
+    prob = O.HHAdjointProblem(P, C, 0.1e-9, dt=0.05)
+    rng = np.random.default_rng(1)
+    hidden = {k: gb[k] * np.exp(rng.uniform(np.log(0.5), np.log(2), len(C))) for k in gb}
+    T = 250
+    target, _ = prob.forward(hidden, {}, T)
+    assert target.max() > 20.0            # the trajectory spikes
+    loss, grad = prob.gradient(gb, {}, target)
+    h = 1e-6
+    for k, i in (("na", 3), ("k", 11), ("l", 0)):
+        gp = {kk: vv.copy() for kk, vv in gb.items()}
+        gm = {kk: vv.copy() for kk, vv in gb.items()}
+        d = h * gb[k][i]
+        gp[k][i] += d
+        gm[k][i] -= d
+        fd = (prob.forward(gp, {}, T, target)[1] - prob.forward(gm, {}, T, target)[1]) / (2 * d)
+        assert grad[k][i] == pytest.approx(fd, rel=1e-4)
+    for g in "mhn":
+        for j in range(2):
+            tp = {g: (1 + h, 1.0) if j == 0 else (1.0, 1 + h)}
+            tm = {g: (1 - h, 1.0) if j == 0 else (1.0, 1 - h)}
+            fd = (prob.forward(gb, tp, T, target)[1] - prob.forward(gb, tm, T, target)[1]) / (2 * h)
+            assert grad["theta"][g][j] == pytest.approx(fd, rel=1e-4)
+
+
+def test_theta_tables_match_oracle():
+    from neuwon_amd import channels
+    th = [1.1, 0.9, 1.0, 1.2, 0.8, 1.05]
+    got = channels.hh_gate_table_theta(0.05, 6.3, th)
+    ref = O.hh_theta_tables(0.05, 6.3, {"m": (1.1, 0.9), "h": (1.0, 1.2), "n": (0.8, 1.05)})
+    for gi, g in enumerate("mhn"):
+        np.testing.assert_allclose(got[gi], ref[g], rtol=1e-12, atol=1e-18)
+    # derivative columns against differences of the value columns
+    eps = 1e-6
+    up = channels.hh_gate_table_theta(0.05, 6.3, [1.1 + eps, 0.9, 1, 1.2, 0.8, 1.05])
+    dn = channels.hh_gate_table_theta(0.05, 6.3, [1.1 - eps, 0.9, 1, 1.2, 0.8, 1.05])
+    fd = (up[0, :, :2] - dn[0, :, :2]) / (2 * eps)
+    np.testing.assert_allclose(got[0, :, 2:4], fd, rtol=1e-5, atol=1e-12)
