@@ -487,7 +487,7 @@ k_hh_mechanism(int64_t M, double dt, double q10, const float* __restrict__ table
 // ----------------------------------------------------------------------------------------------
 // LTI kinetic kernel: thread per instance, S states in registers.
 // ----------------------------------------------------------------------------------------------
-template <int S>
+template <int S, bool HOLD>
 __global__ void __launch_bounds__(256)
 k_lti(int num_nodes, float vmin, float vmax, const float* __restrict__ table, int64_t M,
       const float* __restrict__ voltage, const int32_t* __restrict__ seg_index, float* states,
@@ -496,34 +496,48 @@ k_lti(int num_nodes, float vmin, float vmax, const float* __restrict__ table, in
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int nb = num_nodes - 1;
-    const float bucket_frq = (float)nb / (vmax - vmin);
-    const float v = voltage[seg_index ? (int64_t)seg_index[i] : i];
+    const double bucket_frq = (double)nb / ((double)vmax - (double)vmin);
+    const double v = voltage[seg_index ? (int64_t)seg_index[i] : i];
     int b;
-    float f;
-    bucket_of<float>(v, vmin, bucket_frq, nb, b, f);
+    double fd;
+    bucket_of<double>(v, (double)vmin, bucket_frq, nb, b, fd);
     const float* T0 = table + (int64_t)b * S * S;
     const float* T1 = T0 + S * S;
-    float s[S];
+    // States and sums in FP64 registers, propagator entries in FP32 (as stored).
+    double s[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) s[k] = states[k * stride + i];
-    for (int step = 0; step < n_steps; ++step) {
-        float out[S];
+    // With the input held constant over the launch (HOLD) the interpolated propagator is formed
+    // once and kept in registers; every step is then identical to a call with n_steps = 1.
+    float Pm[HOLD ? S * S : 1];
+    if (HOLD) {
 #pragma unroll
-        for (int r = 0; r < S; ++r) out[r] = 0.0f;
+        for (int q = 0; q < S * S; ++q)
+            Pm[q] = (float)fma(fd, (double)T1[q] - (double)T0[q], (double)T0[q]);
+    }
+    for (int step = 0; step < n_steps; ++step) {
+        double out[S];
+#pragma unroll
+        for (int r = 0; r < S; ++r) out[r] = 0.0;
 #pragma unroll
         for (int c = 0; c < S; ++c) {  // column-major walk, as the reference kernel (codegen.py:81-94)
 #pragma unroll
             for (int r = 0; r < S; ++r) {
-                const float t0 = T0[r * S + c];
-                const float pe = fmaf(f, T1[r * S + c] - t0, t0);
-                out[r] = fmaf(pe, s[c], out[r]);
+                double pe;
+                if (HOLD) {
+                    pe = Pm[r * S + c];
+                } else {
+                    const double t0 = T0[r * S + c];
+                    pe = fma(fd, (double)T1[r * S + c] - t0, t0);
+                }
+                out[r] = fma(pe, s[c], out[r]);
             }
         }
         if (conserve_sum > 0.0f) {  // codegen.py:95-102
-            float tot = 0.0f;
+            double tot = 0.0;
 #pragma unroll
             for (int r = 0; r < S; ++r) tot += out[r];
-            const float corr = conserve_sum / tot;
+            const double corr = (double)conserve_sum / tot;
 #pragma unroll
             for (int r = 0; r < S; ++r) out[r] *= corr;
         }
@@ -531,12 +545,12 @@ k_lti(int num_nodes, float vmin, float vmax, const float* __restrict__ table, in
         for (int r = 0; r < S; ++r) s[r] = out[r];
     }
 #pragma unroll
-    for (int k = 0; k < S; ++k) states[k * stride + i] = s[k];
+    for (int k = 0; k < S; ++k) states[k * stride + i] = (float)s[k];
     if (cond_out) {
-        float w = 0.0f;
+        double w = 0.0;
 #pragma unroll
-        for (int k = 0; k < S; ++k) w = fmaf(weights[k], s[k], w);
-        cond_out[i] = gmax[i] * w;
+        for (int k = 0; k < S; ++k) w = fma((double)weights[k], (double)(float)s[k], w);
+        cond_out[i] = (float)((double)gmax[i] * w);
     }
 }
 
@@ -762,9 +776,14 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
     hipStream_t s = (hipStream_t)stream;
 #define NW_LTI_CASE(K)                                                                             \
     case K:                                                                                        \
-        k_lti<K><<<grid, 256, 0, s>>>(num_nodes, vmin, vmax, node_table, M, voltage, seg_index,    \
-                                      states, state_stride, conserve_sum, n_steps, weights, gmax,  \
-                                      cond_out);                                                   \
+        if (n_steps > 1)                                                                           \
+            k_lti<K, true><<<grid, 256, 0, s>>>(num_nodes, vmin, vmax, node_table, M, voltage,     \
+                                                seg_index, states, state_stride, conserve_sum,     \
+                                                n_steps, weights, gmax, cond_out);                 \
+        else                                                                                       \
+            k_lti<K, false><<<grid, 256, 0, s>>>(num_nodes, vmin, vmax, node_table, M, voltage,    \
+                                                 seg_index, states, state_stride, conserve_sum,    \
+                                                 n_steps, weights, gmax, cond_out);                \
         break;
     switch (S) {
         NW_LTI_CASE(1) NW_LTI_CASE(2) NW_LTI_CASE(3) NW_LTI_CASE(4) NW_LTI_CASE(5)
