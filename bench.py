@@ -21,10 +21,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per compartment-step of the step kernel on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counter runs scaled by what the same counters read for a plain 1 GiB device copy
-# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
-MEASURED_TRAFFIC_PER_CMPT_STEP = 6890.6
+# HBM bytes per compartment-step of the step kernel on this workload, per propagator format, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs scaled by what the same counters read for a
+# plain 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); None where not measured
+MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6890.6, "symmetric": None}
+
+
+def algorithmic_bytes(fmt, nnz_bar):
+    """Bytes per compartment-step the step kernel must move: per half step the stored propagator
+    entries (FP32 value + u16 column; the symmetric format stores the diagonal and one triangle)
+    and per row the offset (4 B) and FP64 row sum (8 B) plus, symmetric, the FP64 weight (8 B);
+    per step 68 B of per-segment mechanism state and constants (gNa, gK, gL, 1/C read twice,
+    m, h, n read twice and written once)."""
+    if fmt == "symmetric":
+        return 2.0 * (6.0 * (nnz_bar + 1.0) / 2.0 + 20.0) + 68.0
+    return 2.0 * (6.0 * nnz_bar + 12.0) + 68.0
 
 
 def parse():
@@ -37,6 +48,8 @@ def parse():
     ap.add_argument("--launch-steps", type=int, default=0,
                     help="steps per kernel launch (0: all timed steps in one launch)")
     ap.add_argument("--block-threads", type=int, default=0)
+    ap.add_argument("--format", choices=("symmetric", "general"), default="symmetric",
+                    help="propagator storage: symmetric half (S = W P W^-1) or general CSR")
     ap.add_argument("--cpu-sample-cells", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -123,7 +136,8 @@ def main():
     pop = population.HHPopulation(cells, dt=0.1, celsius=6.3, prune=1e-9, gate_mode="table",
                                   stim={"mode": "poisson", "amp_na": 0.5, "rate_hz": 5.0,
                                         "duration_ms": 1.0, "seed": args.seed},
-                                  block_threads=args.block_threads)
+                                  block_threads=args.block_threads,
+                                  symmetric=args.format == "symmetric")
     st = pop.propagator.stats
     nnz_bar = pop.propagator.mean_nnz_per_row
     log(f"population ready in {time.time() - t0:.1f}s: {pop.num_segments} compartments, "
@@ -152,12 +166,12 @@ def main():
     kern_ms = sum(a.elapsed_time(b) for _, a, b in ev)
     launches = len(ev)
     value = N * K / wall
-    # algorithmic bytes per compartment-step: 2 half steps x (8 nnz + 28) + 68 mechanism bytes
-    b_step = 16.0 * nnz_bar + 124.0
+    b_step = algorithmic_bytes(args.format, nnz_bar)
     avg_launch_s = kern_ms / 1e3 / launches
     bytes_per_launch = b_step * N * (K / launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     stored_bytes_step = (st["stored"] * 6.0 * 2) / N  # FP32 value + u16 column, two half steps
+    traffic = MEASURED_TRAFFIC_PER_CMPT_STEP[args.format]
     out = {
         "metric": "compartment-steps/s @10M cmpts: FP32 batched vs FP64 plain; adjoint grads/s",
         "value": value,
@@ -172,12 +186,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded random-tree cells, Poisson stimulus)",
         "config": {"workload": "batched HH population, FP32 state, FP64-built propagator pruned 1e-9",
-                   "compartments": N, "cells": pop.num_cells, "dt_ms": 0.1,
+                   "propagator_format": args.format, "compartments": N, "cells": pop.num_cells, "dt_ms": 0.1,
                    "nnz_per_row": round(nnz_bar, 2), "steps_per_launch": L,
                    "propagator_build_s": round(st["build_seconds"], 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC_PER_CMPT_STEP * N * (K / launches),
+                     "traffic": None if traffic is None else traffic * N * (K / launches),
                      "kernel": "nw::k_hh_step", "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_cmpt_step": b_step,
                      "stored_propagator_bytes_per_cmpt_step": stored_bytes_step},

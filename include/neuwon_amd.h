@@ -47,6 +47,8 @@ int nw_device_info(int device, char* name_out, int name_len, int* num_cus, int* 
  * ------------------------------------------------------------------------------------------ */
 typedef struct nw_propagator nw_propagator;
 
+#define NW_PROPAGATOR_SYMMETRIC 1
+
 typedef struct nw_propagator_stats {
     int64_t num_cells;
     int64_t num_rows;
@@ -56,12 +58,24 @@ typedef struct nw_propagator_stats {
     int32_t max_row_nnz;
     double  build_seconds;
     int64_t device_bytes;    /* bytes held on the device by the handle                            */
+    int32_t symmetric;       /* 1: symmetric format (see nw_propagator_build_ex)                  */
+    int32_t pad_;
 } nw_propagator_stats;
 
 int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_host,
                         const int32_t* parent_local_host, const double* R_host, const double* C_host,
                         double time_step_ms, double prune, nw_propagator** out);
+/* flags = NW_PROPAGATOR_SYMMETRIC stores S = W P W^-1 (w = sqrt(C), symmetric because
+ * A = C^-1 L with L symmetric, electric.py:92-108) on and above the diagonal only, halving
+ * the stored entries; a pair (i, j) is kept when max(P_ij, P_ji) >= prune.
+ * P x = W^-1 S W x.  Used by nw_hh_step; the other entry points refuse it. */
+int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_ptr_host,
+                           const int32_t* parent_local_host, const double* R_host,
+                           const double* C_host, double time_step_ms, double prune, int32_t flags,
+                           nw_propagator** out);
 int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* out);
+/* Symmetric format only: the per-segment w of one cell. */
+int nw_propagator_cell_weights(const nw_propagator* p, int64_t cell, double* w_host);
 int nw_propagator_destroy(nw_propagator* p);
 
 /* One cell's pruned FP64 propagator, built by the same device path, copied back as CSR.
@@ -71,7 +85,9 @@ int nw_propagator_cell_f64(int device, int32_t n, const int32_t* parent_local_ho
                            double prune, int64_t* nnz_out, int64_t* row_ptr_host,
                            int32_t* col_host, double* val_host);
 
-/* Copy one cell's stored (FP32 value, u16 column) CSR back to the host (for checks). */
+/* Copy one cell's stored (FP32 value, u16 column) CSR back to the host (for checks).  Rows are
+ * padded to a multiple of 4 entries (value 0, column = the row); within each 256-entry chunk of
+ * m entries the k-th entry in column order sits at slot 4 (k mod m/4) + k / (m/4). */
 int nw_propagator_cell_export(const nw_propagator* p, int64_t cell, uint32_t* row_off_host,
                               uint16_t* col_host, float* val_host);
 
@@ -167,7 +183,7 @@ typedef struct nw_hh_params {
     float   ap_threshold;
     int32_t conductances_valid;
     int64_t step0;            /* global step index of the first step (input ticks continue)      */
-    int32_t block_threads;    /* 0 = library default                                              */
+    int32_t block_threads;    /* 128/256/512/1024; 0 = 512 (symmetric format) or 256 (general)    */
     int32_t gate_mode;        /* 0: two-state LTI tables; 1: exact exponential update of hh.mod    */
     float   q10;              /* gate_mode 1: rate factor 3^((celsius-6.3)/10), hh.mod:67         */
     int32_t pad_;

@@ -50,82 +50,200 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
+// Sum across the 64 lanes with DPP row shifts and row broadcasts (register moves, no LDS round
+// trip): within every row of 16 lanes shifts by 1, 2, 4, 8 leave the row's sum in its lane 15,
+// row_bcast:15 and row_bcast:31 carry those into lane 63.  Returns the total in every lane.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const int2 b = __builtin_bit_cast(int2, x);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(0, b.x, CTRL, ROW_MASK, 0xf, true);
+    r.y = __builtin_amdgcn_update_dpp(0, b.y, CTRL, ROW_MASK, 0xf, true);
+    return __builtin_bit_cast(double, r);
+}
+__device__ __forceinline__ double wave_total(double x) {
+    x += dpp_f64<0x111, 0xf>(x);  // row_shr:1
+    x += dpp_f64<0x112, 0xf>(x);  // row_shr:2
+    x += dpp_f64<0x114, 0xf>(x);  // row_shr:4
+    x += dpp_f64<0x118, 0xf>(x);  // row_shr:8
+    x += dpp_f64<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+    x += dpp_f64<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+    const int2 b = __builtin_bit_cast(int2, x);
+    int2 t;
+    t.x = __builtin_amdgcn_readlane(b.x, 63);
+    t.y = __builtin_amdgcn_readlane(b.y, 63);
+    return __builtin_bit_cast(double, t);
+}
+
 // Stored rows are padded to a multiple of kRowAlign entries (padding: value 0, column = the row),
 // so a lane can fetch kRowAlign consecutive values (16 B) and columns (8 B) with one vector load.
 constexpr int kRowAlign = 4;
-constexpr int kRowUnroll = 2;  // vector loads in flight per lane per round (512 entries per wave)
+// Vector loads per lane per chunk: 2 (512 entries per wave) for the general format, whose rows hold
+// ~540 entries at the 1e-9 pruning; 1 (256) for the symmetric format's ~290-entry rows.
+constexpr int kRowUnrollGeneral = 2;
+constexpr int kRowUnrollSym = 1;
+// Chunk buffers per wave (chunks in flight + 1).
+constexpr int kBuffersGeneral = 2;
+constexpr int kBuffersSym = 3;
 
+// Order of a row's entries in memory.  Lane l of a wavefront loads the 4 entries at 4l..4l+3 of
+// each 256-entry chunk; the build stores the chunk's k-th entry (column order) at slot
+//     4 (k mod m) + k / m,   m = (entries in the chunk) / 4,
+// so that for every q the lanes' q-th entries are consecutive columns: the LDS gathers of x (and
+// the scatters of the symmetric product) then touch consecutive banks instead of every fourth.
+__host__ __device__ __forceinline__ uint32_t stored_slot(uint32_t k, uint32_t row_len) {
+    constexpr uint32_t kChunkEntries = 64u * kRowAlign;
+    const uint32_t c0 = k - k % kChunkEntries;
+    const uint32_t len = (row_len - c0 < kChunkEntries) ? row_len - c0 : kChunkEntries;
+    const uint32_t m = len / kRowAlign;
+    const uint32_t i = k - c0;
+    return c0 + kRowAlign * (i % m) + i / m;
+}
+
+template <int U>
 struct RowChunk {
-    float4 a[kRowUnroll];
-    uint2 c[kRowUnroll];  // four u16 columns
+    float4 a[U];
+    uint2 c[U];  // four u16 columns
 };
 
-__device__ __forceinline__ void row_load(RowChunk& ch, const float* __restrict__ vals,
+// Loads one chunk at `base` of a row ending at `end`.  Every lane loads: lanes past the end of the
+// row reload the chunk's first entry (in bounds, every stored row has at least kRowAlign entries)
+// and row_valid() masks them out when the chunk is consumed.  Loads issued unconditionally let
+// the compiler wait for exactly one chunk while the next is in flight.
+template <int U>
+__device__ __forceinline__ void row_load(RowChunk<U>& ch, const float* __restrict__ vals,
                                          const uint16_t* __restrict__ cols, uint32_t base,
                                          uint32_t end, int lane) {
 #pragma unroll
-    for (int u = 0; u < kRowUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
         const uint32_t k = base + (uint32_t)(u * kWave + lane) * kRowAlign;
-        if (k < end) {
-            ch.a[u] = *reinterpret_cast<const float4*>(vals + k);
-            ch.c[u] = *reinterpret_cast<const uint2*>(cols + k);
-        } else {
-            ch.a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            ch.c[u] = make_uint2(0u, 0u);
-        }
+        const uint32_t kk = (k < end) ? k : base;
+        ch.a[u] = *reinterpret_cast<const float4*>(vals + kk);
+        ch.c[u] = *reinterpret_cast<const uint2*>(cols + kk);
     }
 }
 
-__device__ __forceinline__ double row_consume(const RowChunk& ch, const float* __restrict__ xoff,
-                                              double acc) {
+// Whether part u of a lane's chunk lies inside the row (`remaining` = end - base of the chunk).
+__device__ __forceinline__ bool row_valid(int u, int lane, uint32_t remaining) {
+    return (uint32_t)(u * kWave + lane) * kRowAlign < remaining;
+}
+
+template <int U>
+__device__ __forceinline__ double row_consume(const RowChunk<U>& ch, const float* __restrict__ xoff,
+                                              uint32_t remaining, int lane, double acc) {
 #pragma unroll
-    for (int u = 0; u < kRowUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
+        const bool ok = row_valid(u, lane, remaining);
+        const float4 a = ok ? ch.a[u] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t c01 = ch.c[u].x, c23 = ch.c[u].y;
-        acc = fma((double)ch.a[u].x, (double)xoff[c01 & 0xFFFFu], acc);
-        acc = fma((double)ch.a[u].y, (double)xoff[c01 >> 16], acc);
-        acc = fma((double)ch.a[u].z, (double)xoff[c23 & 0xFFFFu], acc);
-        acc = fma((double)ch.a[u].w, (double)xoff[c23 >> 16], acc);
+        acc = fma((double)a.x, (double)xoff[c01 & 0xFFFFu], acc);
+        acc = fma((double)a.y, (double)xoff[c01 >> 16], acc);
+        acc = fma((double)a.z, (double)xoff[c23 & 0xFFFFu], acc);
+        acc = fma((double)a.w, (double)xoff[c23 >> 16], acc);
     }
     return acc;
+}
+
+// Walks the stored rows first, first + stride, ... of one cell, one wavefront per row, in chunks of
+// kChunk entries, with NB - 1 chunks (2 or 3 buffers) always loading while one is consumed.  The
+// buffers are used in turn, so the loads' destination registers stay fixed and consuming one
+// chunk waits only for its own loads; every step issues its load (past the last chunk the first
+// one is loaded again, unused), which keeps those waits exact.  The offsets of the next row to
+// enter the pipeline (and rowval of it, with HAS_VAL) are fetched one row ahead.
+//   consume(chunk, r, remaining, acc) -> acc   products of one chunk of row r
+//   finish(r, acc, rowval[r])                  after the row's last chunk (acc: this lane's sum)
+struct ChunkPos {
+    int r;          // row; >= nrows once every row is done
+    uint32_t base;  // first entry of the chunk
+    uint32_t end;   // end of the row
+    double val;     // rowval[r]
+};
+
+template <int U, int NB, bool HAS_VAL, typename Consume, typename Finish>
+__device__ __forceinline__ void stream_rows(const uint32_t* __restrict__ roff,
+                                            const float* __restrict__ vals,
+                                            const uint16_t* __restrict__ cols,
+                                            const double* __restrict__ rowval, int nrows,
+                                            int first, int stride, int lane, Consume consume,
+                                            Finish finish) {
+    static_assert(NB == 2 || NB == 3, "two or three chunk buffers");
+    constexpr uint32_t kChunk = kWave * kRowAlign * U;
+    if (first >= nrows) return;
+    // row offsets and values are read-only for the kernel's lifetime and indexed by the
+    // wave-uniform row: read them through the constant address space (scalar loads, which do not
+    // enter the vector-memory counter that the chunk loads are waited on with)
+    using ConstU32 = const __attribute__((address_space(4))) uint32_t*;
+    using ConstF64 = const __attribute__((address_space(4))) double*;
+    const ConstU32 ro = (ConstU32)roff;
+    const ConstF64 rv = (ConstF64)rowval;
+    auto row_pos = [&](int r) -> ChunkPos {  // first chunk of row r (loads clamped in bounds)
+        const int rc = min(r, nrows - 1);
+        return ChunkPos{r, ro[rc], ro[rc + 1], HAS_VAL ? (double)rv[rc] : 0.0};
+    };
+    const ChunkPos p0 = row_pos(first);
+    ChunkPos pre = row_pos(first + stride);
+    auto next = [&](const ChunkPos& p) -> ChunkPos {
+        if (p.r >= nrows) return p;
+        if (p.base + kChunk < p.end) return ChunkPos{p.r, p.base + kChunk, p.end, p.val};
+        const ChunkPos q = pre;
+        pre = row_pos(min(q.r + stride, nrows));
+        return q;
+    };
+    auto load = [&](RowChunk<U>& ch, const ChunkPos& p) {
+        const bool live = p.r < nrows;
+        row_load(ch, vals, cols, live ? p.base : p0.base, live ? p.end : p0.end, lane);
+    };
+    RowChunk<U> A, B, C;
+    ChunkPos d0 = p0, d1 = next(p0);
+    load(A, d0);
+    if (NB == 3) load(B, d1);
+    double acc = 0.0;
+    // consume `cur` (chunk d0) after issuing the load of the chunk NB - 1 ahead into `ld`
+    auto step = [&](RowChunk<U>& cur, RowChunk<U>& ld) -> bool {
+        ChunkPos d2 = d1;
+        if (NB == 3) {
+            d2 = next(d1);
+            load(ld, d2);
+        } else {
+            load(ld, d1);
+        }
+        acc = consume(cur, d0.r, d0.end - d0.base, acc);
+        if (d0.base + kChunk >= d0.end) {
+            finish(d0.r, acc, d0.val);
+            acc = 0.0;
+        }
+        d0 = d1;
+        d1 = (NB == 3) ? d2 : next(d1);
+        return d0.r < nrows;
+    };
+    if (NB == 3) {
+        while (step(A, C) && step(B, A) && step(C, B)) {
+        }
+    } else {
+        while (step(A, B) && step(B, A)) {
+        }
+    }
 }
 
 // out[r] = (P x)_r - vref for the rows r = first, first + stride, ... of one cell, one wavefront
 // per row.  x is stored in LDS as FP32 offsets from vref, products accumulate in FP64 and the
 // reference part is restored with the exact FP64 sum of the stored row:
 //     P x = vref * rowsum + P (x - vref).
-// The next row's offsets and first chunk are fetched before the current row is reduced.
 __device__ __forceinline__ void spmv_rows(const uint32_t* __restrict__ roff,
                                           const float* __restrict__ vals,
                                           const uint16_t* __restrict__ cols,
                                           const double* __restrict__ rowsum,
                                           const float* __restrict__ xoff, float* __restrict__ out,
                                           double vref, int nrows, int first, int stride, int lane) {
-    int r = first;
-    if (r >= nrows) return;
-    uint32_t beg = roff[r], end = roff[r + 1];
-    RowChunk ch;
-    row_load(ch, vals, cols, beg, end, lane);
-    constexpr uint32_t kChunk = kWave * kRowAlign * kRowUnroll;
-    while (true) {
-        const int rn = r + stride;
-        uint32_t nbeg = 0, nend = 0;
-        if (rn < nrows) {
-            nbeg = roff[rn];
-            nend = roff[rn + 1];
-        }
-        double acc = row_consume(ch, xoff, 0.0);
-        for (uint32_t base = beg + kChunk; base < end; base += kChunk) {
-            row_load(ch, vals, cols, base, end, lane);
-            acc = row_consume(ch, xoff, acc);
-        }
-        if (rn < nrows) row_load(ch, vals, cols, nbeg, nend, lane);
-        const double s = fma(vref, rowsum[r], wave_sum(acc));
-        if (lane == 0) out[r] = (float)(s - vref);
-        if (rn >= nrows) break;
-        r = rn;
-        beg = nbeg;
-        end = nend;
-    }
+    stream_rows<kRowUnrollGeneral, kBuffersGeneral, true>(
+        roff, vals, cols, rowsum, nrows, first, stride, lane,
+        [&](const RowChunk<kRowUnrollGeneral>& ch, int, uint32_t remaining, double acc) {
+            return row_consume(ch, xoff, remaining, lane, acc);
+        },
+        [&](int r, double acc, double rs) {
+            const double s = fma(vref, rs, wave_total(acc));
+            if (lane == 0) out[r] = (float)(s - vref);
+        });
 }
 
 // Same product with one thread per row, for propagators whose rows hold a few entries (the
@@ -150,6 +268,40 @@ __device__ __forceinline__ void spmv_rows_lane(const uint32_t* __restrict__ roff
         }
         out[r] = (float)(fma(vref, rowsum[r], acc) - vref);
     }
+}
+
+// Symmetric format: z += S y with S stored on and above the diagonal.  One wavefront per stored
+// row: the row part is reduced across the lanes, the transposed part (S_rj y_r into z_j, j > r)
+// goes through LDS FP64 atomics.  z must be zeroed before and is complete after the block's
+// barrier.
+__device__ __forceinline__ void spmv_sym_rows(const uint32_t* __restrict__ roff,
+                                              const float* __restrict__ vals,
+                                              const uint16_t* __restrict__ cols,
+                                              const float* __restrict__ ys, double* zs, int nrows,
+                                              int first, int stride, int lane) {
+    stream_rows<kRowUnrollSym, kBuffersSym, false>(
+        roff, vals, cols, nullptr, nrows, first, stride, lane,
+        [&](const RowChunk<kRowUnrollSym>& ch, int r, uint32_t remaining, double acc) {
+            const double yr = ys[r];
+#pragma unroll
+            for (int u = 0; u < kRowUnrollSym; ++u) {
+                const bool ok = row_valid(u, lane, remaining);
+                const uint32_t cc[4] = {ch.c[u].x & 0xFFFFu, ch.c[u].x >> 16, ch.c[u].y & 0xFFFFu,
+                                        ch.c[u].y >> 16};
+                const float av[4] = {ch.a[u].x, ch.a[u].y, ch.a[u].z, ch.a[u].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double sv = ok ? (double)av[q] : 0.0;
+                    acc = fma(sv, (double)ys[cc[q]], acc);
+                    if (ok && cc[q] != (uint32_t)r) atomicAdd(&zs[cc[q]], sv * yr);
+                }
+            }
+            return acc;
+        },
+        [&](int r, double acc, double) {
+            acc = wave_total(acc);
+            if (lane == 0) atomicAdd(&zs[r], acc);
+        });
 }
 
 // Rows of at most this many stored entries use one thread per row.
@@ -220,6 +372,10 @@ struct nw_propagator {
     float* vals = nullptr;         // [nnz]
     uint16_t* cols = nullptr;      // [nnz] column local to the cell
     double* rowsum = nullptr;      // [num_rows] exact FP64 sum of every stored FP32 row
+    // symmetric format: rows hold S = W P W^-1 (W = diag(w), w = sqrt(C)) on and above the
+    // diagonal; P x = W^-1 S W x
+    int symmetric = 0;
+    double* wgt = nullptr;         // [num_rows] w of every segment (symmetric format)
     int32_t* order = nullptr;      // cells by decreasing stored entries (launch order)
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;

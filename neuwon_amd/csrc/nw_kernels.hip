@@ -46,6 +46,7 @@ struct HHArgs {
     int64_t trace_stride;
     float* record;       // optional: per step 5 arrays (v, m, h, n at the start, v after half A)
     int64_t num_rows;
+    const double* wgt;   // symmetric format: w of every segment
 };
 
 // Stimulus current (A) of the cell's stimulated segment on input tick `tick` (1-based).
@@ -113,12 +114,25 @@ __device__ __forceinline__ void hh_conductance(double gna_bar, double gk_bar, do
 }
 
 // Every per-segment quantity is computed in FP64 registers from FP32 storage; voltages live in LDS
-// as FP32 offsets from the cell's reference voltage and the row products accumulate in FP64 (spmv_rows).
-template <int BLOCK, int GATE_MODE>
+// as FP32 offsets from the cell's reference voltage and the row products accumulate in FP64.
+//
+// SYM = false: general CSR (spmv_rows); LDS holds x - vref and v - vref (FP32).
+// SYM = true:  symmetric CSR of S = W P W^-1 (spmv_sym_rows); LDS holds y = W (x - vref) (FP32) and
+//              z = S y (FP64), from which v_i = vref * rowsum_i + z_i / w_i.  z is cleared by the
+//              thread that reads it, before the next product accumulates into it.
+template <int BLOCK, int GATE_MODE, bool SYM>
 __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
-    extern __shared__ float lds[];
-    float* xs = lds;            // x - vref of the current half step
-    float* vs = lds + a.nmax;   // v - vref
+    extern __shared__ double lds64[];
+    float* xs;                  // x - vref (general) or y (symmetric) of the current half step
+    float* vs = nullptr;        // v - vref (general)
+    double* zs = nullptr;       // z (symmetric)
+    if (SYM) {
+        zs = lds64;
+        xs = reinterpret_cast<float*>(lds64 + a.nmax);
+    } else {
+        xs = reinterpret_cast<float*>(lds64);
+        vs = xs + a.nmax;
+    }
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -127,6 +141,7 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
     const float* vals = a.vals + a.cell_nz0[cell];
     const uint16_t* cols = a.cols + a.cell_nz0[cell];
     const double* rs = a.rowsum + row0;
+    const double* wg = SYM ? a.wgt + row0 : nullptr;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     constexpr int NW = BLOCK / kWave;
@@ -149,7 +164,30 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
     }
 
     const double vref = (double)a.v[row0];  // the cell's reference voltage for this launch
-    for (int i = threadIdx.x; i < nrows; i += BLOCK) vs[i] = (float)((double)a.v[row0 + i] - vref);
+    // voltage of segment i after a product; in the symmetric form the thread that owns i clears z_i
+    auto take_v = [&](int i) -> double {
+        if (SYM) {
+            const double z = zs[i];
+            zs[i] = 0.0;
+            return fma(vref, rs[i], z / wg[i]);
+        }
+        return vref + (double)vs[i];
+    };
+    auto peek_v = [&](int i) -> double {
+        return SYM ? fma(vref, rs[i], zs[i] / wg[i]) : vref + (double)vs[i];
+    };
+    auto put_x = [&](int i, double x) {
+        xs[i] = SYM ? (float)((x - vref) * wg[i]) : (float)(x - vref);
+    };
+    auto product = [&]() {
+        if (SYM) spmv_sym_rows(roff, vals, cols, xs, zs, nrows, wave, NW, lane);
+        else spmv_rows(roff, vals, cols, rs, xs, vs, vref, nrows, wave, NW, lane);
+    };
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+        const double v = (double)a.v[row0 + i];
+        if (SYM) zs[i] = (v - vref * rs[i]) * wg[i];
+        else vs[i] = (float)(v - vref);
+    }
     __syncthreads();
 
     for (int step = 0; step < a.n_steps; ++step) {
@@ -160,14 +198,14 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
             const double invc = a.inv_c[gi];
-            const double v = vref + (double)vs[i];
+            const double v = take_v(i);
             double g = 0.0, d = 0.0;
             if (g_valid)
                 hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], a.m[gi], a.h[gi], a.n[gi], e_na,
                                e_k, e_l, g, d);
             const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick_a) : 0.0;
             const double alpha = exp(-hdt_s * g * invc);
-            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - vref);
+            put_x(i, d - (d - v) * alpha + hdt * I * invc);
             if (a.record) {
                 float* rec = a.record + (int64_t)step * 5 * a.num_rows + gi;
                 rec[0] = (float)v;
@@ -178,12 +216,12 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
         }
         __syncthreads();
         // ---- B: v = P x
-        spmv_rows(roff, vals, cols, rs, xs, vs, vref, nrows, wave, NW, lane);
+        product();
         __syncthreads();
         // ---- C: mechanisms at v(t + dt/2), then relax for the second half step
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
-            const double v = vref + (double)vs[i];
+            const double v = take_v(i);
             if (a.record) a.record[((int64_t)step * 5 + 4) * a.num_rows + gi] = (float)v;
             double m = a.m[gi], h = a.h[gi], n = a.n[gi];
             if (GATE_MODE == 1) {
@@ -206,22 +244,25 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
             const double I =
                 (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick_a + 1) : 0.0;
             const double alpha = exp(-hdt_s * g * invc);
-            xs[i] = (float)((d - (d - v) * alpha + hdt * I * invc) - vref);
+            put_x(i, d - (d - v) * alpha + hdt * I * invc);
         }
         __syncthreads();
         // ---- D: v = P x
-        spmv_rows(roff, vals, cols, rs, xs, vs, vref, nrows, wave, NW, lane);
+        product();
         __syncthreads();
-        if (threadIdx.x == 0 && ap_row >= 0) {
-            const double vap = vref + (double)vs[ap_row];
-            const bool over = vap >= (double)p.ap_threshold;
-            if (over && !ap_state) ++ap_count;
-            ap_state = over ? 1 : 0;
-            if (a.trace) a.trace[(int64_t)cell * a.trace_stride + step] = (float)vap;
+        if (ap_row >= 0) {
+            if (threadIdx.x == 0) {
+                const double vap = peek_v(ap_row);
+                const bool over = vap >= (double)p.ap_threshold;
+                if (over && !ap_state) ++ap_count;
+                ap_state = over ? 1 : 0;
+                if (a.trace) a.trace[(int64_t)cell * a.trace_stride + step] = (float)vap;
+            }
+            if (SYM) __syncthreads();  // z[ap_row] is cleared in the next phase A
         }
     }
     for (int i = threadIdx.x; i < nrows; i += BLOCK) {
-        const float vf = (float)(vref + (double)vs[i]);
+        const float vf = (float)peek_v(i);
         a.v[row0 + i] = vf;
         if (a.record) {  // the state after the last step closes the record
             float* rec = a.record + (int64_t)a.n_steps * 5 * a.num_rows + row0 + i;
@@ -697,17 +738,26 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
     HHArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
              p->max_cell_rows,
              n_steps, *prm, gate_table, v, m, h, n, g_na, g_k, g_l, inv_c, stim_local, ap_local,
-             ap_state, ap_count, trace, trace_stride, record, p->num_rows};
-    const size_t lds = 2 * sizeof(float) * (size_t)p->max_cell_rows;
+             ap_state, ap_count, trace, trace_stride, record, p->num_rows, p->wgt};
+    const bool sym = p->symmetric != 0;
+    const size_t lds = (sym ? sizeof(double) + sizeof(float) : 2 * sizeof(float)) *
+                       (size_t)p->max_cell_rows;
     NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
     hipStream_t s = (hipStream_t)stream;
-    const int bt = prm->block_threads ? prm->block_threads : 256;
+    // default: 512 threads for the symmetric format (two workgroups per CU for 5000-segment cells
+    // under the LDS budget), 256 for the general one (flat from 256 to 1024 at 10M compartments)
+    const int bt = prm->block_threads ? prm->block_threads : (sym ? 512 : 256);
     const unsigned grid = (unsigned)p->num_cells;
     NW_REQUIRE(prm->gate_mode == 0 || prm->gate_mode == 1, "gate_mode must be 0 or 1");
 #define NW_HH_LAUNCH(B)                                                                   \
     case B:                                                                               \
-        if (prm->gate_mode == 1) k_hh_step<B, 1><<<grid, B, lds, s>>>(a);                 \
-        else k_hh_step<B, 0><<<grid, B, lds, s>>>(a);                                     \
+        if (sym) {                                                                        \
+            if (prm->gate_mode == 1) k_hh_step<B, 1, true><<<grid, B, lds, s>>>(a);       \
+            else k_hh_step<B, 0, true><<<grid, B, lds, s>>>(a);                           \
+        } else {                                                                          \
+            if (prm->gate_mode == 1) k_hh_step<B, 1, false><<<grid, B, lds, s>>>(a);      \
+            else k_hh_step<B, 0, false><<<grid, B, lds, s>>>(a);                          \
+        }                                                                                 \
         break;
     switch (bt) {
         NW_HH_LAUNCH(128) NW_HH_LAUNCH(256) NW_HH_LAUNCH(512) NW_HH_LAUNCH(1024)
@@ -729,6 +779,7 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
                    grad_theta && loss,
                "nw_hh_adjoint: null argument");
     NW_REQUIRE(prm->gate_mode == 0, "the adjoint is written for the table gate mode");
+    NW_REQUIRE(!p->symmetric, "nw_hh_adjoint takes the general (non-symmetric) format");
     NW_REQUIRE(prm->stim_mode == 0 || prm->stim_mode == 1, "adjoint supports constant stimulus only");
     NW_REQUIRE(n_steps >= 0 && prm->num_nodes >= 2, "bad arguments");
     if (n_steps == 0) return NW_OK;
@@ -800,6 +851,7 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
 int nw_ca_step(const nw_propagator* p, const nw_ca_params* prm, int32_t n_steps, float* ca,
                float* cab, float* dcdt, void* stream) {
     NW_REQUIRE(p && prm && ca && cab && dcdt, "nw_ca_step: null argument");
+    NW_REQUIRE(!p->symmetric, "nw_ca_step takes the general (non-symmetric) format");
     NW_REQUIRE(n_steps >= 0 && prm->dt_ms > 0.0f, "bad step count or time step");
     if (n_steps == 0) return NW_OK;
     CaArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,

@@ -120,30 +120,128 @@ __global__ void k_fill(const double* P, int n, double prune, const uint32_t* row
         diag32 = (float)(s64 - s32);
         if (lane == 0) rowsum[row] = s32 + (double)diag32;
     }
-    uint32_t pos = row_off_local[row];
+    const uint32_t beg = row_off_local[row], len = row_off_local[row + 1] - beg;
+    uint32_t pos = 0;  // entries of the row written so far (column order)
     for (int j0 = 0; j0 < n; j0 += kWave) {
         int j = j0 + lane;
         double p = (j < n) ? pr[j] : 0.0;
         bool keep = (j < n) && (j == row || fabs(p) >= prune);
         uint64_t mask = __ballot(keep);
         if (keep) {
-            uint32_t at = pos + (uint32_t)__popcll(mask & lt);
+            uint32_t k = pos + (uint32_t)__popcll(mask & lt);
             if (kF32) {
+                const uint32_t at = beg + stored_slot(k, len);
                 vals32[at] = (j == row) ? diag32 : (float)p;
                 cols16[at] = (uint16_t)j;
             } else {
-                vals64[at] = p;
-                cols32[at] = j;
+                vals64[beg + k] = p;
+                cols32[beg + k] = j;
             }
         }
         pos += (uint32_t)__popcll(mask);
     }
     if (kF32) {  // padding up to the next row: value 0, column = the row itself
-        const uint32_t stop = row_off_local[row + 1];
-        for (uint32_t q = pos + lane; q < stop; q += kWave) {
-            vals32[q] = 0.0f;
-            cols16[q] = (uint16_t)row;
+        for (uint32_t k = pos + lane; k < len; k += kWave) {
+            const uint32_t at = beg + stored_slot(k, len);
+            vals32[at] = 0.0f;
+            cols16[at] = (uint16_t)row;
         }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Symmetric format.  S = W P W^-1 with w = sqrt(C) is symmetric (A = C^-1 L, L symmetric); the
+// two triangles of the computed P are averaged into S so S_ij and S_ji are bitwise equal.
+// A pair (i, j) is kept when max(P_ij, P_ji) >= prune (the reference prunes P entry by entry,
+// electric.py:114; one triangle can only store the pair, so the pair is kept if either entry is).
+// ----------------------------------------------------------------------------------------------
+__global__ void k_symmetrize(const double* __restrict__ P, const double* __restrict__ w, int n,
+                             double* __restrict__ S) {
+    __shared__ double tile[32][33];
+    const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {             // tile of P at (bj, bi), transposed into LDS
+        const int r = bj + k, c = bi + tx;
+        tile[tx][k] = (r < n && c < n) ? P[(int64_t)r * n + c] : 0.0;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int i = bi + k, j = bj + tx;
+        if (i < n && j < n) {
+            const double pij = P[(int64_t)i * n + j], pji = tile[k][tx];
+            S[(int64_t)i * n + j] = 0.5 * ((w[i] / w[j]) * pij + (w[j] / w[i]) * pji);
+        }
+    }
+}
+
+__device__ __forceinline__ bool keep_pair(double sij, double wi, double wj, double prune) {
+    const double r = wj / wi;
+    return fabs(sij) * fmax(r, 1.0 / r) >= prune;
+}
+
+__global__ void k_count_sym(const double* S, const double* w, int n, double prune, int32_t* counts) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (row >= n) return;
+    const double* sr = S + (int64_t)row * n;
+    const double wi = w[row];
+    int cnt = 1;  // the diagonal
+    for (int j0 = row + 1; j0 < n; j0 += kWave) {
+        const int j = j0 + lane;
+        const bool keep = (j < n) && keep_pair(sr[j], wi, w[j], prune);
+        cnt += __popcll(__ballot(keep));
+    }
+    if (lane == 0) counts[row] = cnt;
+}
+
+// Row `row` of the symmetric store: the diagonal, then kept entries j > row in column order
+// (placed by stored_slot).  The
+// FP32 diagonal is chosen so that the implied FP32 row of P, sum_j S32_ij w_j / w_i over the whole
+// row (both triangles), sums in FP64 to the FP64 row sum of P over the kept pairs.
+__global__ void k_fill_sym(const double* S, const double* w, int n, double prune,
+                           const uint32_t* row_off_local, float* vals32, uint16_t* cols16,
+                           double* rowsum, double* wgt) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (row >= n) return;
+    const double* sr = S + (int64_t)row * n;
+    const double wi = w[row];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    double s64 = 0.0, s32 = 0.0;
+    for (int j0 = 0; j0 < n; j0 += kWave) {
+        const int j = j0 + lane;
+        if (j < n && j != row && keep_pair(sr[j], wi, w[j], prune)) {
+            const double r = w[j] / wi;
+            s64 += sr[j] * r;
+            s32 += (double)(float)sr[j] * r;
+        }
+    }
+    s64 = wave_sum(s64) + sr[row];
+    s32 = wave_sum(s32);
+    const float diag32 = (float)(s64 - s32);
+    const uint32_t beg = row_off_local[row], len = row_off_local[row + 1] - beg;
+    if (lane == 0) {
+        rowsum[row] = s32 + (double)diag32;
+        wgt[row] = wi;
+        vals32[beg + stored_slot(0, len)] = diag32;
+        cols16[beg + stored_slot(0, len)] = (uint16_t)row;
+    }
+    uint32_t pos = 1;  // entries of the row written so far (column order)
+    for (int j0 = row + 1; j0 < n; j0 += kWave) {
+        const int j = j0 + lane;
+        const bool keep = (j < n) && keep_pair(sr[j], wi, w[j], prune);
+        const uint64_t mask = __ballot(keep);
+        if (keep) {
+            const uint32_t at = beg + stored_slot(pos + (uint32_t)__popcll(mask & lt), len);
+            vals32[at] = (float)sr[j];
+            cols16[at] = (uint16_t)j;
+        }
+        pos += (uint32_t)__popcll(mask);
+    }
+    for (uint32_t k = pos + lane; k < len; k += kWave) {
+        const uint32_t at = beg + stored_slot(k, len);
+        vals32[at] = 0.0f;
+        cols16[at] = (uint16_t)row;
     }
 }
 
@@ -154,7 +252,7 @@ struct Workspace {
     int nmax = 0;
     double* X[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int32_t *d_ri = nullptr, *d_ci = nullptr, *d_counts = nullptr;
-    double *d_v = nullptr, *d_diag = nullptr;
+    double *d_v = nullptr, *d_diag = nullptr, *d_w = nullptr;
     uint32_t* d_off = nullptr;
     rocblas_handle blas = nullptr;
     hipStream_t stream = nullptr;
@@ -167,6 +265,7 @@ struct Workspace {
         NW_HIP(hipMalloc(&d_ci, sizeof(int32_t) * 2 * n));
         NW_HIP(hipMalloc(&d_v, sizeof(double) * 2 * n));
         NW_HIP(hipMalloc(&d_diag, sizeof(double) * n));
+        NW_HIP(hipMalloc(&d_w, sizeof(double) * n));
         NW_HIP(hipMalloc(&d_counts, sizeof(int32_t) * n));
         NW_HIP(hipMalloc(&d_off, sizeof(uint32_t) * (n + 1)));
         NW_HIP(hipStreamCreate(&stream));
@@ -182,6 +281,7 @@ struct Workspace {
         if (d_ci) (void)hipFree(d_ci);
         if (d_v) (void)hipFree(d_v);
         if (d_diag) (void)hipFree(d_diag);
+        if (d_w) (void)hipFree(d_w);
         if (d_counts) (void)hipFree(d_counts);
         if (d_off) (void)hipFree(d_off);
         if (blas) rocblas_destroy_handle(blas);
@@ -270,10 +370,15 @@ static int expm_cell(Workspace& w, int n, const int32_t* parent, const double* R
 
 // Row counts -> local CSR offsets on the host (each row rounded up to `align` entries).
 static int count_rows(Workspace& w, const double* P, int n, double prune, int align,
-                      std::vector<uint32_t>& off, int32_t& max_row, int64_t& kept) {
+                      std::vector<uint32_t>& off, int32_t& max_row, int64_t& kept,
+                      bool sym = false) {
     const int rows_per_block = 4;
-    k_count<<<(n + rows_per_block - 1) / rows_per_block, rows_per_block * kWave, 0, w.stream>>>(
-        P, n, prune, w.d_counts);
+    if (sym)
+        k_count_sym<<<(n + rows_per_block - 1) / rows_per_block, rows_per_block * kWave, 0,
+                      w.stream>>>(P, w.d_w, n, prune, w.d_counts);
+    else
+        k_count<<<(n + rows_per_block - 1) / rows_per_block, rows_per_block * kWave, 0,
+                  w.stream>>>(P, n, prune, w.d_counts);
     NW_HIP(hipGetLastError());
     std::vector<int32_t> counts(n);
     NW_HIP(hipMemcpyAsync(counts.data(), w.d_counts, n * sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -286,7 +391,7 @@ static int count_rows(Workspace& w, const double* P, int n, double prune, int al
     for (int i = 0; i < n; ++i) {
         off[i] = (uint32_t)acc;
         acc += (uint64_t)((counts[i] + align - 1) / align * align);
-        kept += counts[i];
+        kept += sym ? 2 * (int64_t)counts[i] - 1 : counts[i];
         max_row = std::max(max_row, counts[i]);
     }
     NW_REQUIRE(acc < 0xFFFFFFFFull, "cell block too large for 32-bit row offsets");
@@ -325,12 +430,16 @@ static void free_prop(nw_propagator* p) {
     if (p->vals) (void)hipFree(p->vals);
     if (p->cols) (void)hipFree(p->cols);
     if (p->rowsum) (void)hipFree(p->rowsum);
+    if (p->wgt) (void)hipFree(p->wgt);
     if (p->order) (void)hipFree(p->order);
     delete p;
 }
 
 static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const int32_t* parent,
-                 const double* R, const double* C, double dt_ms, double prune, nw_propagator* p) {
+                 const double* R, const double* C, double dt_ms, double prune, int flags,
+                 nw_propagator* p) {
+    const bool sym = (flags & NW_PROPAGATOR_SYMMETRIC) != 0;
+    p->symmetric = sym ? 1 : 0;
     auto t0 = std::chrono::steady_clock::now();
     NW_REQUIRE(num_cells > 0, "num_cells must be positive");
     NW_REQUIRE(cell_ptr[0] == 0, "cell_ptr[0] must be 0");
@@ -351,6 +460,7 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
     p->h_cell_nz0.assign(num_cells, 0);
     NW_HIP(hipMalloc(&p->row_off, sizeof(uint32_t) * (p->num_rows + num_cells)));
     NW_HIP(hipMalloc(&p->rowsum, sizeof(double) * p->num_rows));
+    if (sym) NW_HIP(hipMalloc(&p->wgt, sizeof(double) * p->num_rows));
     // capacity guess; grown on demand
     int64_t guess = 0;
     for (int64_t c = 0; c < num_cells; ++c) {
@@ -369,18 +479,34 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
         int n = (int)(cell_ptr[c + 1] - r0);
         double* P = nullptr;
         if ((rc = expm_cell(w, n, parent + r0, R + r0, C + r0, dt_ms, &P))) return rc;
+        if (sym) {  // S = W P W^-1, w = sqrt(C / C_first) (any per-cell scale cancels)
+            std::vector<double> wv(n);
+            for (int i = 0; i < n; ++i) wv[i] = std::sqrt(C[r0 + i] / C[r0]);
+            NW_HIP(hipMemcpyAsync(w.d_w, wv.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                  w.stream));
+            double* Sd = (P == w.X[0]) ? w.X[1] : w.X[0];
+            dim3 blk(32, 8), grd((n + 31) / 32, (n + 31) / 32);
+            k_symmetrize<<<grd, blk, 0, w.stream>>>(P, w.d_w, n, Sd);
+            NW_HIP(hipGetLastError());
+            P = Sd;
+        }
         int32_t max_row = 0;
         int64_t kept = 0;
-        if ((rc = count_rows(w, P, n, prune, kRowAlign, off, max_row, kept))) return rc;
+        if ((rc = count_rows(w, P, n, prune, kRowAlign, off, max_row, kept, sym))) return rc;
         p->max_row_nnz = std::max(p->max_row_nnz, max_row);
         int64_t cnt = off[n];
         if ((rc = grow(p, p->stored + cnt))) return rc;
         uint32_t* dev_off = p->row_off + r0 + c;
         NW_HIP(hipMemcpyAsync(dev_off, off.data(), (n + 1) * sizeof(uint32_t),
                               hipMemcpyHostToDevice, w.stream));
-        k_fill<true><<<(n + 3) / 4, 4 * kWave, 0, w.stream>>>(
-            P, n, prune, dev_off, p->vals + p->stored, p->cols + p->stored, p->rowsum + r0,
-            nullptr, nullptr);
+        if (sym)
+            k_fill_sym<<<(n + 3) / 4, 4 * kWave, 0, w.stream>>>(
+                P, w.d_w, n, prune, dev_off, p->vals + p->stored, p->cols + p->stored,
+                p->rowsum + r0, p->wgt + r0);
+        else
+            k_fill<true><<<(n + 3) / 4, 4 * kWave, 0, w.stream>>>(
+                P, n, prune, dev_off, p->vals + p->stored, p->cols + p->stored, p->rowsum + r0,
+                nullptr, nullptr);
         NW_HIP(hipGetLastError());
         p->h_cell_nz0[c] = p->stored;
         cell_nnz[c] = cnt;
@@ -499,12 +625,21 @@ int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_h
                         const int32_t* parent_local_host, const double* R_host,
                         const double* C_host, double time_step_ms, double prune,
                         nw_propagator** out) {
+    return nw_propagator_build_ex(device, num_cells, cell_ptr_host, parent_local_host, R_host,
+                                  C_host, time_step_ms, prune, 0, out);
+}
+
+int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_ptr_host,
+                           const int32_t* parent_local_host, const double* R_host,
+                           const double* C_host, double time_step_ms, double prune, int32_t flags,
+                           nw_propagator** out) {
     NW_REQUIRE(out && cell_ptr_host && parent_local_host && R_host && C_host,
                "nw_propagator_build: null argument");
+    NW_REQUIRE((flags & ~NW_PROPAGATOR_SYMMETRIC) == 0, "unknown propagator flags %d", flags);
     *out = nullptr;
     auto* p = new nw_propagator();
     int rc = build(device, num_cells, cell_ptr_host, parent_local_host, R_host, C_host,
-                   time_step_ms, prune, p);
+                   time_step_ms, prune, flags, p);
     if (rc) {
         free_prop(p);
         return rc;
@@ -522,9 +657,10 @@ int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
     s->max_cell_rows = p->max_cell_rows;
     s->max_row_nnz = p->max_row_nnz;
     s->build_seconds = p->build_seconds;
+    s->symmetric = p->symmetric;
     s->device_bytes = p->vals_capacity * (int64_t)(sizeof(float) + sizeof(uint16_t)) +
                       (p->num_rows + p->num_cells) * (int64_t)sizeof(uint32_t) +
-                      p->num_rows * (int64_t)sizeof(double) +
+                      p->num_rows * (int64_t)sizeof(double) * (p->symmetric ? 2 : 1) +
                       p->num_cells * (int64_t)(2 * sizeof(int64_t) + sizeof(int32_t));
     return NW_OK;
 }
@@ -577,6 +713,15 @@ int nw_propagator_cell_f64(int device, int32_t n, const int32_t* parent_local_ho
     return NW_OK;
 }
 
+int nw_propagator_cell_weights(const nw_propagator* p, int64_t cell, double* w_host) {
+    NW_REQUIRE(p && w_host && cell >= 0 && cell < p->num_cells, "bad argument");
+    NW_REQUIRE(p->symmetric, "the propagator is not in the symmetric format");
+    int64_t r0 = p->h_cell_row0[cell];
+    int n = (int)(p->h_cell_row0[cell + 1] - r0);
+    NW_HIP(hipMemcpy(w_host, p->wgt + r0, n * sizeof(double), hipMemcpyDeviceToHost));
+    return NW_OK;
+}
+
 int nw_propagator_cell_export(const nw_propagator* p, int64_t cell, uint32_t* row_off_host,
                               uint16_t* col_host, float* val_host) {
     NW_REQUIRE(p && cell >= 0 && cell < p->num_cells, "cell index out of range");
@@ -599,6 +744,7 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
                           const float* drive_v, const float* sum_i, const float* inv_c,
                           float* integral_v, void* stream) {
     NW_REQUIRE(p && v && sum_g && drive_v && sum_i && inv_c, "nw_electric_half_step: null argument");
+    NW_REQUIRE(!p->symmetric, "nw_electric_half_step takes the general (non-symmetric) format");
     HalfArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
                p->max_cell_rows, half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v};
     size_t lds = 2 * sizeof(float) * (size_t)p->max_cell_rows;

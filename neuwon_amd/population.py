@@ -22,21 +22,28 @@ def _torch():
 
 
 class Propagator:
-    """Owner of an ``nw_propagator`` handle (per-cell pruned CSR, FP32 values)."""
+    """Owner of an ``nw_propagator`` handle (per-cell pruned CSR, FP32 values).
 
-    def __init__(self, cell_ptr, parent_local, R, C, half_dt_ms, prune, device=0):
+    symmetric=True stores S = W P W^-1 (w = sqrt(C)) on and above the diagonal only; a pair (i, j)
+    is then kept when max(P_ij, P_ji) >= prune.  Only the fused HH step reads that format.
+    """
+
+    def __init__(self, cell_ptr, parent_local, R, C, half_dt_ms, prune, device=0, symmetric=False):
         lib = _lib.load()
         self.cell_ptr = np.ascontiguousarray(cell_ptr, dtype=np.int64)
         self._parent = np.ascontiguousarray(parent_local, dtype=np.int32)
         self._R = np.ascontiguousarray(R, dtype=np.float64)
         self._C = np.ascontiguousarray(C, dtype=np.float64)
         self.half_dt_ms, self.prune, self.device = float(half_dt_ms), float(prune), int(device)
+        self.symmetric = bool(symmetric)
         h = ctypes.c_void_p()
         with _lib.lock():
-            _lib.check(lib.nw_propagator_build(self.device, len(self.cell_ptr) - 1,
-                                               _lib.ptr(self.cell_ptr), _lib.ptr(self._parent),
-                                               _lib.ptr(self._R), _lib.ptr(self._C),
-                                               self.half_dt_ms, self.prune, ctypes.byref(h)))
+            _lib.check(lib.nw_propagator_build_ex(self.device, len(self.cell_ptr) - 1,
+                                                  _lib.ptr(self.cell_ptr), _lib.ptr(self._parent),
+                                                  _lib.ptr(self._R), _lib.ptr(self._C),
+                                                  self.half_dt_ms, self.prune,
+                                                  _lib.NW_PROPAGATOR_SYMMETRIC if symmetric else 0,
+                                                  ctypes.byref(h)))
         self.handle = h
         st = _lib.NwPropagatorStats()
         _lib.check(lib.nw_propagator_stats_get(self.handle, ctypes.byref(st)))
@@ -47,6 +54,25 @@ class Propagator:
         return self.stats["nnz"] / max(1, self.stats["num_rows"])
 
     def export_cell(self, cell):
+        """The FP32 propagator block of one cell as a scipy matrix (FP64 values).  In the
+        symmetric format P_ij = S_ij w_j / w_i is rebuilt from the stored triangle."""
+        m = self.export_cell_stored(cell)
+        if not self.symmetric:
+            return m
+        w = self.cell_weights(cell)
+        full = (m + m.T - _diag_part(m)).tocsr()
+        full = (full.multiply(w[None, :]).multiply(1.0 / w[:, None])).tocsr()
+        full.eliminate_zeros()
+        return full
+
+    def cell_weights(self, cell):
+        """w of every segment of one cell (symmetric format)."""
+        n = int(self.cell_ptr[cell + 1] - self.cell_ptr[cell])
+        w = np.empty(n, dtype=np.float64)
+        _lib.check(_lib.load().nw_propagator_cell_weights(self.handle, cell, _lib.ptr(w)))
+        return w
+
+    def export_cell_stored(self, cell):
         """Stored (FP32 value, u16 column) CSR block of one cell as a scipy matrix (FP64)."""
         import scipy.sparse
         n = int(self.cell_ptr[cell + 1] - self.cell_ptr[cell])
@@ -72,6 +98,11 @@ class Propagator:
             self.close()
         except Exception:
             pass
+
+
+def _diag_part(m):
+    import scipy.sparse
+    return scipy.sparse.diags(m.diagonal(), format="csr")
 
 
 def cell_propagator_f64(parent_local, R, C, half_dt_ms, prune, device=0):
@@ -103,11 +134,13 @@ class HHPopulation:
 
     stim: dict(mode='constant'|'poisson'|None, amp_na, rate_hz, duration_ms, seed) applied to the
     root (segment 0) of every cell.
+    symmetric: store the propagator in the symmetric half format (half the bytes per step).
     """
 
     def __init__(self, cells, dt=0.1, celsius=6.3, v0=-70.0, cytoplasmic_resistance=100.0,
                  membrane_capacitance=1.0, gbar=None, reversal=None, prune=1e-9, gate_mode="table",
-                 stim=None, ap_threshold=20.0, device=0, block_threads=0, insert_area="final"):
+                 stim=None, ap_threshold=20.0, device=0, block_threads=0, insert_area="final",
+                 symmetric=False):
         torch = _torch()
         self.torch = torch
         self.dev = torch.device("cuda", device)
@@ -124,7 +157,7 @@ class HHPopulation:
                                 cytoplasmic_resistance, membrane_capacitance)
         self.R, self.C, self.geo, self.parent_local = R, C, geo, g["parent_local"]
         self.propagator = Propagator(self.cell_ptr, g["parent_local"], R, C, 0.5 * self.dt, prune,
-                                     device)
+                                     device, symmetric=symmetric)
         gbar = dict(channels.HH_GBAR, **(gbar or {}))
         self.reversal = dict(channels.HH_REVERSAL, **(reversal or {}))
         # magnitude = surface area when the mechanism is inserted: after the cell is grown
