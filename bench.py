@@ -27,12 +27,20 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6890.6, "symmetric": 3753.3}
 
 
-def algorithmic_bytes(fmt, nnz_bar):
-    """Bytes per compartment-step the step kernel must move: per half step the stored propagator
-    entries (FP32 value + u16 column; the symmetric format stores the diagonal and one triangle)
-    and per row the offset (4 B) and FP64 row sum (8 B) plus, symmetric, the FP64 weight (8 B);
-    per step 68 B of per-segment mechanism state and constants (gNa, gK, gL, 1/C read twice,
-    m, h, n read twice and written once)."""
+def algorithmic_bytes(nnz_bar):
+    """Canonical algorithmic bytes per compartment-step of the fused step: two half steps of
+    8 nnz + 28 B (FP32 value + i32 index per kept entry, row pointer, v/g/d/I/(1/C) reads, v write)
+    and a 68-B HH mechanism pass.  nnz_bar = kept entries per row (both triangles).  The canonical
+    count stays the same whatever the storage format, so a compressed format shows up as an
+    effective bandwidth above what it moves."""
+    return 16.0 * nnz_bar + 124.0
+
+
+def format_bytes(fmt, nnz_bar):
+    """Bytes per compartment-step the stored format itself needs: per half step the stored entries
+    (FP32 value + u16 column; the symmetric format stores the diagonal and one triangle) and per row
+    the offset (4 B) and FP64 row sum (8 B) plus, symmetric, the FP64 weight (8 B); per step 68 B of
+    per-segment mechanism state and constants."""
     if fmt == "symmetric":
         return 2.0 * (6.0 * (nnz_bar + 1.0) / 2.0 + 20.0) + 68.0
     return 2.0 * (6.0 * nnz_bar + 12.0) + 68.0
@@ -53,6 +61,8 @@ def parse():
     ap.add_argument("--cpu-sample-cells", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the gradients/s measurement of the reverse pass")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -102,6 +112,39 @@ def cpu_baseline(pop, args):
             "sample": f"first {sample} cells ({r_end} compartments, {P.nnz} FP64 propagator entries) "
                       f"of the same population; {rates[1][1]} steps on 1 thread, {rates[cores][1]} "
                       f"steps on {cores} threads; oracle C FP64 restatement (OpenMP over cells)"}
+
+
+def adjoint_rate(seed=4):
+    """Gradients per second of the trajectory loss through the fused step: 1024 cells x 500
+    compartments, 2000 steps at dt 0.05 ms, checkpoints every 100 steps, targets from a forward run
+    with hidden per-segment conductances (logU(0.5, 2) x the defaults).  One warm-up gradient, then
+    the mean of two timed ones."""
+    import torch
+    from neuwon_amd import adjoint, synth
+    T, amp = 2000, 0.5
+    cells = synth.uniform_population(num_cells=1024, n=500, seed=seed)
+    prob = adjoint.HHAdjoint(cells, dt=0.05, stim_amp_na=amp, checkpoint_every=100)
+    pop = prob.pop
+    rng = np.random.default_rng(seed)
+    gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
+    hidden = {k: gb[k] * np.exp(rng.uniform(np.log(0.5), np.log(2.0), len(gb[k]))) for k in gb}
+    f32 = lambda a: torch.as_tensor(a.astype(np.float32), device=pop.dev)
+    prob.forward(f32(hidden["na"]), f32(hidden["k"]), f32(hidden["l"]), np.ones(6),
+                 torch.zeros((pop.num_cells, T), device=pop.dev))
+    target = prob.trace.clone()
+    args = (f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), target)
+    prob.loss_and_grad(*args)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        prob.loss_and_grad(*args)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / 2
+    log(f"adjoint: {el:.3f} s per gradient")
+    return {"gradients_per_s": 1.0 / el, "seconds_per_gradient": el,
+            "config": "1024 cells x 500 compartments, 2000 steps at dt 0.05 ms, checkpoints every "
+                      "100 steps; gradients of sum (v_soma - target)^2 w.r.t. per-segment gNa, gK, "
+                      "gL and 6 HH rate multipliers"}
 
 
 def copy_probe(args):
@@ -166,12 +209,14 @@ def main():
     kern_ms = sum(a.elapsed_time(b) for _, a, b in ev)
     launches = len(ev)
     value = N * K / wall
-    b_step = algorithmic_bytes(args.format, nnz_bar)
+    b_step = algorithmic_bytes(nnz_bar)
     avg_launch_s = kern_ms / 1e3 / launches
     bytes_per_launch = b_step * N * (K / launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     stored_bytes_step = (st["stored"] * 6.0 * 2) / N  # FP32 value + u16 column, two half steps
+    f_step = format_bytes(args.format, nnz_bar)
     traffic = MEASURED_TRAFFIC_PER_CMPT_STEP[args.format]
+    units_per_s = N * (K / launches) / avg_launch_s
     out = {
         "metric": "compartment-steps/s @10M cmpts: FP32 batched vs FP64 plain; adjoint grads/s",
         "value": value,
@@ -194,11 +239,18 @@ def main():
                      "traffic": None if traffic is None else traffic * N * (K / launches),
                      "kernel": "nw::k_hh_step", "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_cmpt_step": b_step,
-                     "stored_propagator_bytes_per_cmpt_step": stored_bytes_step},
+                     "format_bytes_per_cmpt_step": f_step,
+                     "format_GBs": f_step * units_per_s / 1e9,
+                     "stored_propagator_bytes_per_cmpt_step": stored_bytes_step,
+                     "measured_traffic_GBs": None if traffic is None else traffic * units_per_s / 1e9},
     }
     log(f"timed {K} steps: {wall:.3f}s wall, kernel {kern_ms:.1f} ms, {value / 1e9:.3f} G cmpt-steps/s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pop, args)
+    if not args.no_adjoint:
+        del pop
+        torch.cuda.empty_cache()
+        out["adjoint"] = adjoint_rate()
     print(json.dumps(out), flush=True)
 
 

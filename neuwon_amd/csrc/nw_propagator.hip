@@ -461,11 +461,12 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
     NW_HIP(hipMalloc(&p->row_off, sizeof(uint32_t) * (p->num_rows + num_cells)));
     NW_HIP(hipMalloc(&p->rowsum, sizeof(double) * p->num_rows));
     if (sym) NW_HIP(hipMalloc(&p->wgt, sizeof(double) * p->num_rows));
-    // capacity guess; grown on demand
+    // capacity guess (about 540 stored entries per row at prune 1e-9, half that symmetric);
+    // grown on demand
     int64_t guess = 0;
     for (int64_t c = 0; c < num_cells; ++c) {
         int64_t n = cell_ptr[c + 1] - cell_ptr[c];
-        guess += n * std::min<int64_t>(n, 640);
+        guess += n * std::min<int64_t>(n, sym ? 320 : 640);
     }
     int rc;
     if ((rc = grow(p, std::max<int64_t>(guess, 1)))) return rc;
