@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per compartment-step of the step kernel on this workload, per propagator format, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs scaled by what the same counters read for a
 # plain 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); None where not measured
-MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6890.6, "symmetric": 3753.3}
+MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3}
 
 
 def algorithmic_bytes(nnz_bar):
