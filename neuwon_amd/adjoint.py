@@ -16,13 +16,17 @@ from . import _lib, channels, population
 
 
 class HHAdjoint:
-    """Batched cells with HH on every segment and a constant current at every root."""
+    """Batched cells with HH on every segment and a constant current at every root.
+
+    symmetric: propagator store of the forward and reverse passes (the symmetric one moves half
+    the bytes; see population.Propagator)."""
 
     def __init__(self, cells, dt=0.05, celsius=6.3, stim_amp_na=0.5, prune=1e-9,
-                 checkpoint_every=100, v0=-70.0, device=0):
+                 checkpoint_every=100, v0=-70.0, device=0, symmetric=True):
         self.pop = population.HHPopulation(cells, dt=dt, celsius=celsius, v0=v0, prune=prune,
                                            gate_mode="table", device=device,
-                                           stim={"mode": "constant", "amp_na": stim_amp_na})
+                                           stim={"mode": "constant", "amp_na": stim_amp_na},
+                                           symmetric=symmetric)
         self.torch = self.pop.torch
         self.dev = self.pop.dev
         self.K = int(checkpoint_every)

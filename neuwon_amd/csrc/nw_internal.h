@@ -145,7 +145,7 @@ __device__ __forceinline__ double row_consume(const RowChunk<U>& ch, const float
 }
 
 // Walks the stored rows first, first + stride, ... of one cell, one wavefront per row, in chunks of
-// kChunk entries, with NB - 1 chunks (2 or 3 buffers) always loading while one is consumed.  The
+// kChunk entries, with NB - 1 chunks (NB = 2..4 buffers) always loading while one is consumed.  The
 // buffers are used in turn, so the loads' destination registers stay fixed and consuming one
 // chunk waits only for its own loads; every step issues its load (past the last chunk the first
 // one is loaded again, unused), which keeps those waits exact.  The offsets of the next row to
@@ -166,7 +166,7 @@ __device__ __forceinline__ void stream_rows(const uint32_t* __restrict__ roff,
                                             const double* __restrict__ rowval, int nrows,
                                             int first, int stride, int lane, Consume consume,
                                             Finish finish) {
-    static_assert(NB == 2 || NB == 3, "two or three chunk buffers");
+    static_assert(NB >= 2 && NB <= 4, "two to four chunk buffers");
     constexpr uint32_t kChunk = kWave * kRowAlign * U;
     if (first >= nrows) return;
     // row offsets and values are read-only for the kernel's lifetime and indexed by the
@@ -193,34 +193,39 @@ __device__ __forceinline__ void stream_rows(const uint32_t* __restrict__ roff,
         const bool live = p.r < nrows;
         row_load(ch, vals, cols, live ? p.base : p0.base, live ? p.end : p0.end, lane);
     };
-    RowChunk<U> A, B, C;
-    ChunkPos d0 = p0, d1 = next(p0);
-    load(A, d0);
-    if (NB == 3) load(B, d1);
+    RowChunk<U> buf[NB];
+    ChunkPos d[NB - 1];  // d[0]: the chunk consumed next; d[1..]: chunks already loading
+    d[0] = p0;
+    load(buf[0], d[0]);
+#pragma unroll
+    for (int i = 1; i < NB - 1; ++i) {
+        d[i] = next(d[i - 1]);
+        load(buf[i], d[i]);
+    }
     double acc = 0.0;
-    // consume `cur` (chunk d0) after issuing the load of the chunk NB - 1 ahead into `ld`
+    // consume `cur` (chunk d[0]) after issuing the load of the chunk NB - 1 ahead into `ld`
     auto step = [&](RowChunk<U>& cur, RowChunk<U>& ld) -> bool {
-        ChunkPos d2 = d1;
-        if (NB == 3) {
-            d2 = next(d1);
-            load(ld, d2);
-        } else {
-            load(ld, d1);
-        }
-        acc = consume(cur, d0.r, d0.end - d0.base, acc);
-        if (d0.base + kChunk >= d0.end) {
-            finish(d0.r, acc, d0.val);
+        const ChunkPos dn = next(d[NB - 2]);
+        load(ld, dn);
+        acc = consume(cur, d[0].r, d[0].end - d[0].base, acc);
+        if (d[0].base + kChunk >= d[0].end) {
+            finish(d[0].r, acc, d[0].val);
             acc = 0.0;
         }
-        d0 = d1;
-        d1 = (NB == 3) ? d2 : next(d1);
-        return d0.r < nrows;
+#pragma unroll
+        for (int i = 0; i < NB - 2; ++i) d[i] = d[i + 1];
+        d[NB - 2] = dn;
+        return d[0].r < nrows;
     };
-    if (NB == 3) {
-        while (step(A, C) && step(B, A) && step(C, B)) {
+    if constexpr (NB == 2) {
+        while (step(buf[0], buf[1]) && step(buf[1], buf[0])) {
+        }
+    } else if constexpr (NB == 3) {
+        while (step(buf[0], buf[2]) && step(buf[1], buf[0]) && step(buf[2], buf[1])) {
         }
     } else {
-        while (step(A, B) && step(B, A)) {
+        while (step(buf[0], buf[3]) && step(buf[1], buf[0]) && step(buf[2], buf[1]) &&
+               step(buf[3], buf[2])) {
         }
     }
 }

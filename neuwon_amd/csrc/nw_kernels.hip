@@ -282,7 +282,8 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 // Reverse pass of n_steps fused steps (table gates), from a record written by k_hh_step.
 // Loss: sum over steps of (v[soma] after the step - target)^2.  Adjoints of the voltage and the
 // three gates are carried across launches; parameter gradients accumulate in FP64.
-//   P^T y = C P (C^-1 y): the transpose product is a forward product of y/C scaled by C.
+//   general store:   P^T y = C P (C^-1 y): a forward product of y/C scaled by C;
+//   symmetric store: P^T y = W S (W^-1 y) with the stored S = W P W^-1.
 // ----------------------------------------------------------------------------------------------
 struct AdjArgs {
     const int64_t* cell_row0;
@@ -314,6 +315,7 @@ struct AdjArgs {
     double* grad_l;
     double* grad_theta;    // [cell][6]
     double* loss;          // [cell]
+    const double* wgt;     // symmetric store: w of every segment
 };
 
 struct GateLookup {
@@ -368,12 +370,22 @@ __device__ __forceinline__ double half_adjoint(double lx, double v_in, double m,
     return al * lx;
 }
 
-template <int BLOCK>
+template <int BLOCK, bool SYM>
 __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
-    extern __shared__ float lds[];
-    float* ys = lds;            // adjoint / C, input of the product
-    float* zs = lds + a.nmax;   // product output
-    float* lv = zs + a.nmax;    // adjoint of the voltage
+    extern __shared__ double lds64[];
+    // general: ys = adjoint / C and the FP32 product zs32; symmetric: ys = adjoint / w and the FP64
+    // product zs (cleared by the thread that reads it); lv: adjoint of the voltage
+    float *ys, *zs32 = nullptr, *lv;
+    double* zs = nullptr;
+    if (SYM) {
+        zs = lds64;
+        ys = reinterpret_cast<float*>(lds64 + a.nmax);
+        lv = ys + a.nmax;
+    } else {
+        ys = reinterpret_cast<float*>(lds64);
+        zs32 = ys + a.nmax;
+        lv = zs32 + a.nmax;
+    }
     __shared__ double red[6][BLOCK / kWave];
     __shared__ double red_loss[BLOCK / kWave];
     const nw_hh_params& p = a.prm;
@@ -384,6 +396,8 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
     const float* vals = a.vals + a.cell_nz0[cell];
     const uint16_t* cols = a.cols + a.cell_nz0[cell];
     const double* rs = a.rowsum + row0;
+    const double* wg = SYM ? a.wgt + row0 : nullptr;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int64_t NR = a.num_rows;
     const double hs = 0.5 * (double)p.dt_ms * 1e-3;
     const double vmin = p.vmin;
@@ -392,7 +406,28 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
     const int soma = a.soma_local[cell];
     double th[6] = {0, 0, 0, 0, 0, 0};
     double loss = 0.0;
-    for (int i = threadIdx.x; i < nrows; i += BLOCK) lv[i] = a.lam_v[row0 + i];
+    auto put_y = [&](int i, double lam, double invc) {
+        ys[i] = SYM ? (float)(lam / wg[i]) : (float)(lam * invc);
+    };
+    auto take_lx = [&](int i, double invc) -> double {  // (P^T lambda)_i
+        if (SYM) {
+            const double z = zs[i];
+            zs[i] = 0.0;
+            return z * wg[i];
+        }
+        return (double)zs32[i] / invc;
+    };
+    auto product = [&]() {
+        if (SYM)
+            spmv_sym_rows(roff, vals, cols, ys, zs, nrows, __builtin_amdgcn_readfirstlane(wave),
+                          BLOCK / kWave, lane);
+        else
+            spmv<0>(roff, vals, cols, rs, ys, zs32, 0.0, nrows);
+    };
+    for (int i = threadIdx.x; i < nrows; i += BLOCK) {
+        lv[i] = a.lam_v[row0 + i];
+        if (SYM) zs[i] = 0.0;
+    }
     __syncthreads();
     for (int k = a.n_steps - 1; k >= 0; --k) {
         const int64_t gstep = p.step0 + k;
@@ -405,15 +440,15 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
             lv[soma] += (float)(2.0 * r);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < nrows; i += BLOCK) ys[i] = lv[i] * a.inv_c[row0 + i];
+        for (int i = threadIdx.x; i < nrows; i += BLOCK) put_y(i, lv[i], a.inv_c[row0 + i]);
         __syncthreads();
-        spmv<0>(roff, vals, cols, rs, ys, zs, 0.0, nrows);
+        product();
         __syncthreads();
         // half B and the mechanism
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
             const double invc = a.inv_c[gi];
-            const double lxB = (double)zs[i] / invc;
+            const double lxB = take_lx(i, invc);
             const double v1 = rec[4 * NR + gi];
             const double mN = nxt[NR + gi], hN = nxt[2 * NR + gi], nN = nxt[3 * NR + gi];
             double lm = a.lam_m[gi], lh = a.lam_h[gi], ln = a.lam_n[gi];
@@ -437,16 +472,16 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
             a.grad_na[gi] += gr_na;
             a.grad_k[gi] += gr_k;
             a.grad_l[gi] += gr_l;
-            ys[i] = (float)(lv1 * invc);
+            put_y(i, lv1, invc);
         }
         __syncthreads();
-        spmv<0>(roff, vals, cols, rs, ys, zs, 0.0, nrows);
+        product();
         __syncthreads();
         // half A with the gates at the start of the step
         for (int i = threadIdx.x; i < nrows; i += BLOCK) {
             const int64_t gi = row0 + i;
             const double invc = a.inv_c[gi];
-            const double lxA = (double)zs[i] / invc;
+            const double lxA = take_lx(i, invc);
             double lvk = lxA;
             if (gstep > 0 || p.conductances_valid) {
                 double lm = a.lam_m[gi], lh = a.lam_h[gi], ln = a.lam_n[gi];
@@ -467,7 +502,6 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
     }
     for (int i = threadIdx.x; i < nrows; i += BLOCK) a.lam_v[row0 + i] = lv[i];
     // reduce the rate-parameter gradients of the block (fixed order: deterministic)
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const double s = wave_sum(th[q]);
@@ -779,17 +813,21 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
                    grad_theta && loss,
                "nw_hh_adjoint: null argument");
     NW_REQUIRE(prm->gate_mode == 0, "the adjoint is written for the table gate mode");
-    NW_REQUIRE(!p->symmetric, "nw_hh_adjoint takes the general (non-symmetric) format");
     NW_REQUIRE(prm->stim_mode == 0 || prm->stim_mode == 1, "adjoint supports constant stimulus only");
     NW_REQUIRE(n_steps >= 0 && prm->num_nodes >= 2, "bad arguments");
     if (n_steps == 0) return NW_OK;
     AdjArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
               p->max_cell_rows, n_steps, *prm, record, p->num_rows, table6, g_na, g_k, g_l, inv_c,
               soma_local, target, target_stride, lam_v, lam_m, lam_h, lam_n, grad_na, grad_k,
-              grad_l, grad_theta, loss};
-    const size_t lds = 3 * sizeof(float) * (size_t)p->max_cell_rows;
+              grad_l, grad_theta, loss, p->wgt};
+    const bool sym = p->symmetric != 0;
+    const size_t lds = (sym ? sizeof(double) + 2 * sizeof(float) : 3 * sizeof(float)) *
+                       (size_t)p->max_cell_rows;
     NW_REQUIRE(lds <= 150 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
-    k_hh_adjoint<256><<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
+    if (sym)
+        k_hh_adjoint<256, true><<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
+    else
+        k_hh_adjoint<256, false><<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
     NW_HIP(hipGetLastError());
     return NW_OK;
 }

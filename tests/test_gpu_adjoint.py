@@ -15,11 +15,26 @@ def _say(msg):
     sys.stdout.flush()
 
 
+def _store_pruned(pop, c, half_dt):
+    """FP64 propagator of cell c pruned as the store prunes it: entry rule for the general store,
+    pair rule (either of P_ij, P_ji reaches the threshold) for the symmetric one."""
+    import scipy.sparse
+    r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
+    prune = pop.propagator.prune
+    parent = pop.parent_local[r0:r1].astype(np.int64)
+    if not pop.propagator.symmetric:
+        return O.propagator(parent, pop.R[r0:r1], pop.C[r0:r1], half_dt, prune=prune)
+    full = O.propagator(parent, pop.R[r0:r1], pop.C[r0:r1], half_dt, prune=0.0).toarray()
+    keep = np.maximum(np.abs(full), np.abs(full.T)) >= prune
+    np.fill_diagonal(keep, True)
+    return scipy.sparse.csr_matrix(np.where(keep, full, 0.0))
+
+
 def _oracle_problem(prob, c, stim_amp, dt):
+    """The FP64 oracle adjoint of one cell, fed the propagator the store holds."""
     pop = prob.pop
     r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
-    P = O.propagator(pop.parent_local[r0:r1].astype(np.int64), pop.R[r0:r1], pop.C[r0:r1],
-                     0.5 * dt, prune=pop.propagator.prune)
+    P = _store_pruned(pop, c, 0.5 * dt)
     return r0, r1, O.HHAdjointProblem(P, pop.C[r0:r1], stim_amp, dt=dt)
 
 
@@ -31,12 +46,14 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-def test_adjoint_matches_oracle(torch_gpu):
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_adjoint_matches_oracle(torch_gpu, symmetric):
     from neuwon_amd import adjoint, synth
     torch = torch_gpu
     dt, T, amp = 0.05, 300, 0.1e-9
     cells = synth.uniform_population(num_cells=4, n=90, seed=6)
-    prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100)
+    prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100,
+                             symmetric=symmetric)
     pop = prob.pop
     gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
     rng = np.random.default_rng(7)
@@ -88,7 +105,8 @@ def test_autograd_function(torch_gpu):
     assert np.allclose(params[3].grad.cpu().numpy(), gr["theta"], rtol=1e-9)
 
 
-def test_adjoint_workload_1024x500(torch_gpu):
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_adjoint_workload_1024x500(torch_gpu, symmetric):
     """1024 cells x 500 compartments, 2000 steps at dt 0.05 ms, checkpoints every 100 steps:
     gradients per second and a per-cell check against the FP64 oracle adjoint."""
     from neuwon_amd import adjoint, synth
@@ -96,7 +114,8 @@ def test_adjoint_workload_1024x500(torch_gpu):
     dt, T, amp = 0.05, 2000, 0.5e-9
     t0 = time.time()
     cells = synth.uniform_population(num_cells=1024, n=500, seed=4)
-    prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100)
+    prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100,
+                             symmetric=symmetric)
     pop = prob.pop
     _say(f"setup {time.time() - t0:.1f}s, nnz/row {pop.propagator.mean_nnz_per_row:.1f}")
     rng = np.random.default_rng(4)
@@ -111,7 +130,8 @@ def test_adjoint_workload_1024x500(torch_gpu):
     loss, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), target)
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
-    _say(f"one gradient of {pop.num_segments} compartments x {T} steps: {el:.2f}s "
+    _say(f"{'symmetric' if symmetric else 'general'} store: "
+         f"one gradient of {pop.num_segments} compartments x {T} steps: {el:.2f}s "
          f"({1.0 / el:.3f} gradients/s), loss {loss:.4e}, "
          f"target v_soma max {float(target.max()):.1f} mV")
     # check two cells against the FP64 oracle adjoint fed the same target traces
@@ -126,10 +146,11 @@ def test_adjoint_workload_1024x500(torch_gpu):
 
 def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
     """Central finite differences of the FP64 C oracle (h = 1e-6 relative) on 32 cells of
-    500 compartments, 2000 steps at dt 0.05 ms, against the GPU reverse pass."""
+    500 compartments, 2000 steps at dt 0.05 ms, against the GPU reverse pass (symmetric store,
+    the default; the oracle runs the same pair-pruned propagator)."""
     import os
     import scipy.sparse
-    from neuwon_amd import adjoint, synth, population
+    from neuwon_amd import adjoint, synth
     torch = torch_gpu
     dt, T, amp, ncell = 0.05, 2000, 0.5e-9, 32
     cells = synth.uniform_population(num_cells=1024, n=500, seed=4)[:ncell]
@@ -143,12 +164,8 @@ def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
     target = prob.trace.clone()
     _, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), target)
     tg = target.cpu().numpy().astype(np.float64)
-    blocks = []
-    for c in range(ncell):
-        r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
-        blocks.append(population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1],
-                                                     pop.C[r0:r1], 0.5 * dt, 1e-9))
-    P = scipy.sparse.block_diag(blocks, format="csr")
+    P = scipy.sparse.block_diag([_store_pruned(pop, c, 0.5 * dt) for c in range(ncell)],
+                                format="csr")
     roots = pop.cell_ptr[:-1]
     threads = len(os.sched_getaffinity(0))
 
