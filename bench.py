@@ -72,46 +72,129 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pop, args):
-    """The oracle's C FP64 restatement of the same step (CSR SpMV, table gates, Poisson stimulus),
-    fed the same pruned FP64 propagator the device built, on a bounded sample of the population's
-    cells: one thread (the reference's serial path) and every host core."""
+def _sample_propagator(pop, sample):
+    """FP64 propagator of the first `sample` cells from the device construction path, pruned as
+    the population's store prunes it (entry rule, or the pair rule of the symmetric store)."""
     import scipy.sparse
-    from oracle import neuwon_oracle as O
     from neuwon_amd import population
-    cores = len(os.sched_getaffinity(0))
-    sample = min(args.cpu_sample_cells, pop.num_cells)
-    r_end = int(pop.cell_ptr[sample])
+    prune = pop.propagator.prune
     blocks = []
     for c in range(sample):
         r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
-        blocks.append(population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1],
-                                                      pop.C[r0:r1], 0.5 * pop.dt,
-                                                      pop.propagator.prune))
-    P = scipy.sparse.block_diag(blocks, format="csr")
+        args = (pop.parent_local[r0:r1], pop.R[r0:r1], pop.C[r0:r1], 0.5 * pop.dt)
+        if pop.propagator.symmetric:
+            full = population.cell_propagator_f64(*args, 0.0).toarray()
+            keep = np.maximum(np.abs(full), np.abs(full.T)) >= prune
+            np.fill_diagonal(keep, True)
+            blocks.append(scipy.sparse.csr_matrix(np.where(keep, full, 0.0)))
+            del full, keep
+        else:
+            blocks.append(population.cell_propagator_f64(*args, prune))
+    return scipy.sparse.block_diag(blocks, format="csr")
+
+
+def cpu_baseline(pop, args, total_steps):
+    """The oracle's C FP64 restatement of the same step (CSR SpMV, table gates, Poisson stimulus),
+    fed the same pruned FP64 propagator the device built, on a bounded sample of the population's
+    cells: one thread (the reference's serial path) and every host core.  The all-cores run covers
+    exactly the steps the GPU ran, so its voltages also give the precision of the GPU run on the
+    sample."""
+    from oracle import neuwon_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    sample = min(args.cpu_sample_cells, pop.num_cells)
+    r_end = int(pop.cell_ptr[sample])
+    P = _sample_propagator(pop, sample)
     p = pop.params
     stim = ("poisson", float(p.stim_amp_a), int(p.stim_seed), int(p.poisson_threshold),
             int(p.pulse_ticks))
     gbar = {k: pop.g_bar[k][:r_end] for k in ("na", "k", "l")}
+    make = lambda: O.CHHPopulation(pop.cell_ptr[:sample + 1], P, pop.C[:r_end], gbar, dt=pop.dt,
+                                   celsius=pop.celsius, gate_mode="table", stim=stim)
     rates = {}
-    for threads in (1, cores):
-        co = O.CHHPopulation(pop.cell_ptr[:sample + 1], P, pop.C[:r_end], gbar, dt=pop.dt,
-                             celsius=pop.celsius, gate_mode="table", stim=stim)
-        co.advance(1, threads=threads)   # warm-up
-        steps, t0 = 0, time.perf_counter()
-        while True:
-            co.advance(1, threads=threads)
-            steps += 1
-            el = time.perf_counter() - t0
-            if el >= args.cpu_seconds / 2 or steps >= 2000:
-                break
-        rates[threads] = (r_end * steps / el, steps)
-        log(f"cpu baseline {threads} thread(s): {rates[threads][0] / 1e6:.2f} M cmpt-steps/s")
-    return {"value": rates[cores][0], "unit": "compartment-steps/s", "cores": cores, "kind": "port",
+    # one thread: a bounded run
+    co = make()
+    co.advance(3, threads=1)   # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        co.advance(1, threads=1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds / 2 or steps >= 2000:
+            break
+    rates[1] = (r_end * steps / el, steps)
+    # every core: exactly the GPU's steps (3 untimed warm-up steps first), recording every
+    # cell's root voltage after each step
+    co = make()
+    roots = pop.cell_ptr[:sample]
+    trace = np.empty((sample, total_steps))
+    warm = min(3, total_steps - 1)
+    for k in range(warm):
+        co.advance(1, threads=cores)
+        trace[:, k] = co.v[roots]
+    t0 = time.perf_counter()
+    for k in range(warm, total_steps):
+        co.advance(1, threads=cores)
+        trace[:, k] = co.v[roots]
+    el = time.perf_counter() - t0
+    rates[cores] = (r_end * (total_steps - warm) / el, total_steps - warm)
+    for t in (1, cores):
+        log(f"cpu baseline {t} thread(s): {rates[t][0] / 1e6:.2f} M cmpt-steps/s")
+    base = {"value": rates[cores][0], "unit": "compartment-steps/s", "cores": cores, "kind": "port",
             "value_1_thread": rates[1][0],
             "sample": f"first {sample} cells ({r_end} compartments, {P.nnz} FP64 propagator entries) "
                       f"of the same population; {rates[1][1]} steps on 1 thread, {rates[cores][1]} "
                       f"steps on {cores} threads; oracle C FP64 restatement (OpenMP over cells)"}
+    return base, co.v.copy(), trace, r_end
+
+
+def _trace_error(v, ref, dt, window_ms=2.0, threshold=20.0):
+    """Errors of a voltage trace against the FP64 one: relative over samples more than window_ms
+    from every upward threshold crossing of the reference, absolute (mV) within window_ms."""
+    up = np.nonzero((ref[1:] >= threshold) & (ref[:-1] < threshold))[0] + 1
+    t = np.arange(len(ref)) * dt
+    near = np.zeros(len(ref), dtype=bool)
+    for k in up:
+        near |= np.abs(t - t[k]) <= window_ms
+    d = np.abs(v - ref)
+    rel = d / np.abs(ref)
+    return (len(up), rel[~near].max() if (~near).any() else 0.0,
+            d[near].max() if near.any() else 0.0, np.median(rel))
+
+
+def precision(pop, cells, args, v_ref, trace_ref, r_end, total_steps):
+    """Voltage error of the GPU against the FP64 oracle on the sampled cells: (1) the full state
+    of the timed population after all its steps; (2) every cell's root voltage after every step,
+    from the same cells advanced again as a population of their own in one launch (cells are
+    independent; the rerun differs from the timed run only by the launch split, which moves the
+    per-cell reference voltage, and the order of the symmetric store's atomics: rerun_max_abs_diff
+    reports by how much), split into subthreshold samples (relative) and samples within 2 ms of a
+    spike (absolute)."""
+    import torch
+    from neuwon_amd import population
+    v_gpu = pop.v.cpu().numpy().astype(np.float64)[:r_end]
+    d = np.abs(v_gpu - v_ref)
+    sample = len(trace_ref)
+    sub = population.HHPopulation(cells[:sample], dt=pop.dt, celsius=pop.celsius,
+                                  prune=pop.propagator.prune, gate_mode="table",
+                                  stim=pop.stim, block_threads=args.block_threads,
+                                  symmetric=pop.propagator.symmetric)
+    tr = torch.zeros((sample, total_steps), dtype=torch.float32, device=sub.dev)
+    sub.advance(total_steps, trace=tr)
+    rerun = float(np.abs(sub.v.cpu().numpy().astype(np.float64) - v_gpu).max())
+    tr = tr.cpu().numpy().astype(np.float64)
+    errs = [_trace_error(tr[c], trace_ref[c], pop.dt) for c in range(sample)]
+    return {"state_max_abs_v_error_mV": float(d.max()),
+            "state_median_rel_v_error": float(np.median(d / np.abs(v_ref))),
+            "trace_spikes": int(sum(e[0] for e in errs)),
+            "trace_max_rel_v_error_subthreshold": float(max(e[1] for e in errs)),
+            "trace_max_abs_v_error_near_spikes_mV": float(max(e[2] for e in errs)),
+            "trace_median_rel_v_error": float(np.median([e[3] for e in errs])),
+            "rerun_max_abs_diff_mV": rerun,
+            "sample": f"first {sample} cells, {r_end} compartments, {total_steps} steps",
+            "definition": "FP64 oracle with the same pruned propagator, tables and stimulus; "
+                          "state: every compartment after all steps; trace: root voltage of every "
+                          "sampled cell after every step, subthreshold = more than 2 ms from a "
+                          "spike of the oracle trace"}
 
 
 def adjoint_rate(seed=4):
@@ -246,7 +329,9 @@ def main():
     }
     log(f"timed {K} steps: {wall:.3f}s wall, kernel {kern_ms:.1f} ms, {value / 1e9:.3f} G cmpt-steps/s")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pop, args)
+        out["cpu_baseline"], v_ref, trace_ref, r_end = cpu_baseline(pop, args, W + K)
+        out["precision"] = precision(pop, cells, args, v_ref, trace_ref, r_end, W + K)
+        log(f"precision: {out['precision']}")
     if not args.no_adjoint:
         del pop
         torch.cuda.empty_cache()

@@ -515,8 +515,9 @@ def _make_hh_mechanism(model, name, gate_mode, gbar=None, reversal=None):
     ss = channels.hh_steady_state(model.initial_voltage, model.temperature)
     cls_db.add_attribute("segment", NULL, dtype=np.int64, device=False,
                          on_write=model._invalidate_mechanisms)
-    cls_db.add_attribute("magnitude", 1.0, valid_range=(0.0, None), device=False,
-                         on_write=model._invalidate_mechanisms)
+    # magnitude lives on the device: a change (another mechanism's pointer, set_data) rescales the
+    # instance's maximal conductances at the next advance without rebuilding the model
+    cls_db.add_attribute("magnitude", 1.0, valid_range=(0.0, None))
     for g in "mhn":
         cls_db.add_attribute(g, ss[g])
     for ion in ("na", "k", "l"):
@@ -556,8 +557,7 @@ class LTIMechanism:
         cls_db = DBClass(db, self.name)
         cls_db.add_attribute("segment", NULL, dtype=np.int64, device=False,
                              on_write=model._invalidate_mechanisms)
-        cls_db.add_attribute("magnitude", 1.0, valid_range=(0.0, None), device=False,
-                             on_write=model._invalidate_mechanisms)
+        cls_db.add_attribute("magnitude", 1.0, valid_range=(0.0, None))
         init = self.scheme.initial_state(model.initial_voltage)
         for s, x0 in zip(self.scheme.state_names, init):
             cls_db.add_attribute(s, float(x0))
@@ -668,11 +668,23 @@ class _Segment(DBObject):
             mechanisms = {name: 1.0 for name in mechanisms}
         else:
             raise ValueError(f'Expected dictionary, not "{type(mechanisms)}"')
-        all_mech = type(self)._model.mechanisms
-        out = {}
-        for name, magnitude in mechanisms.items():
-            out[name] = all_mech[name](self, outside, float(magnitude))
-        return out
+        model = type(self)._model
+        all_mech = model.mechanisms
+        instances = {}
+
+        def insert_recursive(name):
+            # a mechanism's other_mechanisms() are created first and handed to its constructor;
+            # each name is instantiated once per call (segment.py:141-160)
+            if name in instances:
+                return instances[name]
+            others = [insert_recursive(x) for x in model._local_dependencies.get(name, ())]
+            inst = all_mech[name](self, outside, float(mechanisms[name]), *others)
+            instances[name] = inst
+            return inst
+
+        for name in mechanisms:
+            insert_recursive(name)
+        return instances
 
     def inject_current(self, current=1.0, duration=1.0):
         """electric.py:119-125: current (nA) for duration (ms), from the next input tick."""
@@ -826,6 +838,7 @@ class RxD_Model:
             self.species[name] = dict(kw)
         # mechanisms (mechanisms.py:56-99)
         self.mechanisms = {}
+        self._local_dependencies = {}
         for m in mechanisms:
             self._load_mechanism(m)
 
@@ -878,6 +891,10 @@ class RxD_Model:
             mech = retval if retval is not None else m
         name = str(mech.get_name())
         assert name not in self.mechanisms
+        if not getattr(mech, "omnipresent", False):   # mechanisms.py:84-96
+            deps = mech.other_mechanisms()
+            self._local_dependencies[name] = ((deps,) if isinstance(deps, str)
+                                              else tuple(str(x) for x in deps))
         self.mechanisms[name] = mech
 
     # ------------------------------------------------------------------ structure changes
@@ -976,7 +993,6 @@ class RxD_Model:
             for col in mc.columns.values():
                 col.to_device(morder, dev)
             mech._seg_dev = torch.as_tensor(rows[morder].astype(np.int32), device=dev)
-            mech._mag_sorted = np.asarray(mc.columns["magnitude"].host, dtype=np.float64)[morder]
             k = len(mech.conductance_sources)
             mech._g_offset = gbuf_len
             for j, src in enumerate(mech.conductance_sources):
@@ -1021,13 +1037,26 @@ class RxD_Model:
     def _dev_or(self, torch):
         return getattr(self, "_dev", None) or torch.device("cuda", self.device)
 
+    def _scaled_by_magnitude(self, mech):
+        """Per-instance maximal conductances (magnitude x density), recomputed when the device
+        magnitude column has been written since the last call."""
+        col = mech._db_class.columns["magnitude"]
+        col._flush()
+        version = col.dev._version
+        if getattr(mech, "_mag_version", None) != version:
+            mag = col.dev
+            if hasattr(mech, "gate_mode"):
+                mech._gbar_dev = {ion: (mag * (mech.gbar[ion] * channels.CM2_TO_UM2)).contiguous()
+                                  for ion in ("na", "k", "l")}
+            else:
+                mech._gmax_dev = (mag * mech.spec.gmax).contiguous()
+            mech._mag_version = version
+
     def _prepare_mechanism(self, mech, dev):
         torch = _torch()
         mc = mech._db_class
-        mag = torch.as_tensor(mech._mag_sorted.astype(np.float32), device=dev)
+        mech._mag_version = None
         if hasattr(mech, "gate_mode"):
-            mech._gbar_dev = {ion: (mag * (mech.gbar[ion] * channels.CM2_TO_UM2)).contiguous()
-                              for ion in ("na", "k", "l")}
             grid = channels.LinearInput(-100.0, 100.0, 2000)
             mech._grid = grid
             mech._table = torch.as_tensor(
@@ -1038,7 +1067,6 @@ class RxD_Model:
             spec.scheme.time_step = self.time_step
             mech._node_table = torch.as_tensor(spec.scheme.node_table().astype(np.float32), device=dev)
             mech._weights = torch.as_tensor(np.asarray(spec.scheme.weights, dtype=np.float32), device=dev)
-            mech._gmax_dev = (mag * spec.gmax).contiguous()
             S = spec.scheme.num_states
             mech._states = torch.stack([mc.columns[s].dev for s in spec.scheme.state_names]).contiguous()
             for k, s in enumerate(spec.scheme.state_names):
@@ -1056,6 +1084,7 @@ class RxD_Model:
         M = mc.size
         if M == 0:
             return
+        self._scaled_by_magnitude(mech)
         P = _lib.ptr
         out = self._gbuf[mech._g_offset:mech._g_offset + 3 * M]
         g = mech._grid
@@ -1073,6 +1102,7 @@ class RxD_Model:
         if M == 0:
             return
         spec = mech.spec
+        self._scaled_by_magnitude(mech)
         P = _lib.ptr
         out = self._gbuf[mech._g_offset:mech._g_offset + M]
         grid = spec.scheme.grid

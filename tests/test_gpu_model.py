@@ -190,3 +190,97 @@ def test_lti_mechanism_api(torch_gpu):
     assert np.abs(states.sum(axis=0) - 1).max() < 1e-5
     v = m.database.get_data("Segment.voltage")
     assert np.all(np.isfinite(v)) and v.max() < 60
+
+
+def test_interacting_mechanisms(torch_gpu):
+    """test_segment.py:69-113: no spontaneous AP at start-up; an AP after 0.1 nA x 0.5 ms; a second
+    mechanism holding a pointer to the hh instance (local.mod: hh' = -hh / 100, solved exactly
+    per step) decays its magnitude until the same stimulus no longer fires.  The plugin gets the hh
+    instance through other_mechanisms() (mechanisms.py:33-40, segment.py:141-160)."""
+    import math
+    from neuwon_amd.model import RxD_Model, Mechanism, TimeSeries
+
+    class Local(Mechanism):
+        instances = []
+
+        @classmethod
+        def initialize(cls, model):
+            cls.model = model
+            cls.instances = []
+
+        @classmethod
+        def get_name(cls):
+            return "local"
+
+        @classmethod
+        def other_mechanisms(cls):
+            return ["hh"]
+
+        def __init__(self, segment, outside, magnitude, hh):
+            self.hh = hh
+            type(self).instances.append(self)
+
+        @classmethod
+        def advance(cls):
+            decay = math.exp(-cls.model.get_time_step() / 100.0)
+            for inst in cls.instances:
+                inst.hh.magnitude = inst.hh.magnitude * decay
+
+    m = RxD_Model(time_step=0.1, temperature=6.3, mechanisms=["hh", Local],
+                  species=[{"name": "na", "reversal_potential": +60},
+                           {"name": "k", "reversal_potential": -88},
+                           {"name": "l", "reversal_potential": -54.3}])
+    root = m.Neuron([0, 0, 0], 10).root
+    inst = root.insert({"hh": 1, "local": 1})
+    assert inst["local"].hh is inst["hh"]
+    mag0 = inst["hh"].magnitude
+    probe = TimeSeries().record(root, "voltage", discard_after=30)
+    while m.clock() < 20:
+        m.advance()
+    quiet = max(probe.get_data())
+    m.clock.reset()
+    root.inject_current(.1, .5)
+    while m.clock() < 20:
+        m.advance()
+    fired = max(probe.get_data())
+    while m.clock() < 500:
+        m.advance()
+    decayed = inst["hh"].magnitude / mag0
+    m.clock.reset()
+    root.inject_current(.1, .5)
+    while m.clock() < 20:
+        m.advance()
+    after = max(probe.get_data())
+    _say(f"interacting: quiet max {quiet:.1f} mV, stimulated max {fired:.1f} mV, magnitude "
+         f"x{decayed:.4f}, stimulated after decay max {after:.1f} mV")
+    assert quiet < -20
+    assert fired > 20
+    assert decayed == pytest.approx(math.exp(-5.2), rel=1e-3)   # 5,200 steps of 0.1 ms
+    assert after < -20
+    m.check()
+
+
+def test_advance_from_worker_thread(torch_gpu):
+    """The reference's GUI advances the model from a worker thread and reads arrays afterwards
+    (gui/model_runner/model_thread.py:100-123): the same calls from a thread give the same
+    voltages as on the main thread."""
+    import threading
+    from neuwon_amd.model import RxD_Model
+
+    def run(steps):
+        m = RxD_Model(time_step=0.1, temperature=6.3, mechanisms=["hh"])
+        root = m.Neuron([0, 0, 0], 10).root
+        for s in [root] + root.add_section([150, 0, 0], 2, maximum_segment_length=15):
+            s.insert("hh")
+        root.inject_current(.2, 2)
+        for _ in range(steps):
+            m.advance()
+        return m.database.get_data("Segment.voltage").copy()
+
+    main = run(150)
+    out = {}
+    t = threading.Thread(target=lambda: out.setdefault("v", run(150)))
+    t.start()
+    t.join(timeout=120)
+    assert not t.is_alive()
+    assert np.array_equal(out["v"], main)
