@@ -142,6 +142,8 @@ class HHPopulation:
                  membrane_capacitance=1.0, gbar=None, reversal=None, prune=1e-9, gate_mode="table",
                  stim=None, ap_threshold=20.0, device=0, block_threads=0, insert_area="final",
                  symmetric=False):
+        if len(cells) == 0:
+            raise ValueError("HHPopulation needs at least one cell")
         torch = _torch()
         self.torch = torch
         self.dev = torch.device("cuda", device)

@@ -183,6 +183,27 @@ def test_propagator_matches_scipy_sparse_expm():
     assert np.abs(Pd @ Pd - P2).max() < 1e-12
 
 
+def test_propagator_symmetric_under_capacitance_scaling():
+    """A = C^-1 L with L symmetric, so S = C^1/2 P C^-1/2 = expm(C^-1/2 L C^-1/2 dt) is symmetric:
+    the identity the symmetric propagator store rests on.  A generator with the coupling
+    transposed (A^T = L C^-1) does not have it."""
+    parent, coords, diam = _random_tree(70, 3)
+    g = O.geometry(parent, coords, diam)
+    R, C = O.passive(g["length"], g["cross_sectional_area"], g["area_at_creation"])
+    assert C.max() / C.min() > 2                       # non-uniform, so the scaling matters
+    w = np.sqrt(C / C[0])
+    P = O.propagator(parent, R, C, 0.05, prune=0.0).toarray()
+    S = w[:, None] * P / w[None, :]
+    assert np.abs(S - S.T).max() < 1e-12 * np.abs(S).max()
+    A = O.coefficient_matrix(parent, R, C, 0.05).toarray()
+    St = w[:, None] * scipy.linalg.expm(A.T) / w[None, :]
+    assert np.abs(St - St.T).max() > 1e-3 * np.abs(St).max()
+    # the transpose product the adjoint uses: P^T y = W S W^-1 y = C P C^-1 y
+    y = np.random.default_rng(0).normal(size=len(C))
+    assert np.allclose(P.T @ y, w * (S @ (y / w)), rtol=1e-12, atol=1e-14)
+    assert np.allclose(P.T @ y, C * (P @ (y / C)), rtol=1e-12, atol=1e-14)
+
+
 def test_propagator_pruning_is_reference_rule():
     parent, coords, diam = _random_tree(80, 2)
     g = O.geometry(parent, coords, diam)
@@ -449,6 +470,29 @@ def test_oracle_adjoint_matches_finite_differences():
             tm = {g: (1 - h, 1.0) if j == 0 else (1.0, 1 - h)}
             fd = (prob.forward(gb, tp, T, target)[1] - prob.forward(gb, tm, T, target)[1]) / (2 * h)
             assert grad["theta"][g][j] == pytest.approx(fd, rel=1e-4)
+
+
+def test_theta_table_derivatives_are_expm_frechet():
+    """The derivative columns of the rate-multiplier tables against scipy.linalg.expm_frechet of
+    the 2-state generator Q = [[-alpha, beta], [alpha, -beta]] (state order closed, open):
+    a = P_oc, b = P_oo - P_oc, and d/d(t_alpha) carries the factor alpha_0."""
+    dt = 0.05
+    tbl = O.hh_theta_tables(dt, 6.3, {"m": (1.1, 0.9), "h": (1.0, 1.2), "n": (0.8, 1.05)})
+    v = -100.0 + np.arange(2001) * 0.1
+    rates = O.hh_alpha_beta(v, 6.3)
+    th = {"m": (1.1, 0.9), "h": (1.0, 1.2), "n": (0.8, 1.05)}
+    dQa = np.array([[-1.0, 0.0], [1.0, 0.0]])
+    dQb = np.array([[0.0, 1.0], [0.0, -1.0]])
+    for g in "mhn":
+        a0, b0 = rates[g]
+        for k in range(0, 2001, 97):
+            al, be = th[g][0] * a0[k], th[g][1] * b0[k]
+            Q = np.array([[-al, be], [al, -be]]) * dt
+            P, La = scipy.linalg.expm_frechet(Q, dQa * dt)
+            _, Lb = scipy.linalg.expm_frechet(Q, dQb * dt)
+            want = [P[1, 0], P[1, 1] - P[1, 0], a0[k] * La[1, 0], a0[k] * (La[1, 1] - La[1, 0]),
+                    b0[k] * Lb[1, 0], b0[k] * (Lb[1, 1] - Lb[1, 0])]
+            np.testing.assert_allclose(tbl[g][k], want, rtol=1e-9, atol=1e-15)
 
 
 def test_theta_tables_match_oracle():
