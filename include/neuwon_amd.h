@@ -68,7 +68,8 @@ int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_h
 /* flags = NW_PROPAGATOR_SYMMETRIC stores S = W P W^-1 (w = sqrt(C), symmetric because
  * A = C^-1 L with L symmetric, electric.py:92-108) on and above the diagonal only, halving
  * the stored entries; a pair (i, j) is kept when max(P_ij, P_ji) >= prune.
- * P x = W^-1 S W x.  Used by nw_hh_step and nw_hh_adjoint; the other entry points refuse it. */
+ * P x = W^-1 S W x.  Read by nw_hh_step, nw_hh_adjoint and nw_electric_half_step; nw_ca_step
+ * refuses it. */
 int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_ptr_host,
                            const int32_t* parent_local_host, const double* R_host,
                            const double* C_host, double time_step_ms, double prune, int32_t flags,

@@ -552,6 +552,7 @@ struct HalfArgs {
     const float* sum_i;
     const float* inv_c;
     float* integral_v;
+    const double* wgt;  // symmetric store: w of every segment
 };
 
 // FP32 storage, FP64 arithmetic: x is formed in FP64 and staged in LDS as an FP32 offset from the
@@ -571,19 +572,32 @@ __device__ __forceinline__ double half_x(const HalfArgs& a, int64_t gi, double d
     return d - diff * alpha + dv;
 }
 
+// SYM = false: x - vref and the product in LDS as FP32 (spmv_rows).  SYM = true (symmetric store):
+// y = W (x - vref) in LDS as FP32, z = S y as FP64 (spmv_sym_rows), v = vref * rowsum + z / w.
+template <bool SYM>
 __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
-    extern __shared__ float xs[];
+    extern __shared__ double lds64[];
     __shared__ double s_vref;
+    float* xs = SYM ? reinterpret_cast<float*>(lds64 + a.nmax) : reinterpret_cast<float*>(lds64);
+    double* zs = SYM ? lds64 : nullptr;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int n = (int)(a.cell_row0[cell + 1] - row0);
     const double dt_ms = a.half_dt_ms;
     const double dt_s = dt_ms * 1e-3;
+    const double* wg = SYM ? a.wgt + row0 : nullptr;
     if (threadIdx.x == 0) s_vref = half_x(a, row0, dt_ms, dt_s, false);
     __syncthreads();
     const double vref = s_vref;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        xs[i] = (float)(half_x(a, row0 + i, dt_ms, dt_s, true) - vref);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double x = half_x(a, row0 + i, dt_ms, dt_s, true) - vref;
+        if (SYM) {
+            xs[i] = (float)(x * wg[i]);
+            zs[i] = 0.0;
+        } else {
+            xs[i] = (float)x;
+        }
+    }
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -591,10 +605,17 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
     const uint32_t* roff = a.row_off + row0 + cell;
     const float* vals = a.vals + a.cell_nz0[cell];
     const uint16_t* cols = a.cols + a.cell_nz0[cell];
+    const double* rs = a.rowsum + row0;
     float* vout = xs + a.nmax;
-    spmv_rows(roff, vals, cols, a.rowsum + row0, xs, vout, vref, n, wave, nwaves, lane);
+    if (SYM)
+        spmv_sym_rows(roff, vals, cols, xs, zs, n, wave, nwaves, lane);
+    else
+        spmv_rows(roff, vals, cols, rs, xs, vout, vref, n, wave, nwaves, lane);
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) a.v[row0 + i] = (float)(vref + (double)vout[i]);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v = SYM ? fma(vref, rs[i], zs[i] / wg[i]) : vref + (double)vout[i];
+        a.v[row0 + i] = (float)v;
+    }
 }
 
 }  // namespace nw
@@ -745,12 +766,17 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
                           const float* drive_v, const float* sum_i, const float* inv_c,
                           float* integral_v, void* stream) {
     NW_REQUIRE(p && v && sum_g && drive_v && sum_i && inv_c, "nw_electric_half_step: null argument");
-    NW_REQUIRE(!p->symmetric, "nw_electric_half_step takes the general (non-symmetric) format");
     HalfArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
-               p->max_cell_rows, half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v};
-    size_t lds = 2 * sizeof(float) * (size_t)p->max_cell_rows;
-    NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
-    k_half_step<<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
+               p->max_cell_rows, half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v, p->wgt};
+    const bool sym = p->symmetric != 0;
+    // general: x and the product as FP32; symmetric: y as FP32 and z as FP64
+    size_t lds = (sym ? sizeof(double) + sizeof(float) : 2 * sizeof(float)) *
+                 (size_t)p->max_cell_rows;
+    NW_REQUIRE(lds <= 160 * 1024 - 64, "cell of %d segments exceeds LDS", p->max_cell_rows);
+    if (sym)
+        k_half_step<true><<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
+    else
+        k_half_step<false><<<(unsigned)p->num_cells, 256, lds, (hipStream_t)stream>>>(a);
     NW_HIP(hipGetLastError());
     return NW_OK;
 }

@@ -784,13 +784,16 @@ class RxD_Model:
     def __init__(self, time_step=0.1, *, temperature=37, initial_voltage=-70,
                  cytoplasmic_resistance=100, membrane_capacitance=1, extracellular_tortuosity=1.55,
                  extracellular_max_distance=20e-6, species=(), mechanisms=(), device=0,
-                 prune=EPSILON):
+                 prune=EPSILON, symmetric_propagator=False):
         self.time_step = float(time_step)
         self.temperature = float(temperature)
         self.initial_voltage = float(initial_voltage)
         self.cytoplasmic_resistance = float(cytoplasmic_resistance)
         self.membrane_capacitance = float(membrane_capacitance)
         self.prune = float(prune)
+        # symmetric_propagator: store sqrt(C) P sqrt(C)^-1, one triangle (half the bytes per half
+        # step; pairs pruned together; sums in atomic order, so not bitwise reproducible)
+        self.symmetric_propagator = bool(symmetric_propagator)
         self.device = int(device)
         self.database = db = Database()
         db._model = self
@@ -1118,7 +1121,7 @@ class RxD_Model:
         prop = self._propagators.get(key)
         if prop is None:
             prop = Propagator(self._cell_ptr, self._parent_local, self._R, self._C, time_step,
-                              self.prune, self.device)
+                              self.prune, self.device, symmetric=self.symmetric_propagator)
             self._propagators = {key: prop}
         return prop
 

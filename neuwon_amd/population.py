@@ -25,8 +25,8 @@ class Propagator:
     """Owner of an ``nw_propagator`` handle (per-cell pruned CSR, FP32 values).
 
     symmetric=True stores S = W P W^-1 (w = sqrt(C)) on and above the diagonal only; a pair (i, j)
-    is then kept when max(P_ij, P_ji) >= prune.  The fused HH step and its adjoint read that
-    format; the generic half step and the species step need the general one.
+    is then kept when max(P_ij, P_ji) >= prune.  The fused HH step, its adjoint and the generic
+    half step read that format; the species step needs the general one.
     """
 
     def __init__(self, cell_ptr, parent_local, R, C, half_dt_ms, prune, device=0, symmetric=False):

@@ -18,10 +18,11 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def test_time_constant(torch_gpu):
-    """test_electric.py:28-46 through Segment._advance_electric."""
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_time_constant(torch_gpu, symmetric):
+    """test_electric.py:28-46 through Segment._advance_electric (both propagator stores)."""
     from neuwon_amd.model import RxD_Model
-    m = RxD_Model()
+    m = RxD_Model(symmetric_propagator=symmetric)
     root = m.Neuron([0, 0, 0], 20).root
     m.advance()   # builds the model on the device
     root.voltage = 0
@@ -38,10 +39,11 @@ def test_time_constant(torch_gpu):
     m.check()
 
 
-def test_inject_current(torch_gpu):
-    """test_electric.py:132-150."""
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_inject_current(torch_gpu, symmetric):
+    """test_electric.py:132-150 (both propagator stores)."""
     from neuwon_amd.model import RxD_Model
-    m = RxD_Model(time_step=.01)
+    m = RxD_Model(time_step=.01, symmetric_propagator=symmetric)
     root = m.Neuron([0, 0, 0], 10).root
     init_v = root.voltage
     while m.clock() < 3:
@@ -58,11 +60,12 @@ def test_inject_current(torch_gpu):
     assert init_Q + delta_Q == pytest.approx(new_Q, rel=1e-5)
 
 
-def test_length_constant(torch_gpu):
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_length_constant(torch_gpu, symmetric):
     """test_electric.py:48-94 (case 1): attenuation 1/e at one length constant."""
     from neuwon_amd.model import RxD_Model
     diam, rm, max_len, dt, ptol = 1, 4e11, 2, .01, .05
-    m = RxD_Model()
+    m = RxD_Model(symmetric_propagator=symmetric)
     root = m.Neuron([0, 0, 0], diam).root
     section = [root] + root.add_section([3000, 0, 0], diam, max_len)
     m.advance()
@@ -84,9 +87,10 @@ def test_length_constant(torch_gpu):
     assert ratio == pytest.approx(1 / np.e, ptol)
 
 
-def _golden_model(mechanism):
+def _golden_model(mechanism, symmetric=False):
     from neuwon_amd.model import RxD_Model, TimeSeries
-    m = RxD_Model(time_step=.1, temperature=6.3, mechanisms=[mechanism])
+    m = RxD_Model(time_step=.1, temperature=6.3, mechanisms=[mechanism],
+                  symmetric_propagator=symmetric)
     hh = m.mechanisms[mechanism]
     root = tip = m.Neuron([-1, 0, 7], 5.7).root
     hh(root)
@@ -105,11 +109,12 @@ def _golden_model(mechanism):
     return np.array(x.get_data())
 
 
-@pytest.mark.parametrize("mechanism", ["hh", "hh_kinetic"])
-def test_model_hh_golden(torch_gpu, mechanism):
+@pytest.mark.parametrize("mechanism,symmetric", [("hh", False), ("hh_kinetic", False),
+                                                ("hh", True)])
+def test_model_hh_golden(torch_gpu, mechanism, symmetric):
     """test_model.py:35-128 through the drop-in API, against the golden trace and the oracle."""
     golden = np.loadtxt(os.path.join(GOLDEN, "hh_trace_test_model.txt"))
-    v = _golden_model(mechanism)
+    v = _golden_model(mechanism, symmetric)
     assert len(v) == 201
     err = np.abs(v - golden)
     _say(f"{mechanism}: max |v - golden| = {err.max():.3f} mV")

@@ -1,6 +1,6 @@
 """Symmetric half storage of the electric propagator (S = W P W^-1, w = sqrt(C), upper triangle):
 the stored triangle against the FP64 propagator, the fused step in that format against the FP64
-oracle and against the general format, and the entry points that refuse the format."""
+oracle and against the general format, and the species step, which refuses the format."""
 import sys
 
 import numpy as np
@@ -193,16 +193,13 @@ def test_symmetric_population_matches_general(torch_gpu, block_threads):
     assert np.array_equal(out[False][1], out[True][1])
 
 
-def test_symmetric_refused_by_other_kernels(torch_gpu):
+def test_symmetric_refused_by_species_step(torch_gpu):
     import ctypes
     from neuwon_amd import _lib, population, synth
     torch = torch_gpu
     pop = population.HHPopulation(synth.single_cell(200, seed=2), symmetric=True)
     lib = _lib.load()
     z = torch.zeros(pop.num_segments, device=pop.dev)
-    rc = lib.nw_electric_half_step(pop.propagator.handle, 0.05, _lib.ptr(z), _lib.ptr(z),
-                                   _lib.ptr(z), _lib.ptr(z), _lib.ptr(z), None, None)
-    assert rc != 0 and b"symmetric" in lib.nw_last_error()
     cp = _lib.NwCaParams()
     cp.dt_ms = 0.1
     rc = lib.nw_ca_step(pop.propagator.handle, ctypes.byref(cp), 1, _lib.ptr(z), _lib.ptr(z),
