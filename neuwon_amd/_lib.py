@@ -103,9 +103,16 @@ def check(rc):
 
 
 def ptr(t):
-    """Device (or host) address of a torch tensor / numpy array as a ctypes void pointer."""
+    """Device (or host) address of a torch tensor / numpy array as a ctypes void pointer.  The
+    entry points take dense arrays, so a strided view (a transpose, a column slice) is refused
+    rather than read in the wrong order."""
     if t is None:
         return None
     if hasattr(t, "data_ptr"):
+        if not t.is_contiguous():
+            raise ValueError(f"tensor of shape {tuple(t.shape)} and strides {t.stride()} is not "
+                             "contiguous")
         return ctypes.c_void_p(t.data_ptr())
+    if not t.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"array of shape {t.shape} is not C-contiguous")
     return ctypes.c_void_p(t.ctypes.data)

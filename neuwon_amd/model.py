@@ -884,9 +884,18 @@ class RxD_Model:
     def _load_mechanism(self, m):
         if isinstance(m, str):
             key = os.path.splitext(os.path.basename(m))[0]
-            if key not in _BUILTIN:
-                raise ValueError(f"unsupported mechanism {m!r} (built-ins: {sorted(_BUILTIN)})")
-            mech = _BUILTIN[key](self)
+            if key in _BUILTIN:
+                mech = _BUILTIN[key](self)
+            elif m.endswith(".mod") and os.path.exists(m):
+                # a kinetic scheme file (nmodl_library/kinetic/*.mod): parsed, tabulated on the
+                # voltage grid, advanced by nw_lti_advance (mechanisms.py:63-66)
+                from . import nmodl
+                rev = {name: kw["reversal_potential"] for name, kw in self.species.items()
+                       if kw.get("reversal_potential") is not None}
+                mech = nmodl.kinetic_mechanism(m, self.time_step, self.temperature, rev).initialize(self)
+            else:
+                raise ValueError(f"unsupported mechanism {m!r} (built-ins: {sorted(_BUILTIN)}; "
+                                 "or the path of a kinetic .mod file)")
         elif isinstance(m, LTIMechanism):
             mech = m.initialize(self)
         else:

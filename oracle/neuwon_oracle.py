@@ -465,6 +465,50 @@ class HHSim:
         self.steps += 1
 
 
+class KineticSim:
+    """Cells whose segments carry linear kinetic mechanisms, stepped like HHSim (the staggered
+    step of rxd_model.py:63-123): every mechanism is advanced by the LTI table kernel
+    (codegen.py:39-106) at v(t + dt/2) and contributes g = gmax * sum_s weights[s] state_s with
+    its reversal potential (species.py:155-160, rxd_model.py:105-109).
+
+    mechanisms: list of dicts with poly (polynomial table of node_table_to_polynomial),
+    weights [S], gmax [n] (Siemens per unit of sum_s weights[s] state_s), E (mV), states [S, n],
+    conserve (CONSERVE total or None).  stim as in HHSim."""
+
+    def __init__(self, P, C, mechanisms, v0=-70.0, dt=0.1, stim=None):
+        self.P, self.C = P, np.asarray(C, dtype=np.float64)
+        self.n = len(self.C)
+        self.mech = [dict(m, states=np.array(m["states"], dtype=np.float64)) for m in mechanisms]
+        self.dt, self.half = float(dt), 0.5 * float(dt)
+        self.v = np.full(self.n, float(v0))
+        self.sum_g = np.zeros(self.n)     # the first half step sees no conductance
+        self.d_num = np.zeros(self.n)
+        self.stim = stim
+        self.input_ticks = 0
+        self.steps = 0
+
+    def _half(self):
+        self.input_ticks += 1
+        I = np.zeros(self.n) if self.stim is None else np.asarray(self.stim(self.input_ticks))
+        d = driving_voltage(self.sum_g, self.d_num)
+        self.v, _ = half_step(self.v, self.sum_g, d, I, self.C, self.P, self.half)
+
+    def _mechanisms(self):
+        self.sum_g = np.zeros(self.n)
+        self.d_num = np.zeros(self.n)
+        for m in self.mech:
+            m["states"] = lti_advance_vec(m["states"], self.v, m["poly"], conserve_sum=m["conserve"])
+            g = m["gmax"] * (np.asarray(m["weights"]) @ m["states"])
+            self.sum_g += g
+            self.d_num += g * m["E"]
+
+    def advance(self):
+        self._half()
+        self._mechanisms()
+        self._half()
+        self.steps += 1
+
+
 # ---------------------------------------------------------------------------------------------
 # C restatement (oracle/neuwon_oracle_c.c) -- same step, FP64, OpenMP over cells
 # ---------------------------------------------------------------------------------------------

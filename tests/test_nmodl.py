@@ -1,0 +1,204 @@
+"""The NMODL kinetic front end on the CPU: the reference's own known answers for its test files
+(neuwon/rxd/lti_sim/tests/test_lti_sim.py:9-64), every file of the kinetic library parsed, and the
+parsed rate matrices against the same rates computed by hand from the files' formulas."""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from neuwon_amd import channels, nmodl
+
+REF = "/root/reference"
+LTI_TESTS = os.path.join(REF, "neuwon/rxd/lti_sim/tests")
+LIBRARY = os.path.join(REF, "nmodl_library/kinetic")
+OWN = os.path.join(os.path.dirname(__file__), "golden", "kinetic")
+
+needs_reference = pytest.mark.skipif(not os.path.isdir(LTI_TESTS),
+                                     reason="the reference's .mod files are not on this machine")
+
+
+def _rate_matrix(km, *inputs):
+    S = km.num_states
+    return np.array([km.derivative(*inputs, *[float(i == j) for i in range(S)])
+                     for j in range(S)]).T
+
+
+def _steady_state(km, *inputs):
+    """lti_model.py:67-78: expm(A * 1 hour), columns normalised, applied to the uniform state."""
+    P = scipy.linalg.expm(_rate_matrix(km, *inputs) * 3600e3)
+    P /= P.sum(axis=0, keepdims=True)
+    return dict(zip(km.states, P.dot(np.full(km.num_states, km.conserve_sum / km.num_states))))
+
+
+@needs_reference
+def test_reference_nav11_known_answers():
+    """test_lti_sim.py:9-25 (v over [-200, 200], initial value the minimum)."""
+    km = nmodl.KineticModel(os.path.join(LTI_TESTS, "Nav11.mod"), celsius=37.0)
+    assert km.name == "na11a"
+    assert km.inputs == ["v"] and km.num_inputs == 1
+    assert km.num_states == 6 and km.states == sorted(km.states)
+    assert km.conserve_sum == 1.0
+    assert km.parameters["C1C2b2"] == 18
+    init = _steady_state(km, -200.0)
+    assert init.pop("C1") == pytest.approx(1)
+    for value in init.values():
+        assert value == pytest.approx(0, abs=1e-6)
+
+
+@needs_reference
+def test_reference_ampa13_known_answers():
+    """test_lti_sim.py:27-42 (the concentration input C, initial value 0)."""
+    km = nmodl.KineticModel(os.path.join(LTI_TESTS, "ampa13.mod"), celsius=37.0, inputs=["C"])
+    assert km.name == "AMPA13"
+    assert km.num_states == 13 and km.states == sorted(km.states)
+    assert km.conserve_sum == 1.0
+    assert km.parameters["Rb1"] == 800
+    init = _steady_state(km, 0.0)
+    assert init.pop("C0") == pytest.approx(1)
+    for value in init.values():
+        assert value == pytest.approx(0, abs=1e-6)
+
+
+@needs_reference
+def test_reference_nmda_inputs():
+    """test_lti_sim.py:44-60: two inputs, sorted (C, v)."""
+    km = nmodl.KineticModel(os.path.join(LTI_TESTS, "NMDA.mod"), celsius=37.0)
+    assert km.name == "NMDA_Mg"
+    assert km.inputs == ["C", "v"]
+    assert km.num_states == 10 and km.conserve_sum == 1.0
+    assert km.parameters["Rb"] == 10e-3
+    init = _steady_state(km, 0.0, -200.0)
+    assert init.pop("UMg") == pytest.approx(1)
+    for value in init.values():
+        assert value == pytest.approx(0, abs=1e-6)
+    with pytest.raises(nmodl.NmodlError):        # the device kernel takes one voltage input
+        nmodl.load_kinetic(os.path.join(LTI_TESTS, "NMDA.mod"))
+
+
+@needs_reference
+def test_reference_nonlinear_detected():
+    """test_lti_sim.py:62-65."""
+    with pytest.raises(nmodl.NmodlError, match="non-linear"):
+        nmodl.KineticModel(os.path.join(LTI_TESTS, "nonlinear.mod"))
+
+
+@needs_reference
+def test_library_parses():
+    """Every file of nmodl_library/kinetic: a linear scheme, CONSERVE 1, rate matrices with zero
+    column sums and non-negative off-diagonal rates at -80, 0 and +40 mV."""
+    files = sorted(f for f in os.listdir(LIBRARY) if f.endswith(".mod"))
+    assert len(files) == 34
+    for f in files:
+        km = nmodl.KineticModel(os.path.join(LIBRARY, f), celsius=37.0)
+        assert km.conserve_sum == 1.0, f
+        if km.inputs != ["v"]:
+            assert km.point_process, f            # AMPA, NMDA: transmitter-gated synapses
+            continue
+        for v in (-80.0, 0.0, 40.0):
+            A = _rate_matrix(km, v)
+            assert np.abs(A.sum(axis=0)).max() < 1e-12 * max(1.0, np.abs(A).max()), f
+            off = A - np.diag(np.diag(A))
+            assert off.min() >= 0.0, f
+
+
+@needs_reference
+def test_nav11_rates_by_hand():
+    """The Nav1.1 generator at 37 degC restated from the file's formulas (rates2 and the six
+    reactions, nmodl_library/kinetic/Nav11.mod) against the parsed one."""
+    km = nmodl.KineticModel(os.path.join(LIBRARY, "Nav11.mod"), celsius=37.0)
+    q10 = 3.0 ** ((37.0 - 20.0) / 10.0)
+
+    def r2(v, b, vv, k):
+        return b / (1.0 + math.exp((v - vv) / k))
+    for v in (-95.0, -40.0, 12.5):
+        k = {
+            ("C1", "C2"): q10 * r2(v, 18, -7, -10),
+            ("C2", "C1"): q10 * (r2(v, 3, -37, 10) + r2(v, 18, -7, -10)),
+            ("C2", "O1"): q10 * r2(v, 18, -7, -10),
+            ("O1", "C2"): q10 * (r2(v, 3, -37, 10) + r2(v, 18, -7, -10)),
+            ("C2", "O2"): q10 * r2(v, 0.08, -10, -15),
+            ("O2", "C2"): q10 * (r2(v, 2, -50, 7) + r2(v, 0.2, -20, -10)),
+            ("O1", "I1"): q10 * (r2(v, 8, -37, 13) + r2(v, 17, -7, -15)),
+            ("I1", "O1"): q10 * r2(v, 0.00001, -37, 10),
+            ("I1", "C1"): q10 * r2(v, 0.21, -61, 7),
+            ("C1", "I1"): q10 * r2(v, 0.3, -61, -5.5),
+            ("I1", "I2"): q10 * r2(v, 0.0015, -90, -5),
+            ("I2", "I1"): q10 * r2(v, 0.0075, -90, 15),
+        }
+        A = np.zeros((6, 6))
+        idx = {s: i for i, s in enumerate(km.states)}
+        for (a, b), rate in k.items():
+            A[idx[b], idx[a]] += rate
+            A[idx[a], idx[a]] -= rate
+        np.testing.assert_allclose(_rate_matrix(km, v), A, rtol=1e-13, atol=1e-15)
+
+
+@needs_reference
+def test_kv11_rates_by_hand():
+    """Kv1.1 (nmodl_library/kinetic/Kv11.mod): INITIAL sets F and A, rates() uses them, and the
+    reactions carry temperature factors; CS -> OS and OS -> CS restated at 30 degC."""
+    celsius = 30.0
+    km = nmodl.KineticModel(os.path.join(LIBRARY, "Kv11.mod"), celsius=celsius)
+    F, A = 96.485, 8.134 * (celsius + 273.15)
+    Zc, Vc, Hvc, Hkc, kc = 1.6756, -17.6122, 0.91934, 0.12166, 0.030179
+    ki, Hki = 0.00025091, 0.44916
+    for v in (-60.0, -10.0, 30.0):
+        kco = kc * math.exp(Zc * F / A * (v - Vc - Hvc * (celsius - 25))) * math.exp(Hkc * (celsius - 25))
+        koc = kc * math.exp(-Zc * F / A * (v - Vc - Hvc * (celsius - 25))) * math.exp(Hkc * (celsius - 25))
+        M = _rate_matrix(km, v)
+        i = {s: n for n, s in enumerate(km.states)}
+        assert M[i["OS"], i["CS"]] == pytest.approx(kco, rel=1e-13)
+        # OS leaves to CS (koc), to IC1 (ki e^{Hki (T-25)}) and to IN
+        assert M[i["CS"], i["OS"]] == pytest.approx(koc, rel=1e-13)
+        assert M[i["IC1"], i["OS"]] == pytest.approx(ki * math.exp(Hki * (celsius - 25)), rel=1e-13)
+    # conductance: g = gbar OS with gbar 36 mS/cm2, ik = g (v - ek) 1e-3 -> 0.036 S/cm2 in OS
+    w = km.conductance_weights("ek")
+    assert w[km.states.index("OS")] == pytest.approx(0.036, rel=1e-14)
+    assert np.count_nonzero(w) == 1
+
+
+@needs_reference
+def test_nav11_conductance_weights():
+    km = nmodl.KineticModel(os.path.join(LIBRARY, "Nav11.mod"), celsius=37.0)
+    w = km.conductance_weights("ena")
+    want = {"O1": 0.1, "O2": 0.1}
+    for s, x in zip(km.states, w):
+        assert x == pytest.approx(want.get(s, 0.0), abs=1e-15)
+    assert km.current_ion == "na"
+
+
+def test_own_files_parse():
+    """The kinetic files under tests/golden/kinetic (written for the GPU tests)."""
+    na = nmodl.KineticModel(os.path.join(OWN, "na6.mod"), celsius=22.0)
+    assert na.name == "na6" and na.num_states == 6 and na.inputs == ["v"]
+    assert na.current_ion == "na"
+    k = nmodl.KineticModel(os.path.join(OWN, "k4.mod"), celsius=22.0)
+    assert k.name == "k4" and k.num_states == 4 and k.current_ion == "k"
+    for km in (na, k):
+        for v in (-90.0, -20.0, 35.0):
+            A = _rate_matrix(km, v)
+            assert np.abs(A.sum(axis=0)).max() < 1e-12 * np.abs(A).max()
+            assert (A - np.diag(np.diag(A))).min() >= 0.0
+    scheme = nmodl.load_kinetic(os.path.join(OWN, "k4.mod"), time_step=0.025, celsius=22.0)
+    assert isinstance(scheme, channels.LTIScheme) and scheme.num_states == 4
+    tbl = scheme.node_table()
+    assert np.abs(tbl.sum(axis=1) - 1.0).max() < 1e-12        # columns of every node sum to one
+
+
+def test_unsupported_constructs_raise(tmp_path):
+    src = """NEURON { SUFFIX bad }
+STATE { a b }
+KINETIC k { ~ a <-> b (1, 2) }
+BREAKPOINT { SOLVE k METHOD sparse }
+NET_RECEIVE (w) { }
+"""
+    f = tmp_path / "bad.mod"
+    f.write_text(src)
+    with pytest.raises(nmodl.NmodlError, match="NET_RECEIVE"):
+        nmodl.KineticModel(str(f))
+    g = tmp_path / "deriv.mod"
+    g.write_text("NEURON { SUFFIX d }\nSTATE { x }\nDERIVATIVE s { x' = -x }\n")
+    with pytest.raises(nmodl.NmodlError):
+        nmodl.KineticModel(str(g))

@@ -508,3 +508,20 @@ def test_theta_tables_match_oracle():
     dn = channels.hh_gate_table_theta(0.05, 6.3, [1.1 - eps, 0.9, 1, 1.2, 0.8, 1.05])
     fd = (up[0, :, :2] - dn[0, :, :2]) / (2 * eps)
     np.testing.assert_allclose(got[0, :, 2:4], fd, rtol=1e-5, atol=1e-12)
+
+
+def test_kinetic_sim_constant_conductance_closed_form():
+    """KineticSim with one mechanism frozen open (zero rates): a single compartment relaxes to
+    the reversal potential exactly as exp(-t g / C), one half step late (the first half step
+    sees no conductance)."""
+    C, g, E, dt, v0 = np.array([2e-12]), 3e-10, -54.0, 0.1, -70.0
+    zero = lambda v, a, b: [0.0, 0.0]
+    poly = O.node_table_to_polynomial(O.lti_node_table(zero, 2, dt))
+    mech = {"poly": poly, "weights": [0.0, 1.0], "gmax": np.array([g]), "E": E,
+            "states": np.array([[0.0], [1.0]]), "conserve": 1.0}
+    sim = O.KineticSim(scipy.sparse.csr_matrix(np.eye(1)), C, [mech], v0=v0, dt=dt)
+    for k in range(1, 41):
+        sim.advance()
+        t_s = (k * dt - 0.5 * dt) * 1e-3
+        want = E + (v0 - E) * np.exp(-t_s * g / C[0])
+        assert sim.v[0] == pytest.approx(want, rel=1e-12)
