@@ -86,3 +86,19 @@ class CaPopulation:
     def cab(self):
         """Bound buffer per compartment (host, FP64)."""
         return self.cab_ref + self.cab_offset.cpu().numpy().astype(np.float64)
+
+    def save(self, path):
+        """Snapshot of Ca, the bound buffer (offset from cab_ref), the pending dc/dt and the step."""
+        from . import checkpoint
+        checkpoint.save(path, {"ca": self.ca, "cab_offset": self.cab_offset, "dcdt": self.dcdt},
+                        {"kind": "CaPopulation", "steps": self.steps, "dt": self.dt,
+                         "num_segments": self.num_segments, "cab_ref": self.cab_ref})
+
+    def load(self, path):
+        from . import checkpoint
+        arrays, meta = checkpoint.load(path, "CaPopulation")
+        if meta["num_segments"] != self.num_segments or meta["cab_ref"] != self.cab_ref:
+            raise ValueError(f"{path}: snapshot of another population")
+        for k in ("ca", "cab_offset", "dcdt"):
+            checkpoint.copy_into(getattr(self, k), arrays[k], k)
+        self.steps = int(meta["steps"])

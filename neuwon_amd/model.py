@@ -1103,7 +1103,7 @@ class RxD_Model:
         _lib.check(_lib.load().nw_hh_mechanism(
             M, 1 if mech.gate_mode == "exact" else 0, self.time_step,
             channels.q10_factor(self.temperature), P(mech._table), g.num_buckets + 1, g.minimum,
-            g.maximum, P(self._col("Segment", "voltage")), P(mc.columns["segment"].dev),
+            g.maximum, P(self._col("Segment", "voltage")), P(mech._seg_dev),
             P(mc.columns["m"].dev), P(mc.columns["h"].dev), P(mc.columns["n"].dev),
             P(mech._gbar_dev["na"]), P(mech._gbar_dev["k"]), P(mech._gbar_dev["l"]), P(out),
             self._stream()))
@@ -1147,6 +1147,63 @@ class RxD_Model:
             P(self._col("Segment", "sum_conductance")), P(self._col("Segment", "driving_voltage")),
             P(self._col("Segment", "sum_current")), P(self._inv_c),
             P(self._col("Segment", "integral_voltage")), self._stream()))
+
+    # ------------------------------------------------------------------ state snapshots
+    def save_state(self, path):
+        """Write a snapshot: every device-resident attribute of every class (voltages,
+        accumulators, mechanism states and magnitudes) in creation order, the structure it belongs
+        to (classes, sizes, parents, coordinates, diameters, mechanism segments), both clocks and
+        the mechanisms' last conductances.  TimeSeries that are playing or recording are not part
+        of it (sql.py:78-128 keeps less: a subset of the columns)."""
+        from . import checkpoint
+        self._ensure_built()
+        arrays, sizes = {}, {}
+        for cname, c in self.database.classes.items():
+            sizes[cname] = c.size
+            cid_rows = np.asarray(c.row_of[np.arange(c.size)], dtype=np.int64)
+            for col in c.columns.values():
+                arr = col.host_array()
+                if col.dev is not None:
+                    arr = arr[cid_rows]
+                arrays[f"{'state' if col.device else 'structure'}:{cname}.{col.name}"] = arr
+        arrays["model:conductances"] = self._gbuf
+        checkpoint.save(path, arrays, {"kind": "RxD_Model", "sizes": sizes,
+                                       "ticks": self.clock.ticks,
+                                       "input_ticks": self.input_hook.ticks,
+                                       "time_step": self.time_step})
+
+    def load_state(self, path):
+        """Restore a snapshot written by ``save_state`` into a model of the same structure."""
+        from . import checkpoint
+        arrays, meta = checkpoint.load(path, "RxD_Model")
+        if meta["time_step"] != self.time_step:
+            raise ValueError(f"{path}: time step {meta['time_step']} ms, this model {self.time_step}")
+        self._ensure_built()
+        sizes = {n: c.size for n, c in self.database.classes.items()}
+        if sizes != meta["sizes"]:
+            raise ValueError(f"{path}: classes and sizes {meta['sizes']} differ from {sizes}")
+        for key, arr in arrays.items():
+            kind, _, name = key.partition(":")
+            if kind == "structure":
+                cname, attr = name.split(".", 1)
+                mine = self.database.get_class(cname).columns[attr].host_array()
+                if not np.array_equal(np.asarray(mine), arr):
+                    raise ValueError(f"{path}: {name} differs from this model's")
+        for key, arr in arrays.items():
+            kind, _, name = key.partition(":")
+            if kind != "state":
+                continue
+            cname, attr = name.split(".", 1)
+            c = self.database.get_class(cname)
+            col = c.columns[attr]
+            col._flush()
+            dev_order = np.empty_like(arr)
+            dev_order[np.asarray(c.row_of[np.arange(c.size)], dtype=np.int64)] = arr
+            col.set_data(dev_order)
+        from .checkpoint import copy_into
+        copy_into(self._gbuf, arrays["model:conductances"], "conductances")
+        self.clock.ticks = int(meta["ticks"])
+        self.input_hook.ticks = int(meta["input_ticks"])
 
     # ------------------------------------------------------------------ the step (rxd_model.py:63-123)
     def advance(self):

@@ -245,3 +245,29 @@ class HHPopulation:
 
     def time(self):
         return self.steps * self.dt
+
+    # state snapshots ----------------------------------------------------------------------
+    _STATE = ("v", "m", "h", "n", "g_na", "g_k", "g_l", "ap_state", "ap_count", "ap_local",
+              "stim_local")
+
+    def save(self, path):
+        """Write every array the next step reads, and the step count (the Poisson stimulus is a
+        function of (seed, cell, tick), so it needs no state of its own)."""
+        from . import checkpoint
+        checkpoint.save(path, {k: getattr(self, k) for k in self._STATE},
+                        {"kind": "HHPopulation", "steps": self.steps, "dt": self.dt,
+                         "num_segments": self.num_segments, "num_cells": self.num_cells,
+                         "symmetric": self.propagator.symmetric, "stim": self.stim or None})
+
+    def load(self, path):
+        """Restore a snapshot written by ``save`` for the same cells."""
+        from . import checkpoint
+        arrays, meta = checkpoint.load(path, "HHPopulation")
+        if (meta["num_segments"], meta["num_cells"]) != (self.num_segments, self.num_cells) or \
+                meta["dt"] != self.dt:
+            raise ValueError(f"{path}: snapshot of {meta['num_cells']} cells / "
+                             f"{meta['num_segments']} segments at dt {meta['dt']}")
+        for k in self._STATE:
+            checkpoint.copy_into(getattr(self, k), arrays[k], k)
+        self.steps = int(meta["steps"])
+        self.set_stim(meta["stim"])

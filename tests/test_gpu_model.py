@@ -289,3 +289,37 @@ def test_advance_from_worker_thread(torch_gpu):
     t.join(timeout=120)
     assert not t.is_alive()
     assert np.array_equal(out["v"], main)
+
+
+@pytest.mark.parametrize("mechanism", ["hh", "hh_kinetic"])
+def test_mechanism_on_a_subset_of_segments(torch_gpu, mechanism):
+    """A mechanism instance reads the voltage of its own segment: a passive neuron created first
+    (its rows come first) does not change the spike of an hh neuron created after it, and the
+    passive neuron's own voltage follows only its own (absent) stimulus."""
+    from neuwon_amd.model import RxD_Model, TimeSeries
+
+    def run(with_passive):
+        m = RxD_Model(time_step=0.025, temperature=6.3, mechanisms=[mechanism])
+        if with_passive:
+            other = m.Neuron([0, 0, 0], 10).root
+            other.add_section([80, 0, 0], 2, maximum_segment_length=20)
+        root = m.Neuron([500, 0, 0], 12).root
+        dend = root.add_section([500, 100, 0], 2, maximum_segment_length=25)
+        for s in [root] + dend[1::2]:
+            s.insert(mechanism)
+        probe = TimeSeries().record(root, "voltage")
+        root.inject_current(0.2, 1.0)
+        for _ in range(400):
+            m.advance()
+        v = m.database.get_data("Segment.voltage")
+        passive = v[:len(v) - len(dend) - 1] if with_passive else np.zeros(0)
+        return np.array(probe.get_data()), passive
+    alone, _ = run(False)
+    beside, passive = run(True)
+    d = np.abs(alone - beside).max()
+    _say(f"{mechanism} on a subset: peak {alone.max():.1f} mV alone, {beside.max():.1f} mV beside a "
+         f"passive neuron (max |dv| {d:.2e} mV); passive neuron in [{passive.min():.3f}, "
+         f"{passive.max():.3f}] mV")
+    assert alone.max() > 20
+    assert d < 1e-3
+    assert np.abs(passive - (-70.0)).max() < 1e-3
