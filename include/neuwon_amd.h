@@ -246,6 +246,56 @@ int nw_weighted_sum(int64_t n, const double* w, const float* x, const float* y, 
 /* Stimulus generator, the same hash the step kernel evaluates (exposed for checks). */
 uint32_t nw_stim_hash(uint32_t seed, uint32_t cell, uint32_t tick);
 
+/* ------------------------------------------------------------------------------------------
+ * Clock events on the device (replaces the per-tick callbacks TimeSeries._play_implementation,
+ * TimeSeries._record_implementation and Trace._callback, neuwon/database/time.py:224-232,
+ * 287-302, 509-515, that Clock.tick runs, time.py:114-117).  One launch per tick of one clock:
+ * t = *tick + 1 is the clock's count after the tick, and *tick holds t when the launch ends, so a
+ * captured graph replays successive ticks.  *num_events is read from device memory as well.
+ * Events act on ticks first_tick <= t < end_tick, in table order (the order of registration):
+ *   NW_EVENT_SET / NW_EVENT_ADD: k = first_index + (t - first_tick), modulo length when loop != 0;
+ *     column[row] = wave[k] (SET) or (float)(wave[k] + (double)column[row]) (ADD), FP32 columns.
+ *   NW_EVENT_RECORD: ring[slot * ring_capacity + t % ring_capacity] = column[row] (as FP32).
+ *   NW_EVENT_TRACE: state[0..2] = mean, var, start of the exponentially weighted statistics of
+ *     column[row] (time.py:509-515): FP64 with every product and sum rounded separately, as the
+ *     reference's Python float arithmetic rounds them.
+ * A run of consecutive RECORD/TRACE events (indices i .. run_end-1) is sampled in parallel; SET and
+ * ADD events are applied one at a time, so several players on one element compose in order.
+ * ------------------------------------------------------------------------------------------ */
+#define NW_EVENT_SET 0
+#define NW_EVENT_ADD 1
+#define NW_EVENT_RECORD 2
+#define NW_EVENT_TRACE 3
+#define NW_DTYPE_F32 0
+#define NW_DTYPE_U8 1
+#define NW_DTYPE_I32 2
+
+typedef struct nw_event {
+    int32_t kind;          /* NW_EVENT_* */
+    int32_t dtype;         /* NW_DTYPE_* of the column (SET/ADD: FP32 only) */
+    void* column;          /* device column */
+    int64_t row;           /* element of the column */
+    int64_t first_tick;
+    int64_t end_tick;
+    const double* wave;    /* SET/ADD: samples (device, FP64) */
+    int64_t length;        /* SET/ADD: number of samples */
+    int64_t first_index;   /* SET/ADD: sample written on first_tick */
+    int64_t slot;          /* RECORD: row of the ring */
+    double* state;         /* TRACE: mean, var, start (device, FP64) */
+    double alpha, beta;    /* TRACE: exp(-dt / period), 1 - alpha */
+    int32_t loop;          /* SET/ADD: wrap around */
+    int32_t run_end;       /* RECORD/TRACE: end of this run of samplers */
+} nw_event;
+
+int nw_clock_tick(int64_t* tick, const int32_t* num_events, const nw_event* events, float* ring,
+                  int64_t ring_capacity, void* stream);
+
+/* TraceAll._callback (time.py:627-637) for every instance of an FP32 column, FP64 arithmetic:
+ *   diff = x - mean; incr = beta diff; mean += incr; var = alpha (var + diff incr).
+ * var may be NULL (no variance). */
+int nw_trace_all(int64_t n, const float* x, float* mean, float* var, double alpha, double beta,
+                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

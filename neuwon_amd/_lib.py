@@ -37,6 +37,17 @@ class NwCaParams(ctypes.Structure):
                 ("cab_ref", c_f64), ("block_threads", c_i32), ("pad_", c_i32)]
 
 
+NW_EVENT_SET, NW_EVENT_ADD, NW_EVENT_RECORD, NW_EVENT_TRACE = 0, 1, 2, 3
+NW_DTYPE_F32, NW_DTYPE_U8, NW_DTYPE_I32 = 0, 1, 2
+
+
+class NwEvent(ctypes.Structure):
+    _fields_ = [("kind", c_i32), ("dtype", c_i32), ("column", c_vp), ("row", c_i64),
+                ("first_tick", c_i64), ("end_tick", c_i64), ("wave", c_vp), ("length", c_i64),
+                ("first_index", c_i64), ("slot", c_i64), ("state", c_vp), ("alpha", c_f64),
+                ("beta", c_f64), ("loop", c_i32), ("run_end", c_i32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "nw_last_error": (ctypes.c_char_p, []),
@@ -66,6 +77,8 @@ SIGNATURES = {
     "nw_stim_hash": (c_u32, [c_u32, c_u32, c_u32]),
     "nw_ca_step": (c_i32, [c_vp, ctypes.POINTER(NwCaParams), c_i32, c_vp, c_vp, c_vp, c_vp]),
     "nw_weighted_sum": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_f64, c_vp, c_i32, c_vp]),
+    "nw_clock_tick": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nw_trace_all": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_f64, c_f64, c_vp]),
     "nw_hh_mechanism": (c_i32, [c_i64, c_i32, c_f32, c_f32, c_vp, c_i32, c_f32, c_f32, c_vp, c_vp,
                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
 }

@@ -22,13 +22,14 @@ Each half step runs nw_accumulate + nw_electric_half_step, the mechanisms run th
 (nw_hh_mechanism, nw_lti_advance) or, for Python plugin mechanisms, their ``advance()``; there is
 no CPU execution path.
 """
+import collections
 import contextlib
 import math
 import os
 
 import numpy as np
 
-from . import _lib, channels, geometry
+from . import _lib, channels, events, geometry
 
 EPSILON = float(np.finfo(np.float64).eps)
 NULL = -1
@@ -45,13 +46,15 @@ def _torch():
 # Clock and TimeSeries (neuwon/database/time.py)
 # ================================================================================================
 class Clock:
-    """Tick counter with callbacks; time.py:54-121."""
+    """Tick counter with callbacks; time.py:54-121.  A clock of a model may also carry device
+    callbacks (``events.DeviceEvents``), which run first on every tick."""
 
     def __init__(self, tick_period, units=""):
         self.dt = self.time_step = self.tick_period = float(tick_period)
         self.ticks = 0
         self.units = str(units)
         self._callbacks = []
+        self._events = None
 
     def get_time(self):
         return self.ticks * self.dt
@@ -81,67 +84,166 @@ class Clock:
         keep = [fn for fn in current if not fn()]
         self._callbacks = keep + self._callbacks   # plus any registered while running
 
+    def _set_ticks(self, ticks, callbacks=False):
+        """Move the clock; with callbacks (reset, set_time) they run at the new time."""
+        self.ticks = int(ticks)
+        if self._events is not None:
+            self._events.clock_moved(self.ticks if callbacks else self.ticks + 1)
+        if callbacks:
+            self._call()
+
     def reset(self):
-        self.ticks = 0
-        self._run_callbacks()
+        """time.py:105-108: zero, then the callbacks."""
+        self._set_ticks(0, callbacks=True)
 
     def set_time(self, new_time):
-        self.ticks = round(float(new_time) / self.dt)
-        self._run_callbacks()
+        self._set_ticks(round(float(new_time) / self.dt), callbacks=True)
 
     def tick(self):
         self.ticks += 1
+        self._call()
+
+    def _call(self):
+        ev = self._events
+        if ev is not None:
+            if ev.active():
+                ev.launch()
+            else:
+                ev.launched += 1
         self._run_callbacks()
+
+    def _count(self, k):
+        """k ticks whose device work a replayed step graph has done (such a clock has no host
+        callbacks)."""
+        self.ticks += k
+        self._events.launched += k
+        self._events.dev_count += k
 
     def is_now(self, time):
         return round(float(time) / self.dt) == self.ticks
 
 
 class TimeSeries:
-    """Buffer of (timestamp, value) samples with play back and recording (time.py:123-408 subset)."""
+    """Buffer of (timestamp, value) samples with play back and recording (time.py:123-408 subset).
+    Play back into, and recording of, a device attribute of a model run on the device, one table
+    launch per tick for all of a clock's series (``events.DeviceEvents``); other attributes use
+    host callbacks."""
 
     def __init__(self, initial_data=None, timestamps=None):
-        self.data = list(initial_data) if initial_data is not None else []
-        self.timestamps = list(timestamps) if timestamps is not None else []
+        self.data = collections.deque(initial_data) if initial_data is not None else collections.deque()
+        self.timestamps = collections.deque(timestamps) if timestamps is not None else collections.deque()
         self._playing = None
         self._recording = False
+        self._entry = None
+        self.play_data = None
+
+    def _sync(self):
+        if self._entry is not None and isinstance(self._entry, events.Recorder):
+            self._entry.events.flush()
 
     def get_data(self):
+        self._sync()
         return list(self.data)
 
     def get_timestamps(self):
+        self._sync()
         return list(self.timestamps)
 
     def __len__(self):
+        self._sync()
         return len(self.data)
 
     def set_data(self, data_samples, timestamps):
-        self.data = list(data_samples)
-        self.timestamps = list(timestamps)
+        assert self.is_stopped()
+        self.data = collections.deque(data_samples)
+        self.timestamps = collections.deque(timestamps)
+        assert len(self.data) == len(self.timestamps)
+        return self
+
+    def clear(self):
+        assert self.is_stopped()
+        self.data, self.timestamps = collections.deque(), collections.deque()
         return self
 
     def constant_wave(self, value, duration):
         """time.py:406-408."""
         return self.set_data([value, value], [0, duration])
 
+    def square_wave(self, extremum_1, extremum_2, period, duty_cycle=0.5):
+        """time.py:410-420: one cycle."""
+        a, b, period, duty_cycle = float(extremum_1), float(extremum_2), float(period), float(duty_cycle)
+        assert 0.0 <= duty_cycle <= 1.0
+        mid = period * duty_cycle
+        return self.set_data([a, a, b, b], [0, mid, mid, period])
+
+    def sine_wave(self, extremum_1, extremum_2, period):
+        """time.py:422-432: one cycle, 1000 points."""
+        a, b, period = float(extremum_1), float(extremum_2), float(period)
+        amplitude = 0.5 * (a - b)
+        offset = b + amplitude
+        return self.set_data(offset + amplitude * np.sin(np.linspace(0.0, 2.0 * np.pi, 1000)),
+                             np.linspace(0, period, 1000))
+
+    def triangle_wave(self, extremum_1, extremum_2, period):
+        """time.py:434-439."""
+        a, b, period = float(extremum_1), float(extremum_2), float(period)
+        return self.set_data([a, b, a], [0.0, 0.5 * period, period])
+
+    def sawtooth_wave(self, extremum_1, extremum_2, period):
+        """time.py:441-446."""
+        a, b, period = float(extremum_1), float(extremum_2), float(period)
+        return self.set_data([a, b, a], [0.0, period, period])
+
+    def is_playing(self):
+        if isinstance(self._entry, events.Player):
+            return self._entry.playing()
+        return self._playing is not None
+
+    def is_recording(self):
+        if isinstance(self._entry, events.Recorder):
+            return self._entry.recording()
+        return self._recording
+
     def is_stopped(self):
-        return self._playing is None and not self._recording
+        return not self.is_playing() and not self.is_recording()
 
     def stop(self):
+        if self._entry is not None:
+            self._entry.events.remove(self._entry)
+            self._entry = None
         self._playing = None
         self._recording = False
+        return self
 
     def play(self, db_object, component, mode="+=", loop=False, clock=None, immediate=True):
         """Write one value per clock tick into ``db_object.component`` (time.py:245-302)."""
-        assert self._playing is None
+        assert self.is_stopped()
+        self._entry = None
+        model = getattr(type(db_object), "_model", None)
         clock = clock or db_object._model.clock
         dt = clock.get_tick_period()
         start = math.floor(self.timestamps[0] / dt)
         end = math.ceil(self.timestamps[-1] / dt)
         ts = dt * np.arange(start, end)
-        values = np.interp(ts, self.timestamps, self.data, left=self.data[0], right=self.data[-1])
+        import scipy.interpolate
+        f = scipy.interpolate.interp1d(list(self.timestamps), list(self.data),
+                                       fill_value=(self.data[0], self.data[-1]), bounds_error=False)
+        values = np.vectorize(f)(ts)
+        self.play_data = values
+        if mode not in ("=", "+="):
+            raise NotImplementedError(mode)
         self._playing = dict(obj=db_object, component=component, data=values, index=0,
                              loop=bool(loop), mode=mode)
+        if events.device_column(db_object, component) is not None and model is not None:
+            if immediate:
+                self._play_tick()
+                if self._playing is None:
+                    return self
+            ev = model._events_for(clock)
+            self._entry = ev.add(events.Player(ev, db_object, component, values,
+                                               self._playing["index"], loop, mode))
+            self._playing = None
+            return self
         clock.register_callback(self._play_tick)
         if immediate:
             self._play_tick()
@@ -155,10 +257,8 @@ class TimeSeries:
         obj, name = st["obj"], st["component"]
         if st["mode"] == "=":
             setattr(obj, name, value)
-        elif st["mode"] == "+=":
-            setattr(obj, name, value + getattr(obj, name))
         else:
-            raise NotImplementedError(st["mode"])
+            setattr(obj, name, value + getattr(obj, name))
         st["index"] += 1
         if st["index"] >= len(st["data"]):
             if st["loop"]:
@@ -171,11 +271,26 @@ class TimeSeries:
     def record(self, db_object, component, record_duration=np.inf, discard_after=np.inf,
                clock=None, immediate=True):
         """Sample ``db_object.component`` after every clock tick (time.py:189-232)."""
+        assert self.is_stopped()
+        self._entry = None
+        model = getattr(type(db_object), "_model", None)
         self._clock = clock or db_object._model.clock
         self._rec = (db_object, component)
         self._recording = True
         self._remaining = float(record_duration)
         self._discard_after = float(discard_after)
+        col = events.device_column(db_object, component)
+        if col is not None and model is not None:
+            n = events.num_samples(record_duration, self._clock.dt)
+            if immediate and n:
+                self._record_tick()
+                n -= 1
+            ev = model._events_for(self._clock)
+            sink = events.SampleBuffer(self, ev, discard_after,
+                                       integer=np.dtype(col.device_dtype).kind in "ui")
+            self._entry = ev.add(events.Recorder(ev, db_object, component, n, sink))
+            self._recording = False
+            return self
         self._clock.register_callback(self._record_tick)
         if immediate:
             self._record_tick()
@@ -186,13 +301,127 @@ class TimeSeries:
             self._recording = False
             return True
         obj, name = self._rec
-        self.data.append(float(getattr(obj, name)))
+        self.data.append(getattr(obj, name))
         self.timestamps.append(self._clock())
         while self.timestamps[-1] - self.timestamps[0] > self._discard_after:
-            self.data.pop(0)
-            self.timestamps.pop(0)
+            self.data.popleft()
+            self.timestamps.popleft()
         self._remaining -= self._clock.dt
         return False
+
+
+class Trace:
+    """Exponentially weighted mean and standard deviation of one attribute of one object
+    (time.py:461-537).  On a device attribute the update runs in the clock's device table in
+    FP64 with the reference's roundings; otherwise as a host callback."""
+
+    def __init__(self, db_object, component, period):
+        self.db_object, self.component_name = db_object, str(component)
+        self.period = float(period)
+        assert self.period > 0.0
+        model = type(db_object)._model
+        self.clock = model.clock
+        dt = self.clock.get_tick_period()
+        self.alpha = float(np.exp(-dt / self.period))
+        self.beta = 1.0 - self.alpha
+        self.mean = self.var = 0.0
+        self.start = 1.0
+        self._entry = None
+        self._callback()           # the current value is the first sample
+        if events.device_column(db_object, component) is not None:
+            torch = _torch()
+            model._ensure_built()
+            self._state = torch.tensor([self.mean, self.var, self.start], dtype=torch.float64,
+                                       device=model._dev)
+            ev = model._events_for(self.clock)
+            self._entry = ev.add(events.Tracer(ev, db_object, component, self._state,
+                                               self.alpha, self.beta))
+        else:
+            self.clock.register_callback(self._callback)
+
+    def _pull(self):
+        if self._entry is not None:
+            self.mean, self.var, self.start = self._state.cpu().tolist()
+
+    def reset(self):
+        self.mean = self.var = 0.0
+        self.start = 1.0
+        if self._entry is not None:
+            self._state.copy_(_torch().tensor([0.0, 0.0, 1.0], dtype=_torch().float64))
+
+    def _callback(self):
+        value = float(getattr(self.db_object, self.component_name))
+        self.mean += self.beta * (value - self.mean)
+        self.start *= self.alpha
+        true_mean = self.mean / (1.0 - self.start)
+        self.var += self.beta * (value - true_mean) ** 2
+        self.var *= self.alpha
+
+    def get_mean(self):
+        self._pull()
+        return self.mean / (1.0 - self.start)
+
+    def get_variance(self):
+        self._pull()
+        return self.var / (1.0 - self.start)
+
+    def get_standard_deviation(self):
+        return self.get_variance() ** 0.5
+
+
+class TraceAll:
+    """Exponentially weighted mean and variance of every instance of an attribute, kept as new
+    attributes of its class (time.py:539-647): ``nw_trace_all`` after every tick of the model's
+    clock."""
+
+    def __init__(self, db_component, period, mean=True, variance=True):
+        self.component = db_component
+        self.period = float(period)
+        assert self.period > 0.0 and mean
+        cls = db_component.db_class
+        model = cls.database._model
+        name = db_component.name
+        if mean is True:
+            mean = f"{name}_mean"
+        if variance is True:
+            variance = f"{name}_variance"
+        model._ensure_built()
+        self.model, self.cls = model, cls
+        self.clock = model.clock
+        dt = self.clock.get_tick_period()
+        self.alpha = float(np.exp(-dt / self.period))
+        self.beta = 1.0 - self.alpha
+        self.mean = cls.add_attribute(mean, db_component.initial_value, units=db_component.units)
+        self.var = cls.add_attribute(variance, 0.0, valid_range=(0.0, None)) if variance else None
+        order = cls._order if getattr(cls, "_order", None) is not None else np.arange(cls.size)
+        for col in (self.mean, self.var):
+            if col is not None:
+                col.to_device(order, model._dev)
+        self.mean.dev.copy_(db_component.dev)
+        self._launch(model._stream())           # the current value is the first sample
+        model._events_for(self.clock).trace_all.append(self)
+
+    def _launch(self, stream):
+        P = _lib.ptr
+        x = self.component.dev
+        _lib.check(_lib.load().nw_trace_all(
+            x.numel(), P(x), P(self.mean.dev), P(self.var.dev) if self.var is not None else None,
+            self.alpha, self.beta, stream))
+
+    def reset(self):
+        self.mean.dev.copy_(self.component.dev)
+        if self.var is not None:
+            self.var.dev.zero_()
+
+    def get_mean(self):
+        return self.mean.host_array()
+
+    def get_variance(self):
+        assert self.var is not None
+        return self.var.host_array()
+
+    def get_standard_deviation(self):
+        return self.get_variance() ** 0.5
 
 
 # ================================================================================================
@@ -784,7 +1013,7 @@ class RxD_Model:
     def __init__(self, time_step=0.1, *, temperature=37, initial_voltage=-70,
                  cytoplasmic_resistance=100, membrane_capacitance=1, extracellular_tortuosity=1.55,
                  extracellular_max_distance=20e-6, species=(), mechanisms=(), device=0,
-                 prune=EPSILON, symmetric_propagator=False):
+                 prune=EPSILON, symmetric_propagator=False, step_graph=True):
         self.time_step = float(time_step)
         self.temperature = float(temperature)
         self.initial_voltage = float(initial_voltage)
@@ -794,12 +1023,20 @@ class RxD_Model:
         # symmetric_propagator: store sqrt(C) P sqrt(C)^-1, one triangle (half the bytes per half
         # step; pairs pruned together; sums in atomic order, so not bitwise reproducible)
         self.symmetric_propagator = bool(symmetric_propagator)
+        # step_graph: when every callback of a step runs on the device (device mechanisms only,
+        # play back / recording / traces of device attributes), advance() replays a captured
+        # graph of the step's kernels instead of launching them one by one from Python
+        self.step_graph = bool(step_graph)
         self.device = int(device)
         self.database = db = Database()
         db._model = self
         self.input_hook = Clock(0.5 * self.time_step, units="ms")
         self.advance_hook = self.clock = Clock(self.time_step, units="ms")
         db.add_clock(self.clock)
+        for c in (self.input_hook, self.clock):
+            c._events = events.DeviceEvents(self, c)
+        self._graph = self._graph_key = self._graph_seen = None
+        self._build_count = 0
         self._children = []
         self._neuron_types = {}
         self._built = False
@@ -869,6 +1106,12 @@ class RxD_Model:
 
     def get_time_step(self):
         return self.time_step
+
+    def _events_for(self, clock):
+        """The device callback table of a clock (created for clocks other than the model's)."""
+        if clock._events is None:
+            clock._events = events.DeviceEvents(self, clock)
+        return clock._events
 
     def register_input_callback(self, function):
         self.input_hook.register_callback(function)
@@ -1036,6 +1279,7 @@ class RxD_Model:
         self._acc_idx = torch.as_tensor(sidx[so].astype(np.int64), device=dev)
         self._acc_E = torch.as_tensor(sE[so].astype(np.float32), device=dev)
         self._built = True
+        self._build_count += 1
 
     def _upload_references(self):
         torch = _torch()
@@ -1056,12 +1300,24 @@ class RxD_Model:
         col._flush()
         version = col.dev._version
         if getattr(mech, "_mag_version", None) != version:
+            # written into the existing arrays, so a captured step graph keeps reading them
             mag = col.dev
             if hasattr(mech, "gate_mode"):
-                mech._gbar_dev = {ion: (mag * (mech.gbar[ion] * channels.CM2_TO_UM2)).contiguous()
-                                  for ion in ("na", "k", "l")}
+                new = {ion: (mag * (mech.gbar[ion] * channels.CM2_TO_UM2)).contiguous()
+                       for ion in ("na", "k", "l")}
+                old = getattr(mech, "_gbar_dev", None)
+                if old is not None and all(old[i].shape == new[i].shape for i in new):
+                    for i in new:
+                        old[i].copy_(new[i])
+                else:
+                    mech._gbar_dev = new
             else:
-                mech._gmax_dev = (mag * mech.spec.gmax).contiguous()
+                new = (mag * mech.spec.gmax).contiguous()
+                old = getattr(mech, "_gmax_dev", None)
+                if old is not None and old.shape == new.shape:
+                    old.copy_(new)
+                else:
+                    mech._gmax_dev = new
             mech._mag_version = version
 
     def _prepare_mechanism(self, mech, dev):
@@ -1092,11 +1348,15 @@ class RxD_Model:
         return self.database.get_class(cls).columns[name].dev
 
     def _run_hh_mechanism(self, mech):
+        if mech._db_class.size:
+            self._scaled_by_magnitude(mech)
+            self._hh_kernel(mech)
+
+    def _hh_kernel(self, mech):
         mc = mech._db_class
         M = mc.size
         if M == 0:
             return
-        self._scaled_by_magnitude(mech)
         P = _lib.ptr
         out = self._gbuf[mech._g_offset:mech._g_offset + 3 * M]
         g = mech._grid
@@ -1109,12 +1369,16 @@ class RxD_Model:
             self._stream()))
 
     def _run_lti_mechanism(self, mech):
+        if mech._db_class.size:
+            self._scaled_by_magnitude(mech)
+            self._lti_kernel(mech)
+
+    def _lti_kernel(self, mech):
         mc = mech._db_class
         M = mc.size
         if M == 0:
             return
         spec = mech.spec
-        self._scaled_by_magnitude(mech)
         P = _lib.ptr
         out = self._gbuf[mech._g_offset:mech._g_offset + M]
         grid = spec.scheme.grid
@@ -1140,7 +1404,9 @@ class RxD_Model:
             return
         for col in self.database.get_class("Segment").columns.values():
             col._flush()
-        prop = self._propagator(time_step)
+        self._half_step_kernel(self._propagator(time_step), time_step)
+
+    def _half_step_kernel(self, prop, time_step):
         P = _lib.ptr
         _lib.check(_lib.load().nw_electric_half_step(
             prop.handle, time_step, P(self._col("Segment", "voltage")),
@@ -1202,8 +1468,8 @@ class RxD_Model:
             col.set_data(dev_order)
         from .checkpoint import copy_into
         copy_into(self._gbuf, arrays["model:conductances"], "conductances")
-        self.clock.ticks = int(meta["ticks"])
-        self.input_hook.ticks = int(meta["input_ticks"])
+        self.clock._set_ticks(int(meta["ticks"]))
+        self.input_hook._set_ticks(int(meta["input_ticks"]))
 
     # ------------------------------------------------------------------ the step (rxd_model.py:63-123)
     def advance(self):
@@ -1211,12 +1477,83 @@ class RxD_Model:
         self._ensure_built()
         if self._refs_dirty:
             self._upload_references()
+        if self.step_graph and self._graphable():
+            self._advance_graphed()
+            return
         self._advance_species()
         self._advance_mechanisms()
         self._advance_species()
         self._advance_ap_detector()
         self.advance_hook.tick()
 
+    # ------------------------------------------------------------------ the step as one graph
+    def _device_mechanisms(self):
+        return [m for m in self.mechanisms.values()
+                if isinstance(m, type) and issubclass(m, _DeviceMechanism)]
+
+    def _graphable(self):
+        """Every callback of the step runs on the device: no host callbacks on either clock and
+        no Python plugin mechanisms."""
+        if self.clock._callbacks or self.input_hook._callbacks:
+            return False
+        if len(self._device_mechanisms()) != len(self.mechanisms):
+            return False
+        return self.database.get_class("Segment").size > 0
+
+    def _launch_step(self, prop):
+        """The kernels of one step (rxd_model.py:63-123) with no host work between them."""
+        half = 0.5 * self.time_step
+        inputs, clock = self.input_hook._events, self.clock._events
+        for _ in range(2):
+            self._accumulate_kernel()
+            inputs.launch_kernels(self._stream())
+            self._half_step_kernel(prop, half)
+            if _ == 0:
+                self._zero_accumulators()
+                for m in self.mechanisms.values():
+                    if hasattr(m, "gate_mode"):
+                        self._hh_kernel(m)
+                    else:
+                        self._lti_kernel(m)
+                self._copy_species_conductances()
+        self._ap_kernel()
+        clock.launch_kernels(self._stream())
+
+    def _advance_graphed(self):
+        """One step as a replay of a captured graph of ``_launch_step``.  The first step of each
+        configuration runs the same kernels eagerly; the next one captures them.  Host writes,
+        magnitude changes and the clocks' tables are brought up to date before every replay."""
+        torch = _torch()
+        for c in self.database.classes.values():
+            for col in c.columns.values():
+                col._flush()
+        for m in self._device_mechanisms():
+            if m._db_class.size:
+                self._scaled_by_magnitude(m)
+        self.input_hook._events.prepare(2)
+        self.clock._events.prepare(1)
+        prop = self._propagator(0.5 * self.time_step)
+        key = (self._build_count, id(prop), self._gbuf.data_ptr(), self._ap_seg.data_ptr(),
+               float(self.database.get_class("Neuron").class_attributes["AP_detector_threshold"]),
+               tuple(id(m) for m in self.mechanisms.values()),
+               self.input_hook._events.graph_key(), self.clock._events.graph_key())
+        if key != self._graph_key:
+            self._graph = None
+            if self._graph_seen != key:
+                self._graph_seen = key
+                self._launch_step(prop)
+                self.input_hook._count(2)
+                self.clock._count(1)
+                return
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.device(self._dev), torch.cuda.graph(g):
+                self._launch_step(prop)
+            self._graph, self._graph_key = g, key
+        self._graph.replay()
+        self.input_hook._count(2)
+        self.clock._count(1)
+
+    # ------------------------------------------------------------------ the step, eagerly
     def _advance_species(self):
         """Accumulate conductances and currents, run input callbacks, one dt/2 electric step
         (rxd_model.py:91-113)."""
@@ -1225,34 +1562,48 @@ class RxD_Model:
         if N == 0:
             self.input_hook.tick()
             return
-        P = _lib.ptr
         for col in seg.columns.values():
             col._flush()
+        self._accumulate_kernel()
+        self.input_hook.tick()
+        self._electric_step(0.5 * self.time_step)
+
+    def _accumulate_kernel(self):
+        seg = self.database.get_class("Segment")
+        N = seg.size
+        P = _lib.ptr
         _lib.check(_lib.load().nw_accumulate(
             N, P(self._acc_ptr), P(self._gbuf), P(self._acc_idx), P(self._acc_E),
             P(self._col("Segment", "nonspecific_current")), P(self._col("Segment", "sum_conductance")),
             P(self._col("Segment", "driving_voltage")), P(self._col("Segment", "sum_current")),
             self._stream()))
-        self.input_hook.tick()
-        self._electric_step(0.5 * self.time_step)
 
     def _advance_mechanisms(self):
         """Zero the accumulators and advance every mechanism (rxd_model.py:115-123)."""
-        self._gbuf.zero_()
-        for name, _ in self._species_cols:          # species._zero_accumulators (species.py:147-153)
-            self._col("Segment", f"{name}_conductance").zero_()
-        self._col("Segment", "nonspecific_current").zero_()
+        self._zero_accumulators()
         for name, m in self.mechanisms.items():
             try:
                 m.advance()
             except Exception:
                 print(f"ERROR in mechanism {name}.")
                 raise
+        self._copy_species_conductances()
+
+    def _zero_accumulators(self):
+        self._gbuf.zero_()
+        for name, _ in self._species_cols:          # species._zero_accumulators (species.py:147-153)
+            self._col("Segment", f"{name}_conductance").zero_()
+        self._col("Segment", "nonspecific_current").zero_()
+
+    def _copy_species_conductances(self):
         for name, off in self._species_cols:
             seg_col = self._col("Segment", f"{name}_conductance")
             self._gbuf[off:off + len(seg_col)].copy_(seg_col)
 
     def _advance_ap_detector(self):
+        self._ap_kernel()
+
+    def _ap_kernel(self):
         neu = self.database.get_class("Neuron")
         if neu.size == 0:
             return
