@@ -139,6 +139,37 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
                    const float* gmax, float* cond_out, void* stream);
 
 /* ------------------------------------------------------------------------------------------
+ * LTI kinetic advance with one or two inputs of either mapping (replaces the kernel generated by
+ * lti_sim/codegen.py:39-106 for schemes whose inputs are the voltage and/or a concentration, e.g.
+ * AMPA.mod's glutamate, NMDA.mod's transmitter and voltage; inputs mapped as
+ * lti_sim/inputs.py:72-109).  Input k of instance i is x = values[index ? index[i] : i] and
+ * maps to a fractional node position u:
+ *   NW_INPUT_LINEAR:  u = (x - minimum) * (num_nodes - 1) / (maximum - minimum)
+ *   NW_INPUT_LOG2:    u = (log2(x + scale) - log2(minimum + scale)) * (num_nodes - 1)
+ *                         / (log2(maximum + scale) - log2(minimum + scale))
+ * node b = trunc(u) clamped to [0, num_nodes - 2] (fraction 1 above, 0 below).  node_table holds
+ * [node_0][node_1][S][S] FP32 propagators (row-major entries, input 0 slowest); one input
+ * interpolates linearly, two bilinearly.  States, CONSERVE and the conductance output as in
+ * nw_lti_advance.  The inputs are read once per launch (held over n_steps).
+ * ------------------------------------------------------------------------------------------ */
+#define NW_INPUT_LINEAR 0
+#define NW_INPUT_LOG2 1
+
+typedef struct nw_lti_input {
+    const float* values;    /* device */
+    const int32_t* index;   /* device, may be NULL */
+    int32_t mapping;        /* NW_INPUT_* */
+    int32_t num_nodes;      /* >= 2 */
+    double minimum, maximum;
+    double scale;           /* NW_INPUT_LOG2 */
+} nw_lti_input;
+
+int nw_lti_advance_nd(int32_t S, int32_t num_inputs, const nw_lti_input* inputs_host,
+                      const float* node_table, int64_t M, float* states, int64_t state_stride,
+                      float conserve_sum, int32_t n_steps, const float* weights, const float* gmax,
+                      float* cond_out, void* stream);
+
+/* ------------------------------------------------------------------------------------------
  * HH mechanism instances (replaces the advance generated for hh.mod, nmodl/__init__.py:408-432
  * with the LTI or exact gate update, and its breakpoint, hh.mod:41-46): M instances, instance i on
  * segment seg_index[i] (NULL: segment i).  gate_mode 0: two-state LTI tables (layout as nw_hh_step);

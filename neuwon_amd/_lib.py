@@ -48,6 +48,14 @@ class NwEvent(ctypes.Structure):
                 ("beta", c_f64), ("loop", c_i32), ("run_end", c_i32)]
 
 
+NW_INPUT_LINEAR, NW_INPUT_LOG2 = 0, 1
+
+
+class NwLtiInput(ctypes.Structure):
+    _fields_ = [("values", c_vp), ("index", c_vp), ("mapping", c_i32), ("num_nodes", c_i32),
+                ("minimum", c_f64), ("maximum", c_f64), ("scale", c_f64)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "nw_last_error": (ctypes.c_char_p, []),
@@ -69,6 +77,8 @@ SIGNATURES = {
     "nw_ap_detect": (c_i32, [c_i64, c_vp, c_vp, c_f32, c_vp, c_vp, c_vp]),
     "nw_lti_advance": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
                                c_f32, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "nw_lti_advance_nd": (c_i32, [c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_i64, c_f32, c_i32,
+                                  c_vp, c_vp, c_vp, c_vp]),
     "nw_hh_step": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "nw_hh_adjoint": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp,

@@ -86,6 +86,45 @@ class LinearInput:
         return self.get_input_value(np.arange(self.num_buckets + 1, dtype=np.float64))
 
 
+class LogarithmicInput:
+    """Bucket mapping of a concentration: log2(x + scale) on linear buckets; lti_sim/inputs.py:86-109
+    (minimum 0)."""
+
+    def __init__(self, minimum, maximum, num_buckets, scale):
+        self.minimum = float(minimum)
+        self.maximum = float(maximum)
+        assert self.minimum == 0.0, "logarithmic inputs start at 0"
+        assert self.minimum < self.maximum
+        self.num_buckets = int(num_buckets)
+        self.scale = float(scale)
+        self.range = self.maximum - self.minimum
+        self.log2_minimum = np.log2(self.minimum + self.scale)
+        self.log2_maximum = np.log2(self.maximum + self.scale)
+        self.log2_range = self.log2_maximum - self.log2_minimum
+        self.bucket_frq = self.num_buckets / self.log2_range
+        self.bucket_width = self.log2_range / self.num_buckets
+
+    def get_bucket_value(self, value):
+        return (np.log2(value + self.scale) - self.log2_minimum) * self.bucket_frq
+
+    def get_input_value(self, bucket_value):
+        return 2.0 ** (self.log2_minimum + bucket_value * self.bucket_width) - self.scale
+
+    def get_bucket_location(self, value):
+        location = self.get_bucket_value(value)
+        b = int(location) if np.isfinite(location) else -1
+        if b < 0:
+            return 0, 0.0
+        if b >= self.num_buckets:
+            return self.num_buckets - 1, 1.0
+        return b, location - b
+
+    def nodes(self):
+        x = self.get_input_value(np.arange(self.num_buckets + 1, dtype=np.float64))
+        x[0], x[-1] = self.minimum, self.maximum        # inputs.py:57-59
+        return x
+
+
 def two_state_propagator(alpha, beta, dt):
     """Exact expm([[-a, b], [a, -b]] dt), state order (C, O), vectorised over a and b.
     Returns (P_OC, P_OO): open' = P_OC * closed + P_OO * open."""
@@ -110,59 +149,86 @@ def hh_gate_table(dt, celsius=6.3, grid=None):
 
 
 class LTIScheme:
-    """A kinetic scheme that is linear in its states with a single voltage input.
+    """A kinetic scheme that is linear in its states, driven by one or two inputs (the membrane
+    voltage and/or a concentration).
 
-    ``derivative(v, *states) -> list of d(state)/dt`` (1/ms); ``conserve_sum`` as the CONSERVE
-    statement (None when absent).  Mirrors LTI_Model (lti_sim/lti_model.py:5-78)."""
+    ``derivative(*inputs, *states) -> list of d(state)/dt`` (1/ms); ``inputs``: the bucket
+    mapping of each input (``LinearInput`` / ``LogarithmicInput``; default one voltage input over
+    [-100, 100] mV, 2000 buckets), ``input_names`` their names; ``conserve_sum`` as the CONSERVE
+    statement (None when absent); ``rate_matrices`` optionally evaluates the rate matrices of many
+    input values at once.  Mirrors LTI_Model (lti_sim/lti_model.py:5-78)."""
 
     def __init__(self, name, state_names, derivative, conserve_sum=None, time_step=0.1,
-                 grid=None, weights=None):
+                 grid=None, weights=None, inputs=None, input_names=None, rate_matrices=None):
         self.name = str(name)
         self.state_names = list(state_names)
         self.num_states = len(self.state_names)
         self.derivative = derivative
         self.conserve_sum = None if conserve_sum is None else float(conserve_sum)
         self.time_step = float(time_step)
-        self.grid = grid or LinearInput(-100.0, 100.0, 2000)
+        self.inputs = list(inputs) if inputs is not None else \
+            [grid or LinearInput(-100.0, 100.0, 2000)]
+        self.input_names = list(input_names) if input_names is not None else ["v"]
+        assert len(self.inputs) == len(self.input_names) and 1 <= len(self.inputs) <= 2
+        self.grid = self.inputs[0]
         self.weights = None if weights is None else np.asarray(weights, dtype=np.float64)
+        self._rate_matrices = rate_matrices
         self._check_is_lti()
+
+    @property
+    def num_inputs(self):
+        return len(self.inputs)
 
     def _check_is_lti(self):
         """lti_model.py:40-49."""
         rng = np.random.default_rng(12345)
         for _ in range(3):
-            v = rng.uniform(self.grid.minimum, self.grid.maximum)
+            x = [rng.uniform(g.minimum, g.maximum) for g in self.inputs]
             s1 = rng.uniform(0.0, 1.0, self.num_states)
-            d1 = np.asarray(self.derivative(v, *s1), dtype=np.float64)
-            d2 = np.asarray(self.derivative(v, *(2.0 * s1)), dtype=np.float64)
-            if not (np.all(np.isfinite(d1)) and np.all(np.abs(d1 - d2 / 2.0) < 1e-12)):
+            d1 = np.asarray(self.derivative(*x, *s1), dtype=np.float64)
+            d2 = np.asarray(self.derivative(*x, *(2.0 * s1)), dtype=np.float64)
+            if not (np.all(np.isfinite(d1)) and
+                    np.all(np.abs(d1 - d2 / 2.0) <= 1e-12 * np.maximum(1.0, np.abs(d1)))):
                 raise ValueError(f"{self.name}: non-linear system detected")
 
-    def rate_matrix(self, v):
+    def rate_matrix(self, *x):
         S = self.num_states
         A = np.empty((S, S))
         for col in range(S):
-            unit = [float(x == col) for x in range(S)]
-            A[:, col] = self.derivative(v, *unit)
+            unit = [float(k == col) for k in range(S)]
+            A[:, col] = self.derivative(*x, *unit)
         return A
 
-    def make_matrix(self, v, time_step=None):
-        """lti_model.py:51-65."""
+    def _rates(self, *arrays):
+        """Rate matrices [..., S, S] at arrays of input values (broadcast together)."""
+        arrays = np.broadcast_arrays(*[np.asarray(a, dtype=np.float64) for a in arrays])
+        if self._rate_matrices is not None:
+            return self._rate_matrices(*arrays)
+        flat = [a.ravel() for a in arrays]
+        A = np.stack([self.rate_matrix(*[f[k] for f in flat]) for k in range(flat[0].size)])
+        return A.reshape(arrays[0].shape + (self.num_states, self.num_states))
+
+    def make_matrix(self, *x, time_step=None):
+        """lti_model.py:51-65: expm(A dt), every column divided by its sum."""
         dt = self.time_step if time_step is None else float(time_step)
-        P = scipy.linalg.expm(self.rate_matrix(v) * dt)
+        P = scipy.linalg.expm(self.rate_matrix(*x) * dt)
         P /= P.sum(axis=0, keepdims=True)
         return P
 
     def node_table(self):
-        """FP64 [node][row][col] propagators on the grid nodes."""
-        return np.stack([self.make_matrix(v) for v in self.grid.nodes()])
+        """FP64 [node_0](x [node_1])[row][col] propagators on the grid nodes (input 0 slowest)."""
+        grids = np.meshgrid(*[g.nodes() for g in self.inputs], indexing="ij")
+        A = self._rates(*grids)
+        P = scipy.linalg.expm(A * self.time_step)
+        P /= P.sum(axis=-2, keepdims=True)
+        return P
 
-    def initial_state(self, v):
-        """Steady state at voltage v: expm(A * 1 hour) applied to the uniform state
-        (lti_model.py:67-78, evaluated at the given voltage)."""
+    def initial_state(self, *x):
+        """Steady state at the given input values: expm(A * 1 hour) applied to the uniform state
+        (lti_model.py:67-78, evaluated at the given inputs)."""
         if self.conserve_sum is None:
             return np.zeros(self.num_states)
-        P = self.make_matrix(v, time_step=3600e3)
+        P = self.make_matrix(*x, time_step=3600e3)
         return P.dot(np.full(self.num_states, self.conserve_sum / self.num_states))
 
 

@@ -630,6 +630,94 @@ k_lti(int num_nodes, float vmin, float vmax, const float* __restrict__ table, in
 }
 
 // ----------------------------------------------------------------------------------------------
+// LTI kinetic advance with K = 1 or 2 inputs (linear or log2 mapping), thread per instance.
+// ----------------------------------------------------------------------------------------------
+struct LtiInputs {
+    nw_lti_input in[2];
+};
+
+__device__ __forceinline__ void input_node(const nw_lti_input& q, int64_t i, int& b, double& f) {
+    const double x = (double)q.values[q.index ? (int64_t)q.index[i] : i];
+    const int nb = q.num_nodes - 1;
+    double u;
+    if (q.mapping == NW_INPUT_LOG2) {
+        const double lo = log2(q.minimum + q.scale), hi = log2(q.maximum + q.scale);
+        u = (log2(x + q.scale) - lo) * ((double)nb / (hi - lo));
+    } else {
+        u = (x - q.minimum) * ((double)nb / (q.maximum - q.minimum));
+    }
+    // int() of the reference truncates; NaN (a negative concentration below -scale) and values
+    // below the range take node 0 with fraction 0
+    if (!(u >= 0.0)) {
+        b = 0;
+        f = 0.0;
+    } else if (u >= (double)nb) {
+        b = nb - 1;
+        f = 1.0;
+    } else {
+        b = (int)u;
+        f = u - (double)b;
+    }
+}
+
+template <int S, int K>
+__global__ __launch_bounds__(256) void
+k_lti_nd(LtiInputs q, const float* __restrict__ table, int64_t M, float* states, int64_t stride,
+         float conserve_sum, int n_steps, const float* __restrict__ weights,
+         const float* __restrict__ gmax, float* __restrict__ cond_out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    int b0, b1 = 0;
+    double f0, f1 = 0.0;
+    input_node(q.in[0], i, b0, f0);
+    if (K == 2) input_node(q.in[1], i, b1, f1);
+    const int n1 = K == 2 ? q.in[1].num_nodes : 1;
+    // corner weights of the (bi)linear interpolation
+    const float* T00 = table + ((int64_t)b0 * n1 + b1) * S * S;
+    const float* T10 = T00 + (int64_t)n1 * S * S;      // next node of input 0
+    const float* T01 = T00 + S * S;                     // next node of input 1
+    const float* T11 = T10 + S * S;
+    const double w00 = (1.0 - f0) * (K == 2 ? 1.0 - f1 : 1.0), w10 = f0 * (K == 2 ? 1.0 - f1 : 1.0);
+    const double w01 = K == 2 ? (1.0 - f0) * f1 : 0.0, w11 = K == 2 ? f0 * f1 : 0.0;
+    double s[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) s[k] = states[k * stride + i];
+    for (int step = 0; step < n_steps; ++step) {
+        double out[S];
+#pragma unroll
+        for (int r = 0; r < S; ++r) out[r] = 0.0;
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+#pragma unroll
+            for (int r = 0; r < S; ++r) {
+                const int e = r * S + c;
+                double pe = w00 * (double)T00[e] + w10 * (double)T10[e];
+                if (K == 2) pe += w01 * (double)T01[e] + w11 * (double)T11[e];
+                out[r] = fma(pe, s[c], out[r]);
+            }
+        }
+        if (conserve_sum > 0.0f) {  // codegen.py:95-102
+            double tot = 0.0;
+#pragma unroll
+            for (int r = 0; r < S; ++r) tot += out[r];
+            const double corr = (double)conserve_sum / tot;
+#pragma unroll
+            for (int r = 0; r < S; ++r) out[r] *= corr;
+        }
+#pragma unroll
+        for (int r = 0; r < S; ++r) s[r] = out[r];
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) states[k * stride + i] = (float)s[k];
+    if (cond_out) {
+        double w = 0.0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) w = fma((double)weights[k], (double)(float)s[k], w);
+        cond_out[i] = (float)((double)gmax[i] * w);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
 // Segmented accumulation: thread per segment over its instances (sorted by segment).
 // ----------------------------------------------------------------------------------------------
 __global__ void k_accumulate(int64_t num_segments, const int64_t* __restrict__ ptr,
@@ -882,6 +970,47 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
         default: NW_REQUIRE(false, "state count %d not supported (1..16)", S);
     }
 #undef NW_LTI_CASE
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_lti_advance_nd(int32_t S, int32_t num_inputs, const nw_lti_input* inputs_host,
+                      const float* node_table, int64_t M, float* states, int64_t state_stride,
+                      float conserve_sum, int32_t n_steps, const float* weights, const float* gmax,
+                      float* cond_out, void* stream) {
+    NW_REQUIRE(inputs_host && node_table && states, "nw_lti_advance_nd: null argument");
+    NW_REQUIRE(num_inputs == 1 || num_inputs == 2, "1 or 2 inputs (got %d)", num_inputs);
+    NW_REQUIRE(M >= 0 && state_stride >= M && n_steps >= 0, "bad sizes");
+    NW_REQUIRE(!cond_out || (weights && gmax), "conductance output needs weights and gmax");
+    LtiInputs q{};
+    for (int k = 0; k < num_inputs; ++k) {
+        const nw_lti_input& in = inputs_host[k];
+        NW_REQUIRE(in.values, "input %d: null values", k);
+        NW_REQUIRE(in.num_nodes >= 2 && in.maximum > in.minimum, "input %d: bad grid", k);
+        NW_REQUIRE(in.mapping == NW_INPUT_LINEAR ||
+                       (in.mapping == NW_INPUT_LOG2 && in.scale > 0.0 && in.minimum + in.scale > 0.0),
+                   "input %d: bad mapping", k);
+        q.in[k] = in;
+    }
+    if (M == 0 || n_steps == 0) return NW_OK;
+    const unsigned grid = (unsigned)((M + 255) / 256);
+    hipStream_t s = (hipStream_t)stream;
+#define NW_LTI_ND(K_)                                                                              \
+    case K_:                                                                                       \
+        if (num_inputs == 1)                                                                       \
+            k_lti_nd<K_, 1><<<grid, 256, 0, s>>>(q, node_table, M, states, state_stride,           \
+                                                 conserve_sum, n_steps, weights, gmax, cond_out); \
+        else                                                                                       \
+            k_lti_nd<K_, 2><<<grid, 256, 0, s>>>(q, node_table, M, states, state_stride,           \
+                                                 conserve_sum, n_steps, weights, gmax, cond_out); \
+        break;
+    switch (S) {
+        NW_LTI_ND(1) NW_LTI_ND(2) NW_LTI_ND(3) NW_LTI_ND(4) NW_LTI_ND(5) NW_LTI_ND(6)
+        NW_LTI_ND(7) NW_LTI_ND(8) NW_LTI_ND(9) NW_LTI_ND(10) NW_LTI_ND(11) NW_LTI_ND(12)
+        NW_LTI_ND(13) NW_LTI_ND(14) NW_LTI_ND(15) NW_LTI_ND(16)
+        default: NW_REQUIRE(false, "state count %d not supported (1..16)", S);
+    }
+#undef NW_LTI_ND
     NW_HIP(hipGetLastError());
     return NW_OK;
 }

@@ -24,6 +24,7 @@ no CPU execution path.
 """
 import collections
 import contextlib
+import ctypes
 import math
 import os
 
@@ -772,14 +773,21 @@ def _make_hh_mechanism(model, name, gate_mode, gbar=None, reversal=None):
 
 class LTIMechanism:
     """Factory of a kinetic mechanism from an ``channels.LTIScheme`` whose conductance is
-    gmax * magnitude * sum_s weights[s] state_s with reversal potential ``reversal`` (mV); gmax is a
-    density in S/cm^2 (scaled by the segment's surface area, nmodl/__init__.py:97-107, 499)."""
+    gmax * magnitude * sum_s weights[s] state_s with reversal potential ``reversal`` (mV).  For a
+    density mechanism gmax is in S/cm^2 and the magnitude is scaled by the segment's surface area
+    (nmodl/__init__.py:97-107, 499); for a point process (point_process=True) gmax * weights are
+    Siemens and the magnitude is the inserted one (nmodl/__init__.py:494-497).  Inputs of the
+    scheme other than the voltage ('v') are attributes of each instance, initially the input's
+    minimum (lti_sim/inputs.py:10-21)."""
 
-    def __init__(self, scheme, gmax_s_per_cm2, reversal, name=None):
+    def __init__(self, scheme, gmax_s_per_cm2, reversal, name=None, point_process=False):
         self.scheme = scheme
-        self.gmax = float(gmax_s_per_cm2) * channels.CM2_TO_UM2
+        self.point_process = bool(point_process)
+        self.gmax = float(gmax_s_per_cm2) * (1.0 if self.point_process else channels.CM2_TO_UM2)
         self.reversal = float(reversal)
         self.name = name or scheme.name
+        self.input_names = list(getattr(scheme, "input_names", ["v"]))
+        self.inputs = list(getattr(scheme, "inputs", [scheme.grid]))
 
     def initialize(self, model):
         db = model.database
@@ -787,17 +795,23 @@ class LTIMechanism:
         cls_db.add_attribute("segment", NULL, dtype=np.int64, device=False,
                              on_write=model._invalidate_mechanisms)
         cls_db.add_attribute("magnitude", 1.0, valid_range=(0.0, None))
-        init = self.scheme.initial_state(model.initial_voltage)
+        start = [model.initial_voltage if n == "v" else g.minimum
+                 for n, g in zip(self.input_names, self.inputs)]
+        for n, x0 in zip(self.input_names, start):
+            if n != "v":
+                cls_db.add_attribute(n, float(x0))
+        init = self.scheme.initial_state(*start)
         for s, x0 in zip(self.scheme.state_names, init):
             cls_db.add_attribute(s, float(x0))
         spec = self
+        point = self.point_process
 
         def __init__(self, segment, outside=None, magnitude=1.0, *others):
             cid = type(self)._db_class.new_row()
             object.__setattr__(self, "_cid", cid)
             model._invalidate_mechanisms()
             self.segment = segment._cid
-            self.magnitude = float(magnitude) * segment.surface_area
+            self.magnitude = float(magnitude) * (1.0 if point else segment.surface_area)
 
         mech = type(self.name, (_DeviceMechanism,), {"__slots__": (), "__init__": __init__,
                                                     "_name": self.name, "_db_class": cls_db,
@@ -1130,8 +1144,8 @@ class RxD_Model:
             if key in _BUILTIN:
                 mech = _BUILTIN[key](self)
             elif m.endswith(".mod") and os.path.exists(m):
-                # a kinetic scheme file (nmodl_library/kinetic/*.mod): parsed, tabulated on the
-                # voltage grid, advanced by nw_lti_advance (mechanisms.py:63-66)
+                # a kinetic scheme file (nmodl_library/kinetic/*.mod): parsed, tabulated on its
+                # input grids, advanced by nw_lti_advance / nw_lti_advance_nd (mechanisms.py:63-66)
                 from . import nmodl
                 rev = {name: kw["reversal_potential"] for name, kw in self.species.items()
                        if kw.get("reversal_potential") is not None}
@@ -1333,12 +1347,31 @@ class RxD_Model:
         else:
             spec = mech.spec
             spec.scheme.time_step = self.time_step
-            mech._node_table = torch.as_tensor(spec.scheme.node_table().astype(np.float32), device=dev)
+            mech._node_table = torch.as_tensor(
+                np.ascontiguousarray(spec.scheme.node_table(), dtype=np.float32), device=dev)
             mech._weights = torch.as_tensor(np.asarray(spec.scheme.weights, dtype=np.float32), device=dev)
             S = spec.scheme.num_states
             mech._states = torch.stack([mc.columns[s].dev for s in spec.scheme.state_names]).contiguous()
             for k, s in enumerate(spec.scheme.state_names):
                 mc.columns[s].dev = mech._states[k]
+            # inputs other than the voltage alone: the general table kernel, inputs described once
+            # per build (the arrays do not move until the next build)
+            mech._nd_inputs = None
+            if spec.input_names != ["v"]:
+                arr = (_lib.NwLtiInput * len(spec.inputs))()
+                for k, (n, g) in enumerate(zip(spec.input_names, spec.inputs)):
+                    if n == "v":
+                        arr[k].values = self._col("Segment", "voltage").data_ptr()
+                        arr[k].index = mech._seg_dev.data_ptr()
+                    else:
+                        arr[k].values = mc.columns[n].dev.data_ptr()
+                        arr[k].index = None
+                    log = isinstance(g, channels.LogarithmicInput)
+                    arr[k].mapping = _lib.NW_INPUT_LOG2 if log else _lib.NW_INPUT_LINEAR
+                    arr[k].num_nodes = g.num_buckets + 1
+                    arr[k].minimum, arr[k].maximum = g.minimum, g.maximum
+                    arr[k].scale = g.scale if log else 0.0
+                mech._nd_inputs = arr
 
     # ------------------------------------------------------------------ kernels
     def _stream(self):
@@ -1381,6 +1414,12 @@ class RxD_Model:
         spec = mech.spec
         P = _lib.ptr
         out = self._gbuf[mech._g_offset:mech._g_offset + M]
+        if mech._nd_inputs is not None:
+            _lib.check(_lib.load().nw_lti_advance_nd(
+                spec.scheme.num_states, len(mech._nd_inputs), ctypes.addressof(mech._nd_inputs),
+                P(mech._node_table), M, P(mech._states), M, spec.scheme.conserve_sum or 0.0, 1,
+                P(mech._weights), P(mech._gmax_dev), P(out), self._stream()))
+            return
         grid = spec.scheme.grid
         _lib.check(_lib.load().nw_lti_advance(
             spec.scheme.num_states, grid.num_buckets + 1, grid.minimum, grid.maximum,

@@ -520,6 +520,40 @@ class KineticModel:
         g.update({k: float(v) for k, v in inputs.items()})   # inputs win over INITIAL
         return g, self._ns[fn](*args)
 
+    def rate_matrices(self, *arrays):
+        """Rate matrices [..., S, S] at arrays of input values (sorted input order, broadcast
+        together): the KINETIC block run once per state on whole arrays, numpy functions in place
+        of math.  Blocks whose control flow depends on an input fall back to one evaluation per
+        point."""
+        arrays = np.broadcast_arrays(*[np.asarray(a, dtype=np.float64) for a in arrays])
+        S = self.num_states
+        try:
+            ns = self._vector_ns
+        except AttributeError:
+            ns = self._vector_ns = {"_m": np, "_pow": np.power}
+            exec(compile(self.source, self.path, "exec"), ns)
+        try:
+            with np.errstate(all="ignore"):
+                g = self._globals({})
+                g.update(dict(zip(self.inputs, arrays)))
+                ns["_g"] = g
+                ns["_initial"]()
+                g.update(dict(zip(self.inputs, arrays)))
+                A = np.zeros(arrays[0].shape + (S, S))
+                for j in range(S):
+                    d = ns["_kinetic"]([float(k == j) for k in range(S)])
+                    for i in range(S):
+                        A[..., i, j] = d[i]
+            return A
+        except (TypeError, ValueError):
+            flat = [a.ravel() for a in arrays]
+            A = np.empty((flat[0].size, S, S))
+            for k in range(flat[0].size):
+                x = [f[k] for f in flat]
+                for j in range(S):
+                    A[k, :, j] = self.derivative(*x, *[float(q == j) for q in range(S)])
+            return A.reshape(arrays[0].shape + (S, S))
+
     def derivative(self, *values):
         """d(states)/dt (1/ms) at inputs (sorted order) and states (sorted order)."""
         inputs = dict(zip(self.inputs, values[:self.num_inputs]))
@@ -527,19 +561,68 @@ class KineticModel:
         _, d = self._run("_kinetic", inputs, states)
         return d
 
-    def conductance_weights(self, reversal_name, **inputs):
-        """Conductance of each state (S/cm^2 per unit occupancy) from the BREAKPOINT current:
-        the current written (e.g. ina = g (v - ena)) is evaluated with each state set to one and
-        the voltage one millivolt from the reversal potential."""
+    def conductance_weights(self, reversal_name=None, **inputs):
+        """Conductance of each state per unit occupancy from BREAKPOINT, in the file's units
+        (S/cm^2 for a density mechanism, uS for a point process).  A written conductance
+        (e.g. gzero = ...) is read directly; a written current (ina = g (v - ena), i = g (v - Erev))
+        is evaluated with each state set to one at v = 0 and v = 1 mV, the ion's reversal
+        potential (reversal_name) at 0: the weight is the difference."""
         w = np.empty(self.num_states)
         for k, s in enumerate(self.states):
             vals = dict(inputs)
             vals.update({n: float(n == s) for n in self.states})
-            vals[reversal_name] = 0.0
-            vals["v"] = 1.0
-            g, _ = self._run("_breakpoint", vals)
-            w[k] = g[self.current_variable]
+            if reversal_name:
+                vals[reversal_name] = 0.0
+            if self.output_kind == "conductance":
+                g, _ = self._run("_breakpoint", vals)
+                w[k] = g[self.output_variable]
+                continue
+            out = []
+            for v in (0.0, 1.0):
+                vals["v"] = v
+                g, _ = self._run("_breakpoint", vals)
+                out.append(g[self.output_variable])
+            w[k] = out[1] - out[0]
         return w
+
+    def current_reversal(self, **inputs):
+        """The reversal potential a written current implies with the file's own parameters
+        (i(v) = w (v - E): E = -i(0) / w), for currents of no declared species (Erev of a
+        NONSPECIFIC_CURRENT).  None when the file writes a conductance."""
+        if self.output_kind == "conductance":
+            return None
+        k = int(np.argmax(self.conductance_weights(None, **inputs)))
+        vals = dict(inputs)
+        vals.update({n: float(n == self.states[k]) for n in self.states})
+        out = []
+        for v in (0.0, 1.0):
+            vals["v"] = v
+            g, _ = self._run("_breakpoint", vals)
+            out.append(g[self.output_variable])
+        return -out[0] / (out[1] - out[0])
+
+    @property
+    def output_variable(self):
+        """What BREAKPOINT writes: the ion current, else the nonspecific current, else the ion
+        conductance (g<ion>)."""
+        try:
+            return self.current_variable
+        except NmodlError:
+            for ion, rw in self.ions.items():
+                if f"g{ion}" in rw["write"]:
+                    return f"g{ion}"
+            raise NmodlError(f"{self.name} writes neither a current nor a conductance")
+
+    @property
+    def output_kind(self):
+        return "conductance" if self.output_variable.startswith("g") else "current"
+
+    @property
+    def output_ion(self):
+        var = self.output_variable
+        if var.startswith("g"):
+            return var[1:]
+        return self.current_ion
 
     @property
     def current_variable(self):
@@ -559,33 +642,67 @@ class KineticModel:
         return None
 
 
-def load_kinetic(path, time_step=0.1, celsius=37.0, grid=None):
-    """The .mod file's kinetic scheme as an LTIScheme of the membrane voltage (one input)."""
+# Input tables (nmodl/__init__.py:306-316 maps the voltage linearly over [-100, 100] mV and any
+# other input logarithmically over [0, 1000]).  The reference's optimizer searches bucket counts
+# and the log offset per file by timing (optimizer.py:85-263); these are fixed choices, sized for
+# linear interpolation errors of a few 1e-6 per step (tests/test_nmodl.py measures them).
+LOG_SCALE = 1e-4
+BUCKETS_1D = {"v": 2000, "log": 2400}
+BUCKETS_2D = {"v": 400, "log": 600}
+
+
+def input_grid(name, num_inputs=1):
+    """The bucket mapping of one input of a kinetic scheme."""
+    buckets = BUCKETS_1D if num_inputs == 1 else BUCKETS_2D
+    if name == "v":
+        return channels.LinearInput(-100.0, 100.0, buckets["v"])
+    return channels.LogarithmicInput(0.0, 1000.0, buckets["log"], LOG_SCALE)
+
+
+def load_kinetic(path, time_step=0.1, celsius=37.0, grid=None, inputs=None):
+    """The .mod file's kinetic scheme as an LTIScheme of its inputs (one or two: the membrane
+    voltage and/or a concentration, sorted by name as the reference sorts them).  ``grid``
+    replaces the voltage grid of a voltage-only scheme; ``inputs`` gives every grid."""
     km = KineticModel(path, celsius=celsius)
-    if km.inputs != ["v"]:
-        raise NmodlError(f"{km.name}: inputs {km.inputs}; the device kernel takes the voltage only")
+    if not 1 <= km.num_inputs <= 2:
+        raise NmodlError(f"{km.name}: inputs {km.inputs}; the device kernel takes one or two")
+    if inputs is None:
+        if grid is not None and km.inputs == ["v"]:
+            inputs = [grid]
+        else:
+            inputs = [input_grid(n, km.num_inputs) for n in km.inputs]
     scheme = channels.LTIScheme(km.name, km.states, km.derivative, km.conserve_sum, time_step,
-                                grid=grid)
+                                inputs=inputs, input_names=km.inputs,
+                                rate_matrices=km.rate_matrices)
     scheme.model = km
     return scheme
 
 
 def kinetic_mechanism(path, time_step, celsius, reversal_potentials, grid=None):
-    """A model mechanism (``model.LTIMechanism``) from a voltage-gated kinetic .mod file: the
-    conductance of every state from BREAKPOINT, the reversal potential of the written ion from
-    the model's species (or the file's PARAMETER default)."""
+    """A model mechanism (``model.LTIMechanism``) from a kinetic .mod file: the conductance of
+    every state from BREAKPOINT, the reversal potential of the written ion from the model's
+    species (or the file's PARAMETER default, or the reversal its nonspecific current implies).
+    A POINT_PROCESS's conductance is in uS per unit occupancy and its magnitude is not scaled by
+    the segment's area (nmodl/__init__.py:494-497); inputs other than the voltage are
+    attributes of each instance."""
     from .model import LTIMechanism
     scheme = load_kinetic(path, time_step, celsius, grid)
     km = scheme.model
-    if km.point_process:
-        raise NmodlError(f"{km.name} is a POINT_PROCESS; only density mechanisms are supported")
-    ion = km.current_ion
+    ion = km.output_ion
     e_name = f"e{ion}" if ion else None
     if ion is not None and ion in reversal_potentials:
         E = float(reversal_potentials[ion])
     elif e_name and km.parameters.get(e_name) is not None:
         E = float(km.parameters[e_name])
+    elif ion is None and km.current_reversal() is not None:
+        E = float(km.current_reversal())
+    elif km.parameters.get("Erev") is not None:
+        E = float(km.parameters["Erev"])
     else:
         raise NmodlError(f"{km.name}: no reversal potential for ion {ion!r} (declare the species)")
-    scheme.weights = km.conductance_weights(e_name or "e")
+    w = km.conductance_weights(e_name)
+    if km.point_process:
+        scheme.weights = w * 1e-6                           # uS -> S
+        return LTIMechanism(scheme, 1.0, E, name=km.name, point_process=True)
+    scheme.weights = w
     return LTIMechanism(scheme, 1.0, E, name=km.name)

@@ -13,6 +13,10 @@ Pinning (what each function is checked against, see ``tests/test_oracle.py``):
                                    c^T P = c^T, RC relaxation 1-1/e, injected charge dQ = I.t
   * half_step                   -- RC closed form, charge injection, length-constant attenuation
   * hh gates (exact)            -- HH golden trace (neuwon/rxd/tests/test_model.py:77-113)
+  * input_position              -- bucket known answers of lti_sim/tests/test_inputs.py:5-68
+                                   (linear and logarithmic mappings, ends, inverse round trips)
+  * lti_advance_nodes           -- equals lti_advance_vec on voltage tables; at grid nodes the
+                                   step is scipy.linalg.expm exactly; interpolation error bounds
   * LTI tables / kernel         -- 2-state closed-form expm (exact), scipy.linalg.expm, column sums,
                                    bucket-mapping known answers (lti_sim/tests/test_inputs.py:5-35)
   * accumulation                -- np.add.at brute force
@@ -307,6 +311,54 @@ def lti_advance_vec(states, v, poly_table, minimum=-100.0, maximum=100.0, conser
     return scratch
 
 
+def input_position(x, grid):
+    """Fractional node position of input values x on a grid (dict: kind "linear" or "log",
+    minimum, maximum, num_buckets, scale), with the reference's truncating bucket lookup and
+    clamping (codegen.py:42-59): linear u = (x - min) * nb / range (inputs.py:72-84); log
+    u = (log2(x + scale) - log2(min + scale)) * nb / log2range (inputs.py:86-109).  Returns
+    (bucket, fraction) arrays; values below the range (and NaN) take (0, 0)."""
+    x = np.asarray(x, dtype=np.float64)
+    nb = int(grid["num_buckets"])
+    if grid["kind"] == "log":
+        lo = np.log2(grid["minimum"] + grid["scale"])
+        hi = np.log2(grid["maximum"] + grid["scale"])
+        with np.errstate(invalid="ignore", divide="ignore"):
+            u = (np.log2(x + grid["scale"]) - lo) * (nb / (hi - lo))
+    else:
+        u = (x - grid["minimum"]) * (nb / (grid["maximum"] - grid["minimum"]))
+    below = ~(u >= 0.0)
+    above = u >= nb
+    b = np.where(below | above, 0, np.trunc(np.where(below, 0.0, u))).astype(np.int64)
+    f = np.where(below | above, 0.0, u - b)
+    b = np.where(above, nb - 1, b)
+    f = np.where(above, 1.0, f)
+    return b, f
+
+
+def lti_advance_nodes(states, xs, table, grids, conserve_sum=None):
+    """One step of a kinetic scheme whose propagators are tabulated at grid nodes of one or two
+    inputs: table [n0](x [n1])[row][col]; the propagator of each instance is interpolated
+    linearly (one input) or bilinearly (two) between the surrounding nodes, then applied
+    column by column as codegen.py:81-102 does (CONSERVE renormalisation included).
+    states [S][M]; xs: list of input arrays [M]."""
+    T = np.asarray(table, dtype=np.float64)
+    S, M = states.shape
+    pos = [input_position(x, g) for x, g in zip(xs, grids)]
+    if len(pos) == 1:
+        (b0, f0), = pos
+        P = (1.0 - f0)[:, None, None] * T[b0] + f0[:, None, None] * T[b0 + 1]
+    else:
+        (b0, f0), (b1, f1) = pos
+        P = (((1.0 - f0) * (1.0 - f1))[:, None, None] * T[b0, b1] +
+             (f0 * (1.0 - f1))[:, None, None] * T[b0 + 1, b1] +
+             ((1.0 - f0) * f1)[:, None, None] * T[b0, b1 + 1] +
+             (f0 * f1)[:, None, None] * T[b0 + 1, b1 + 1])
+    out = np.einsum("mrc,cm->rm", P, states)
+    if conserve_sum is not None:
+        out *= conserve_sum / out.sum(axis=0)
+    return out
+
+
 def two_state_derivative(rate_fn):
     """Derivative of the scheme ``~ C <-> O (alpha, beta)`` with state order (C, O)."""
     def deriv(v, C, O):
@@ -473,7 +525,9 @@ class KineticSim:
 
     mechanisms: list of dicts with poly (polynomial table of node_table_to_polynomial),
     weights [S], gmax [n] (Siemens per unit of sum_s weights[s] state_s), E (mV), states [S, n],
-    conserve (CONSERVE total or None).  stim as in HHSim."""
+    conserve (CONSERVE total or None).  A mechanism driven by other inputs instead holds table
+    (node table of lti_advance_nodes), grids, and inputs: a function (sim) -> list of input
+    arrays [n] in the table's input order (the voltage is sim.v).  stim as in HHSim."""
 
     def __init__(self, P, C, mechanisms, v0=-70.0, dt=0.1, stim=None):
         self.P, self.C = P, np.asarray(C, dtype=np.float64)
@@ -497,7 +551,12 @@ class KineticSim:
         self.sum_g = np.zeros(self.n)
         self.d_num = np.zeros(self.n)
         for m in self.mech:
-            m["states"] = lti_advance_vec(m["states"], self.v, m["poly"], conserve_sum=m["conserve"])
+            if "table" in m:
+                m["states"] = lti_advance_nodes(m["states"], m["inputs"](self), m["table"],
+                                                m["grids"], m["conserve"])
+            else:
+                m["states"] = lti_advance_vec(m["states"], self.v, m["poly"],
+                                              conserve_sum=m["conserve"])
             g = m["gmax"] * (np.asarray(m["weights"]) @ m["states"])
             self.sum_g += g
             self.d_num += g * m["E"]

@@ -73,8 +73,11 @@ def test_reference_nmda_inputs():
     assert init.pop("UMg") == pytest.approx(1)
     for value in init.values():
         assert value == pytest.approx(0, abs=1e-6)
-    with pytest.raises(nmodl.NmodlError):        # the device kernel takes one voltage input
-        nmodl.load_kinetic(os.path.join(LTI_TESTS, "NMDA.mod"))
+    # two inputs: a logarithmic concentration grid and a linear voltage grid (bilinear tables)
+    sc = nmodl.load_kinetic(os.path.join(LTI_TESTS, "NMDA.mod"), time_step=0.1,
+                            inputs=[channels.LogarithmicInput(0, 1000, 8, 1e-4),
+                                    channels.LinearInput(-100, 100, 8)])
+    assert sc.input_names == ["C", "v"] and sc.node_table().shape == (9, 9, 10, 10)
 
 
 @needs_reference
@@ -202,3 +205,99 @@ NET_RECEIVE (w) { }
     g.write_text("NEURON { SUFFIX d }\nSTATE { x }\nDERIVATIVE s { x' = -x }\n")
     with pytest.raises(nmodl.NmodlError):
         nmodl.KineticModel(str(g))
+
+
+@needs_reference
+def test_ampa_library_rates_by_hand():
+    """nmodl_library/kinetic/AMPA.mod at 37 degC: the transmitter-gated generator restated from
+    the file's reactions and Q10 factors; conductance weights of the four open states (uS)."""
+    km = nmodl.KineticModel(os.path.join(LIBRARY, "AMPA.mod"), celsius=37.0)
+    assert km.inputs == ["gluo"] and km.point_process and km.num_states == 13
+    q = 2.4 ** ((37.0 - 22.0) / 10.0)
+    Rb, Ru = [800, 600, 400, 200], [30, 40, 60, 80]
+    Rd, Rr = [.25, .25, 1, 1], [0.05, 0.05, 0.022, 0.022]
+    Ro, Rc = [3, 4, 4, 4], [1.5, 1, 1, 1.5]
+    idx = {s: i for i, s in enumerate(km.states)}
+    for glu in (0.0, 3e-4, 0.3, 10.0):
+        A = np.zeros((13, 13))
+
+        def add(a, b, kf, kr):
+            A[idx[b], idx[a]] += kf
+            A[idx[a], idx[a]] -= kf
+            A[idx[a], idx[b]] += kr
+            A[idx[b], idx[b]] -= kr
+        for k in range(4):
+            add(f"C{k}", f"C{k + 1}", Rb[k] * glu * q, Ru[k] * q)
+            add(f"C{k + 1}", f"D{k + 1}", Rd[k] * q, Rr[k] * q)
+            add(f"C{k + 1}", f"O{k + 1}", Ro[k] * q, Rc[k] * q)
+        np.testing.assert_allclose(_rate_matrix(km, glu), A, rtol=1e-13, atol=1e-13)
+    w = km.conductance_weights(None)
+    want = {"O1": 0.25, "O2": 0.5, "O3": 0.75, "O4": 1.0}
+    for s, x in zip(km.states, w):
+        assert x == pytest.approx(1e-6 * 20 * want.get(s, 0.0), abs=1e-18)
+    assert km.output_variable == "gzero" and km.output_ion == "zero"
+
+
+@needs_reference
+def test_library_transmitter_schemes_load():
+    """AMPA (one concentration input) and NMDA (concentration and voltage) tabulate on the
+    logarithmic / linear grids; the vectorised rate matrices equal the scalar ones."""
+    from neuwon_amd import channels
+    for f, inputs in (("AMPA.mod", ["gluo"]), ("NMDA.mod", ["C", "v"])):
+        grids = [channels.LinearInput(-100, 100, 20) if n == "v" else
+                 channels.LogarithmicInput(0, 1000, 30, 1e-4) for n in inputs]
+        sc = nmodl.load_kinetic(os.path.join(LIBRARY, f), time_step=0.025, celsius=37.0,
+                                inputs=grids)
+        assert sc.input_names == inputs
+        T = sc.node_table()
+        assert T.shape == tuple(g.num_buckets + 1 for g in grids) + (sc.num_states,) * 2
+        assert np.abs(T.sum(axis=-2) - 1.0).max() < 1e-12
+        km = sc.model
+        xs = [np.array([-30.0, 10.0]) if n == "v" else np.array([1e-3, 0.5]) for n in inputs]
+        A = km.rate_matrices(*xs)
+        for k in range(2):
+            np.testing.assert_allclose(A[k], _rate_matrix(km, *[x[k] for x in xs]), rtol=1e-14,
+                                       atol=1e-14)
+
+
+def test_own_receptor_files():
+    """tests/golden/kinetic/glu5.mod (transmitter read from outside) and mgblock4.mod (a POINTER
+    concentration and the voltage): inputs, generators by hand, point-process weights and the
+    reversal potentials their currents imply."""
+    glu = nmodl.KineticModel(os.path.join(OWN, "glu5.mod"), celsius=30.0)
+    assert glu.inputs == ["gluo"] and glu.point_process
+    t = 2.0 ** ((30.0 - 22.0) / 10.0)
+    idx = {s: i for i, s in enumerate(glu.states)}
+    A = np.zeros((5, 5))
+
+    def add(A, a, b, kf, kr):
+        A[idx[b], idx[a]] += kf
+        A[idx[a], idx[a]] -= kf
+        A[idx[a], idx[b]] += kr
+        A[idx[b], idx[b]] -= kr
+    c = 0.7
+    add(A, "R", "RA", 2 * 5 * c * t, 2 * t)
+    add(A, "RA", "RA2", 5 * c * t, 2 * 2 * t)
+    add(A, "RA2", "O", 6 * t, 1.2 * t)
+    add(A, "RA2", "D", 0.4 * t, 0.03 * t)
+    np.testing.assert_allclose(_rate_matrix(glu, c), A, rtol=1e-14, atol=1e-14)
+    w = glu.conductance_weights(None)
+    assert w[idx["O"]] == pytest.approx(0.002, rel=1e-14) and np.count_nonzero(w) == 1  # 2 nS
+    assert glu.current_reversal() == pytest.approx(0.0, abs=1e-12)
+    mg = nmodl.KineticModel(os.path.join(OWN, "mgblock4.mod"), celsius=30.0)
+    assert mg.inputs == ["C", "v"] and mg.point_process
+    ix = {s: i for i, s in enumerate(mg.states)}
+    for C, v in ((0.0, -70.0), (0.2, 10.0)):
+        B = np.zeros((4, 4))
+        for a, b, kf, kr in (("U", "B", 8 * C, 0.05), ("B", "O", 0.3, 0.1),
+                             ("O", "OB", 1.2 * math.exp(-v / 16), 0.4 * math.exp(v / 16))):
+            B[ix[b], ix[a]] += kf
+            B[ix[a], ix[a]] -= kf
+            B[ix[a], ix[b]] += kr
+            B[ix[b], ix[b]] -= kr
+        np.testing.assert_allclose(_rate_matrix(mg, C, v), B, rtol=1e-14, atol=1e-14)
+    assert mg.conductance_weights(None)[ix["O"]] == pytest.approx(8e-4, rel=1e-14)   # 800 pS
+    assert mg.current_reversal() == pytest.approx(3.0, rel=1e-12)
+    mech = nmodl.kinetic_mechanism(os.path.join(OWN, "glu5.mod"), 0.025, 30.0, {})
+    assert mech.point_process and mech.input_names == ["gluo"] and mech.reversal == 0.0
+    assert mech.scheme.weights[idx["O"]] == pytest.approx(2e-9, rel=1e-14)            # Siemens

@@ -525,3 +525,94 @@ def test_kinetic_sim_constant_conductance_closed_form():
         t_s = (k * dt - 0.5 * dt) * 1e-3
         want = E + (v0 - E) * np.exp(-t_s * g / C[0])
         assert sim.v[0] == pytest.approx(want, rel=1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# Input mappings and node-table kinetics (one or two inputs)
+# ---------------------------------------------------------------------------------------------
+def _grid(g):
+    from neuwon_amd import channels
+    log = isinstance(g, channels.LogarithmicInput)
+    return dict(kind="log" if log else "linear", minimum=g.minimum, maximum=g.maximum,
+                num_buckets=g.num_buckets, scale=g.scale if log else 0.0)
+
+
+def test_input_position_known_answers():
+    """lti_sim/tests/test_inputs.py:5-68 against the oracle's mapping and the product's grids."""
+    import random
+    from neuwon_amd import channels
+    lin = channels.LinearInput(4, 7, 17)
+    log = channels.LogarithmicInput(0, 2e3, 37, scale=1e-3)
+    for g, lo, hi in ((lin, 4, 7), (log, 0, 2e3)):
+        for x, want_b, want_f in ((lo, 0, 0.0), (hi - 1e-12, g.num_buckets - 1, 1.0)):
+            b, f = O.input_position(np.array([x]), _grid(g))
+            assert int(b[0]) == want_b and float(f[0]) == pytest.approx(want_f)
+            assert g.get_bucket_location(x)[0] == want_b
+            assert g.get_bucket_location(x)[1] == pytest.approx(want_f)
+    # bisect_inputs(5, 6): the midpoint in bucket space
+    mid = log.get_input_value(0.5 * (log.get_bucket_value(5) + log.get_bucket_value(6)))
+    assert 5.25 < mid < 5.49
+    assert lin.get_input_value(0.5 * (lin.get_bucket_value(5) + lin.get_bucket_value(6))) == \
+        pytest.approx(5.5)
+    # inverse round trips, small concentrations included (test_inputs.py:38-68)
+    rng = random.Random(5)
+    lin = channels.LinearInput(11, 57, 107)
+    log = channels.LogarithmicInput(0, 57, 107, scale=1e-3)
+    for g in (lin, log):
+        trials = [rng.uniform(g.minimum, g.maximum) for _ in range(100)] + [g.minimum, g.maximum]
+        if g is log:
+            trials += [7.0 ** -x for x in range(100)]
+        for x in trials:
+            u = g.get_bucket_value(x)
+            assert g.get_input_value(u) == pytest.approx(x, abs=1e-12)
+            assert 0.0 <= u <= g.num_buckets + 1e-9
+            b, f = O.input_position(np.array([x]), _grid(g))
+            assert int(b[0]) == min(int(u), g.num_buckets - 1)
+    # below the range and NaN take node 0 (the reference has no lower clamp for concentrations)
+    b, f = O.input_position(np.array([-1.0, np.nan]), _grid(log))
+    assert list(b) == [0, 0] and list(f) == [0.0, 0.0]
+
+
+def test_lti_advance_nodes_pins():
+    """Voltage-only tables: the node-table step equals the polynomial-table kernel; at grid nodes
+    the interpolated step is the exact expm of that node (one and two inputs); between nodes it
+    stays within the measured interpolation error of the exact propagator."""
+    import os
+    from neuwon_amd import channels, nmodl
+    own = os.path.join(os.path.dirname(__file__), "golden", "kinetic")
+    rng = np.random.default_rng(8)
+    # (a) voltage input
+    sc = channels.random_ampa13_scheme(seed=3, time_step=0.1)
+    T = sc.node_table()
+    M = 500
+    st = rng.dirichlet(np.ones(13), M).T
+    v = rng.uniform(-120.0, 120.0, M)
+    a = O.lti_advance_nodes(st, [v], T, [_grid(sc.grid)], 1.0)
+    b = O.lti_advance_vec(st, v, O.node_table_to_polynomial(T), conserve_sum=1.0)
+    assert np.abs(a - b).max() < 1e-14
+    # (b) nodes of two inputs (log concentration x linear voltage), small grids
+    path = os.path.join(own, "mgblock4.mod")
+    grids = [channels.LogarithmicInput(0.0, 1000.0, 60, 1e-4), channels.LinearInput(-100.0, 100.0, 40)]
+    sc2 = nmodl.load_kinetic(path, time_step=0.05, celsius=30.0, inputs=grids)
+    T2 = sc2.node_table()
+    G2 = [_grid(g) for g in grids]
+    nodes = [g.nodes() for g in grids]
+    for i, j in ((0, 0), (17, 9), (33, 21), (59, 39), (60, 40)):
+        x = (nodes[0][i], nodes[1][j])
+        P = sc2.make_matrix(*x)
+        got = O.lti_advance_nodes(np.eye(4), [np.full(4, x[0]), np.full(4, x[1])], T2, G2)
+        assert np.abs(got - P).max() < 1e-13, (i, j)
+    # (c) between nodes: the measured error of the default grids
+    for f, bound in (("glu5.mod", 5e-6), ("mgblock4.mod", 6e-5)):
+        sc3 = nmodl.load_kinetic(os.path.join(own, f), time_step=0.025, celsius=30.0)
+        T3 = sc3.node_table()
+        G3 = [_grid(g) for g in sc3.inputs]
+        worst = 0.0
+        for _ in range(300):
+            x = [rng.uniform(-90, 40) if n == "v" else 10 ** rng.uniform(-5, 1)
+                 for n in sc3.input_names]
+            P = sc3.make_matrix(*x)
+            S = sc3.num_states
+            got = O.lti_advance_nodes(np.eye(S), [np.full(S, xi) for xi in x], T3, G3)
+            worst = max(worst, np.abs(got - P).max())
+        assert worst < bound, (f, worst)
