@@ -3,7 +3,9 @@
 Workload (default): a population of independent random-tree cells, 200-5000 compartments each
 (uniform), drawn until 10M compartments; leak + HH Na/K as two-state kinetic schemes on every
 segment, per-cell gNa/gK scaled logU(0.5, 2); Poisson 5 Hz current pulses (0.5 nA x 1 ms) at every
-root; propagator pruned at 1e-9; FP32 state, FP64 propagator construction; dt = 0.1 ms.
+root; FP32 state; dt = 0.1 ms.  The propagator exp(A dt) of every cell is applied as a Chebyshev
+series of its tree operator (--format chebyshev, no stored matrix, unpruned), or stored pruned at
+1e-9 (--format symmetric / general, FP64 construction).
 
 A step is one staggered time step of every compartment.  W warm-up steps run first, then exactly K
 timed steps, bracketed by torch.cuda.synchronize().  Prints one JSON line.
@@ -24,7 +26,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per compartment-step of the step kernel on this workload, per propagator format, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs scaled by what the same counters read for a
 # plain 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); None where not measured
-MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3}
+MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3, "chebyshev": 151.3}
+# Kept entries per row of the propagator pruned at 1e-9 (pair rule), this seeded 10M workload:
+# the canonical algorithmic count of the Chebyshev store, which keeps no matrix, uses it
+NNZ_BAR_PRUNED_1E9 = 567.99
 
 
 def algorithmic_bytes(nnz_bar):
@@ -43,6 +48,8 @@ def format_bytes(fmt, nnz_bar):
     per-segment mechanism state and constants."""
     if fmt == "symmetric":
         return 2.0 * (6.0 * (nnz_bar + 1.0) / 2.0 + 20.0) + 68.0
+    if fmt == "chebyshev":   # per half step 3 neighbour indices and coefficients, FP64 v in/out
+        return 2.0 * (24.0 + 16.0) + 68.0
     return 2.0 * (6.0 * nnz_bar + 12.0) + 68.0
 
 
@@ -56,8 +63,9 @@ def parse():
     ap.add_argument("--launch-steps", type=int, default=0,
                     help="steps per kernel launch (0: all timed steps in one launch)")
     ap.add_argument("--block-threads", type=int, default=0)
-    ap.add_argument("--format", choices=("symmetric", "general"), default="symmetric",
-                    help="propagator storage: symmetric half (S = W P W^-1) or general CSR")
+    ap.add_argument("--format", choices=("symmetric", "general", "chebyshev"), default="chebyshev",
+                    help="propagator store: symmetric half (S = W P W^-1), general CSR, or "
+                         "chebyshev (no stored matrix: series of the tree operator, unpruned)")
     ap.add_argument("--cpu-sample-cells", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -72,17 +80,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def _sample_propagator(pop, sample):
-    """FP64 propagator of the first `sample` cells from the device construction path, pruned as
-    the population's store prunes it (entry rule, or the pair rule of the symmetric store)."""
+def _sample_propagator(pop, sample, rule, prune):
+    """FP64 propagator of the first `sample` cells from the device construction path, pruned at
+    `prune` by the entry rule, the pair rule of the symmetric store, or not at all ("none")."""
     import scipy.sparse
     from neuwon_amd import population
-    prune = pop.propagator.prune
     blocks = []
     for c in range(sample):
         r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
         args = (pop.parent_local[r0:r1], pop.R[r0:r1], pop.C[r0:r1], 0.5 * pop.dt)
-        if pop.propagator.symmetric:
+        if rule == "none":
+            blocks.append(population.cell_propagator_f64(*args, 0.0))
+        elif rule == "pair":
             full = population.cell_propagator_f64(*args, 0.0).toarray()
             keep = np.maximum(np.abs(full), np.abs(full.T)) >= prune
             np.fill_diagonal(keep, True)
@@ -95,24 +104,31 @@ def _sample_propagator(pop, sample):
 
 def cpu_baseline(pop, args, total_steps):
     """The oracle's C FP64 restatement of the same step (CSR SpMV, table gates, Poisson stimulus),
-    fed the same pruned FP64 propagator the device built, on a bounded sample of the population's
-    cells: one thread (the reference's serial path) and every host core.  The all-cores run covers
-    exactly the steps the GPU ran, so its voltages also give the precision of the GPU run on the
-    sample."""
+    fed the FP64 propagator the device built, pruned at 1e-9 as the store prunes it (the entry
+    rule for the general and Chebyshev stores -- the latter keeps no matrix, and the baseline is
+    the reference's operator of this workload -- the pair rule for the symmetric store), on a
+    bounded sample of the population's cells: one thread (the reference's serial path) and every
+    host core.  The all-cores run covers exactly the steps the GPU ran, so its voltages also give
+    the precision of the GPU run on the sample; for the Chebyshev store, which applies exp(A dt)
+    unpruned, a second all-cores run with the unpruned propagator on the first few cells does."""
     from oracle import neuwon_oracle as O
     cores = len(os.sched_getaffinity(0))
     sample = min(args.cpu_sample_cells, pop.num_cells)
     r_end = int(pop.cell_ptr[sample])
-    P = _sample_propagator(pop, sample)
+    rule = "pair" if pop.propagator.symmetric else "entry"
+    P = _sample_propagator(pop, sample, rule, 1e-9 if pop.propagator.chebyshev else pop.propagator.prune)
     p = pop.params
     stim = ("poisson", float(p.stim_amp_a), int(p.stim_seed), int(p.poisson_threshold),
             int(p.pulse_ticks))
-    gbar = {k: pop.g_bar[k][:r_end] for k in ("na", "k", "l")}
-    make = lambda: O.CHHPopulation(pop.cell_ptr[:sample + 1], P, pop.C[:r_end], gbar, dt=pop.dt,
-                                   celsius=pop.celsius, gate_mode="table", stim=stim)
+
+    def make(cells, P):
+        r = int(pop.cell_ptr[cells])
+        gbar = {k: pop.g_bar[k][:r] for k in ("na", "k", "l")}
+        return O.CHHPopulation(pop.cell_ptr[:cells + 1], P, pop.C[:r], gbar, dt=pop.dt,
+                               celsius=pop.celsius, gate_mode="table", stim=stim)
     rates = {}
     # one thread: a bounded run
-    co = make()
+    co = make(sample, P)
     co.advance(3, threads=1)   # warm-up
     steps, t0 = 0, time.perf_counter()
     while True:
@@ -124,7 +140,7 @@ def cpu_baseline(pop, args, total_steps):
     rates[1] = (r_end * steps / el, steps)
     # every core: exactly the GPU's steps (3 untimed warm-up steps first), recording every
     # cell's root voltage after each step
-    co = make()
+    co = make(sample, P)
     roots = pop.cell_ptr[:sample]
     trace = np.empty((sample, total_steps))
     warm = min(3, total_steps - 1)
@@ -144,7 +160,16 @@ def cpu_baseline(pop, args, total_steps):
             "sample": f"first {sample} cells ({r_end} compartments, {P.nnz} FP64 propagator entries) "
                       f"of the same population; {rates[1][1]} steps on 1 thread, {rates[cores][1]} "
                       f"steps on {cores} threads; oracle C FP64 restatement (OpenMP over cells)"}
-    return base, co.v.copy(), trace, r_end
+    if not pop.propagator.chebyshev:
+        return base, co.v.copy(), trace, r_end
+    few = min(6, sample)
+    P0 = _sample_propagator(pop, few, "none", 0.0)
+    co = make(few, P0)
+    trace = np.empty((few, total_steps))
+    for k in range(total_steps):
+        co.advance(1, threads=cores)
+        trace[:, k] = co.v[pop.cell_ptr[:few]]
+    return base, co.v.copy(), trace, int(pop.cell_ptr[few])
 
 
 def _trace_error(v, ref, dt, window_ms=2.0, threshold=20.0):
@@ -177,7 +202,7 @@ def precision(pop, cells, args, v_ref, trace_ref, r_end, total_steps):
     sub = population.HHPopulation(cells[:sample], dt=pop.dt, celsius=pop.celsius,
                                   prune=pop.propagator.prune, gate_mode="table",
                                   stim=pop.stim, block_threads=args.block_threads,
-                                  symmetric=pop.propagator.symmetric)
+                                  store=pop.propagator.store)
     tr = torch.zeros((sample, total_steps), dtype=torch.float32, device=sub.dev)
     sub.advance(total_steps, trace=tr)
     rerun = float(np.abs(sub.v.cpu().numpy().astype(np.float64) - v_gpu).max())
@@ -191,7 +216,8 @@ def precision(pop, cells, args, v_ref, trace_ref, r_end, total_steps):
             "trace_median_rel_v_error": float(np.median([e[3] for e in errs])),
             "rerun_max_abs_diff_mV": rerun,
             "sample": f"first {sample} cells, {r_end} compartments, {total_steps} steps",
-            "definition": "FP64 oracle with the same pruned propagator, tables and stimulus; "
+            "definition": "FP64 oracle with the store's propagator (pruned as stored; unpruned for "
+                          "the Chebyshev store), tables and stimulus; "
                           "state: every compartment after all steps; trace: root voltage of every "
                           "sampled cell after every step, subthreshold = more than 2 ms from a "
                           "spike of the oracle trace"}
@@ -262,10 +288,9 @@ def main():
     pop = population.HHPopulation(cells, dt=0.1, celsius=6.3, prune=1e-9, gate_mode="table",
                                   stim={"mode": "poisson", "amp_na": 0.5, "rate_hz": 5.0,
                                         "duration_ms": 1.0, "seed": args.seed},
-                                  block_threads=args.block_threads,
-                                  symmetric=args.format == "symmetric")
+                                  block_threads=args.block_threads, store=args.format)
     st = pop.propagator.stats
-    nnz_bar = pop.propagator.mean_nnz_per_row
+    nnz_bar = NNZ_BAR_PRUNED_1E9 if pop.propagator.chebyshev else pop.propagator.mean_nnz_per_row
     log(f"population ready in {time.time() - t0:.1f}s: {pop.num_segments} compartments, "
         f"nnz/row {nnz_bar:.1f}, propagator build {st['build_seconds']:.1f}s, "
         f"{st['device_bytes'] / 1e9:.2f} GB on device")
@@ -290,7 +315,8 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     kern_ms = sum(a.elapsed_time(b) for _, a, b in ev)
-    launches = len(ev)
+    # kernel launches: one per advance() call, two per step on the Chebyshev store
+    launches = 2 * K if pop.propagator.chebyshev else len(ev)
     value = N * K / wall
     b_step = algorithmic_bytes(nnz_bar)
     avg_launch_s = kern_ms / 1e3 / launches
@@ -313,20 +339,34 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded random-tree cells, Poisson stimulus)",
-        "config": {"workload": "batched HH population, FP32 state, FP64-built propagator pruned 1e-9",
+        "config": {"workload": ("batched HH population, FP32 state, exp(A dt) applied as a Chebyshev "
+                                "series of each cell's tree operator (no stored matrix, unpruned)"
+                                if pop.propagator.chebyshev else
+                                "batched HH population, FP32 state, FP64-built propagator pruned 1e-9"),
                    "propagator_format": args.format, "compartments": N, "cells": pop.num_cells, "dt_ms": 0.1,
                    "nnz_per_row": round(nnz_bar, 2), "steps_per_launch": L,
                    "propagator_build_s": round(st["build_seconds"], 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic * N * (K / launches),
-                     "kernel": "nw::k_hh_step", "avg_launch_ms": avg_launch_s * 1e3,
+                     "kernel": ("nw::k_hh_half_cheb (two launches per step)"
+                                if pop.propagator.chebyshev else "nw::k_hh_step"),
+                     "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_cmpt_step": b_step,
                      "format_bytes_per_cmpt_step": f_step,
                      "format_GBs": f_step * units_per_s / 1e9,
                      "stored_propagator_bytes_per_cmpt_step": stored_bytes_step,
                      "measured_traffic_GBs": None if traffic is None else traffic * units_per_s / 1e9},
     }
+    if pop.propagator.chebyshev:
+        # the series runs out of LDS: per row and term the current and previous terms, the
+        # neighbours' terms (about two) and the new term, 8 B each
+        terms = st["mean_terms"]
+        lds_b = 2.0 * terms * 8.0 * (3.0 + 2.0 * (N - pop.num_cells) / N)
+        out["roofline"].update({"lds_bytes_per_cmpt_step": lds_b,
+                                "lds_GBs": lds_b * N * K / (kern_ms / 1e3) / 1e9,
+                                "lds_peak_GBs": 256 * 128 * 2.4,
+                                "series_terms_per_row": terms})
     log(f"timed {K} steps: {wall:.3f}s wall, kernel {kern_ms:.1f} ms, {value / 1e9:.3f} G cmpt-steps/s")
     if not args.no_cpu:
         out["cpu_baseline"], v_ref, trace_ref, r_end = cpu_baseline(pop, args, W + K)

@@ -48,6 +48,7 @@ int nw_device_info(int device, char* name_out, int name_len, int* num_cus, int* 
 typedef struct nw_propagator nw_propagator;
 
 #define NW_PROPAGATOR_SYMMETRIC 1
+#define NW_PROPAGATOR_CHEBYSHEV 2
 
 typedef struct nw_propagator_stats {
     int64_t num_cells;
@@ -59,7 +60,8 @@ typedef struct nw_propagator_stats {
     double  build_seconds;
     int64_t device_bytes;    /* bytes held on the device by the handle                            */
     int32_t symmetric;       /* 1: symmetric format (see nw_propagator_build_ex)                  */
-    int32_t pad_;
+    int32_t chebyshev;       /* 1: no stored matrix, a Chebyshev series of the tree operator      */
+    double  mean_terms;      /* Chebyshev store: series terms per row and product (row-weighted)  */
 } nw_propagator_stats;
 
 int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_host,
@@ -74,7 +76,27 @@ int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_pt
                            const int32_t* parent_local_host, const double* R_host,
                            const double* C_host, double time_step_ms, double prune, int32_t flags,
                            nw_propagator** out);
+/* flags = NW_PROPAGATOR_CHEBYSHEV stores no matrix: P x = exp(B) x with B = A dt (the tree
+ * operator, 2(n-1) edge coefficients per cell) is evaluated as the Chebyshev series
+ * sum_k c_k T_k(2 B / rho + I) x on the spectrum bound rho = 2 max_i |B_ii| (Gershgorin), with
+ * c_0 = e^(-rho/2) I_0(rho/2), c_k = 2 e^(-rho/2) I_k(rho/2), truncated where the remaining
+ * coefficients fall below 1e-10 (nw_chebyshev_exp_coefficients).  Nothing is pruned: the
+ * product is the reference's exp(A dt) x at its default threshold (electric.py:114-115, machine
+ * epsilon) to about 1e-10 of the input's magnitude.  Each row keeps up to 3 neighbours (more
+ * in an overflow list); cells up to 5120 segments.  Read by nw_hh_step and
+ * nw_electric_half_step; nw_hh_adjoint and nw_ca_step refuse it. */
 int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* out);
+/* Chebyshev store: the spectrum bound rho and series coefficients of one cell
+ * (coef_host may be NULL: only *K_out, the index of the last kept coefficient, and *rho_out). */
+int nw_propagator_cell_chebyshev(const nw_propagator* p, int64_t cell, int32_t* K_out,
+                                 double* rho_out, double* coef_host);
+/* Host routine: coefficients of exp((x/2)(t - 1)) = sum_k c_k T_k(t) for x = rho >= 0, i.e.
+ * c_0 = e^(-x/2) I_0(x/2), c_k = 2 e^(-x/2) I_k(x/2) (modified Bessel functions by Miller's
+ * backward recurrence, normalised by e^y = I_0(y) + 2 sum_k I_k(y)), up to the last k with
+ * c_k > tol; the sum of the dropped ones is added to c_0, so sum_k c_k = 1 (a constant vector is
+ * kept exactly).  Writes at most `capacity` values; returns the number of coefficients (K + 1), or
+ * a negative error code. */
+int nw_chebyshev_exp_coefficients(double rho, double tol, double* coef_host, int32_t capacity);
 /* Symmetric format only: the per-segment w of one cell. */
 int nw_propagator_cell_weights(const nw_propagator* p, int64_t cell, double* w_host);
 int nw_propagator_destroy(nw_propagator* p);

@@ -382,7 +382,100 @@ struct nw_propagator {
     int symmetric = 0;
     double* wgt = nullptr;         // [num_rows] w of every segment (symmetric format)
     int32_t* order = nullptr;      // cells by decreasing stored entries (launch order)
+    // Chebyshev store: neighbours of every row (local index, coefficient (2/rho) b_ij), slot 2
+    // holding -(start + 1) of an overflow list ended by index -1 when a row has more than 3
+    int chebyshev = 0;
+    int32_t* nb_idx = nullptr;     // [num_rows][3]
+    float* nb_coef = nullptr;      // [num_rows][3]
+    int32_t* ov_idx = nullptr;     // overflow lists
+    float* ov_coef = nullptr;
+    int32_t* cheb_K = nullptr;     // [num_cells] index of the last series coefficient
+    int64_t* cheb_off = nullptr;   // [num_cells] first coefficient of the cell
+    double* cheb_c = nullptr;      // coefficients of all cells
+    double* cheb_v64 = nullptr;    // [num_rows] voltages between the launches of a step
+    double mean_terms = 0.0;
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;
+    std::vector<int32_t> h_cheb_K;
+    std::vector<int64_t> h_cheb_off;
+    std::vector<double> h_cheb_c, h_rho;
     int64_t vals_capacity = 0;
 };
+
+// Device pointers of a Chebyshev store, passed to the kernels that read it.
+struct ChebStore {
+    const int32_t* nb_idx;
+    const float* nb_coef;
+    const int32_t* ov_idx;
+    const float* ov_coef;
+    const int32_t* K;
+    const int64_t* off;
+    const double* c;
+};
+
+inline ChebStore cheb_store(const nw_propagator* p) {
+    return ChebStore{p->nb_idx, p->nb_coef, p->ov_idx, p->ov_coef, p->cheb_K, p->cheb_off,
+                     p->cheb_c};
+}
+
+// Rows per thread of the Chebyshev products (cells up to kChebBlock * kChebRows segments).
+constexpr int kChebBlock = 1024;
+constexpr int kChebRows = 5;
+
+// One Chebyshev product v = sum_k c_k T_k(A~) x for the rows this thread owns (row
+// threadIdx.x + j * kChebBlock), A~ t = t + sum_j coef_ij (t_j - t_i) over the row's neighbours.
+// On entry buf0[i] = x_i for every row of the cell (after a barrier); on return the thread's
+// rows hold their result in `out[j]` and the call has ended with a barrier, so both LDS buffers
+// are free.  K = 0 returns x.
+__device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const float* __restrict__ nbc,
+                                          const int32_t* __restrict__ ovi, const float* __restrict__ ovc,
+                                          const double* __restrict__ c, int K, int n, double* buf0,
+                                          double* buf1, double (&out)[kChebRows]) {
+    // neighbours 0 and 1 (local rows < 65536) packed in one register, neighbour 2 (or the
+    // overflow marker) in another
+    uint32_t i01[kChebRows];
+    int32_t i2[kChebRows];
+    float cf[kChebRows][3];
+#pragma unroll
+    for (int j = 0; j < kChebRows; ++j) {
+        const int r = threadIdx.x + j * kChebBlock;
+        out[j] = 0.0;
+        i01[j] = r < n ? ((uint32_t)nbi[3 * r] | ((uint32_t)nbi[3 * r + 1] << 16)) : 0u;
+        i2[j] = r < n ? nbi[3 * r + 2] : 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cf[j][d] = r < n ? nbc[3 * r + d] : 0.0f;
+        if (r < n) out[j] = c[0] * buf0[r];
+    }
+    // the term before the current one is still in the buffer about to be written: the
+    // recurrence t_{k+1} = 2 A~ t_k - t_{k-1} reads t_k and t_{k-1} of a row from LDS
+    double* cur = buf0;
+    double* nxt = buf1;
+    for (int k = 1; k <= K; ++k) {
+        const double ck = c[k];
+#pragma unroll
+        for (int j = 0; j < kChebRows; ++j) {
+            const int r = threadIdx.x + j * kChebBlock;
+            if (r < n) {
+                const double tc = cur[r];
+                double s = 0.0;
+                if (cf[j][0] != 0.0f) s = fma((double)cf[j][0], cur[i01[j] & 0xFFFFu] - tc, s);
+                if (cf[j][1] != 0.0f) s = fma((double)cf[j][1], cur[i01[j] >> 16] - tc, s);
+                if (i2[j] >= 0) {
+                    if (cf[j][2] != 0.0f) s = fma((double)cf[j][2], cur[i2[j]] - tc, s);
+                } else {
+                    for (int e = -i2[j] - 1; ovi[e] >= 0; ++e)
+                        s = fma((double)ovc[e], cur[ovi[e]] - tc, s);
+                }
+                const double z = tc + s;
+                const double tn = k == 1 ? z : 2.0 * z - nxt[r];
+                nxt[r] = tn;
+                out[j] = fma(ck, tn, out[j]);
+            }
+        }
+        __syncthreads();
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    __syncthreads();
+}

@@ -47,6 +47,7 @@ struct HHArgs {
     float* record;       // optional: per step 5 arrays (v, m, h, n at the start, v after half A)
     int64_t num_rows;
     const double* wgt;   // symmetric format: w of every segment
+    ChebStore cheb;      // Chebyshev store
 };
 
 // Stimulus current (A) of the cell's stimulated segment on input tick `tick` (1-based).
@@ -275,6 +276,90 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
     if (threadIdx.x == 0 && ap_row >= 0) {
         a.ap_state[cell] = ap_state;
         a.ap_count[cell] = ap_count;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// The step on the Chebyshev store: one launch per half step (HALF 0: relax with the conductances
+// of the previous mechanism step, then v = P x; HALF 1: gates at v(t + dt/2), relax, v = P x, AP
+// detector).  A launch holds one phase and one series, which keeps it within the 128 registers
+// of 1024-thread blocks.  Voltages pass between launches as FP64 (v64); the FP32 voltage array
+// is written after every step.
+// ----------------------------------------------------------------------------------------------
+template <int GATE_MODE, int HALF>
+__global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v64, int step,
+                                                             int from_v32) {
+    extern __shared__ double lds64[];
+    double* b0 = lds64;
+    double* b1 = lds64 + a.nmax;
+    const nw_hh_params& p = a.prm;
+    const int cell = a.order[blockIdx.x];
+    const int64_t row0 = a.cell_row0[cell];
+    const int nrows = (int)(a.cell_row0[cell + 1] - row0);
+    const double hdt = 0.5 * (double)p.dt_ms;
+    const double hdt_s = hdt * 1e-3;
+    const int64_t gstep = p.step0 + step;
+    const int64_t tick = 2 * gstep + 1 + HALF;
+    const int stim_row = a.stim_local ? a.stim_local[cell] : -1;
+    const bool g_valid = HALF == 1 || p.conductances_valid || step > 0;
+    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+        const int64_t gi = row0 + i;
+        const double v = from_v32 ? (double)a.v[gi] : v64[gi];
+        double g = 0.0, d = 0.0;
+        if (HALF == 1) {
+            double m = a.m[gi], h = a.h[gi], n = a.n[gi];
+            if (GATE_MODE == 1) {
+                hh_exact(v, (double)p.q10, (double)p.dt_ms, m, h, n);
+            } else {
+                const int nb = p.num_nodes - 1;
+                const float2* tm = reinterpret_cast<const float2*>(a.table);
+                int b;
+                double f;
+                bucket_of(v, (double)p.vmin, (double)nb / ((double)p.vmax - (double)p.vmin), nb, b, f);
+                m = gate_advance(tm, b, f, m);
+                h = gate_advance(tm + p.num_nodes, b, f, h);
+                n = gate_advance(tm + 2 * p.num_nodes, b, f, n);
+            }
+            const float m32 = (float)m, h32 = (float)h, n32 = (float)n;
+            a.m[gi] = m32;
+            a.h[gi] = h32;
+            a.n[gi] = n32;
+            hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], m32, h32, n32, p.e_na, p.e_k, p.e_l,
+                           g, d);
+        } else if (g_valid) {
+            hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], a.m[gi], a.h[gi], a.n[gi], p.e_na,
+                           p.e_k, p.e_l, g, d);
+        }
+        const double invc = a.inv_c[gi];
+        const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick) : 0.0;
+        const double alpha = exp(-hdt_s * g * invc);
+        b0[i] = d - (d - v) * alpha + hdt * I * invc;
+    }
+    __syncthreads();
+    double out[kChebRows];
+    cheb_rows(a.cheb.nb_idx + 3 * row0, a.cheb.nb_coef + 3 * row0, a.cheb.ov_idx, a.cheb.ov_coef,
+              a.cheb.c + a.cheb.off[cell], a.cheb.K[cell], nrows, b0, b1, out);
+#pragma unroll
+    for (int j = 0; j < kChebRows; ++j) {
+        const int r = threadIdx.x + j * kChebBlock;
+        if (r < nrows) {
+            v64[row0 + r] = out[j];
+            if (HALF == 1) a.v[row0 + r] = (float)out[j];
+            b0[r] = out[j];
+        }
+    }
+    if (HALF == 1) {
+        const int ap_row = a.ap_local ? a.ap_local[cell] : -1;
+        if (ap_row >= 0) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const double vap = b0[ap_row];
+                const bool over = vap >= (double)p.ap_threshold;
+                if (over && !a.ap_state[cell]) ++a.ap_count[cell];
+                a.ap_state[cell] = over ? 1 : 0;
+                if (a.trace) a.trace[(int64_t)cell * a.trace_stride + step] = (float)vap;
+            }
+        }
     }
 }
 
@@ -861,16 +946,36 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
              p->max_cell_rows,
              n_steps, *prm, gate_table, v, m, h, n, g_na, g_k, g_l, inv_c, stim_local, ap_local,
              ap_state, ap_count, trace, trace_stride, record, p->num_rows, p->wgt};
+    a.cheb = cheb_store(p);
     const bool sym = p->symmetric != 0;
-    const size_t lds = (sym ? sizeof(double) + sizeof(float) : 2 * sizeof(float)) *
+    const size_t lds = (p->chebyshev ? 2 * sizeof(double)
+                                     : sym ? sizeof(double) + sizeof(float) : 2 * sizeof(float)) *
                        (size_t)p->max_cell_rows;
     NW_REQUIRE(lds <= 160 * 1024, "cell of %d segments exceeds LDS", p->max_cell_rows);
     hipStream_t s = (hipStream_t)stream;
+    NW_REQUIRE(prm->gate_mode == 0 || prm->gate_mode == 1, "gate_mode must be 0 or 1");
+    if (p->chebyshev) {
+        NW_REQUIRE(prm->block_threads == 0 || prm->block_threads == kChebBlock,
+                   "the Chebyshev store runs %d threads per workgroup", kChebBlock);
+        NW_REQUIRE(record == nullptr, "the Chebyshev store keeps no record for the adjoint");
+        const unsigned grid = (unsigned)p->num_cells;
+        double* v64 = p->cheb_v64;
+        for (int st = 0; st < n_steps; ++st) {
+            if (prm->gate_mode == 1) {
+                k_hh_half_cheb<1, 0><<<grid, kChebBlock, lds, s>>>(a, v64, st, st == 0);
+                k_hh_half_cheb<1, 1><<<grid, kChebBlock, lds, s>>>(a, v64, st, 0);
+            } else {
+                k_hh_half_cheb<0, 0><<<grid, kChebBlock, lds, s>>>(a, v64, st, st == 0);
+                k_hh_half_cheb<0, 1><<<grid, kChebBlock, lds, s>>>(a, v64, st, 0);
+            }
+        }
+        NW_HIP(hipGetLastError());
+        return NW_OK;
+    }
     // default: 512 threads for the symmetric format (two workgroups per CU for 5000-segment cells
     // under the LDS budget), 256 for the general one (flat from 256 to 1024 at 10M compartments)
     const int bt = prm->block_threads ? prm->block_threads : (sym ? 512 : 256);
     const unsigned grid = (unsigned)p->num_cells;
-    NW_REQUIRE(prm->gate_mode == 0 || prm->gate_mode == 1, "gate_mode must be 0 or 1");
 #define NW_HH_LAUNCH(B)                                                                   \
     case B:                                                                               \
         if (sym) {                                                                        \
@@ -900,6 +1005,7 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
                    target && lam_v && lam_m && lam_h && lam_n && grad_na && grad_k && grad_l &&
                    grad_theta && loss,
                "nw_hh_adjoint: null argument");
+    NW_REQUIRE(!p->chebyshev, "nw_hh_adjoint reads a stored propagator (general or symmetric)");
     NW_REQUIRE(prm->gate_mode == 0, "the adjoint is written for the table gate mode");
     NW_REQUIRE(prm->stim_mode == 0 || prm->stim_mode == 1, "adjoint supports constant stimulus only");
     NW_REQUIRE(n_steps >= 0 && prm->num_nodes >= 2, "bad arguments");
@@ -1018,7 +1124,8 @@ int nw_lti_advance_nd(int32_t S, int32_t num_inputs, const nw_lti_input* inputs_
 int nw_ca_step(const nw_propagator* p, const nw_ca_params* prm, int32_t n_steps, float* ca,
                float* cab, float* dcdt, void* stream) {
     NW_REQUIRE(p && prm && ca && cab && dcdt, "nw_ca_step: null argument");
-    NW_REQUIRE(!p->symmetric, "nw_ca_step takes the general (non-symmetric) format");
+    NW_REQUIRE(!p->symmetric && !p->chebyshev,
+               "nw_ca_step takes the general format (not the symmetric or Chebyshev store)");
     NW_REQUIRE(n_steps >= 0 && prm->dt_ms > 0.0f, "bad step count or time step");
     if (n_steps == 0) return NW_OK;
     CaArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,

@@ -422,8 +422,163 @@ static int grow(nw_propagator* p, int64_t need) {
     return NW_OK;
 }
 
+// c_0 = e^-y I_0(y), c_k = 2 e^-y I_k(y) (y = rho/2), k = 0..K, K the last index with c_k > tol,
+// the dropped terms' sum added to c_0.
+// Miller's backward recurrence I_{k-1} = (2k/y) I_k + I_{k+1} from an index where I_k/I_0 is far
+// below tol (I_k/I_0 ~ exp(-k^2 / 2y)), rescaled against overflow, normalised with
+// e^y = I_0(y) + 2 sum_k I_k(y).
+static int exp_chebyshev(double rho, double tol, std::vector<double>& c) {
+    c.clear();
+    const double y = 0.5 * rho;
+    if (!(y > 1e-300)) {
+        c.push_back(1.0);
+        return 0;
+    }
+    const int M = (int)std::ceil(std::sqrt(2.0 * y * 120.0)) + 40;
+    std::vector<double> J(M + 2, 0.0);
+    J[M] = 1e-280;
+    for (int k = M; k >= 1; --k) {
+        J[k - 1] = (2.0 * k / y) * J[k] + J[k + 1];
+        if (J[k - 1] > 1e250)
+            for (int q = k - 1; q <= M; ++q) J[q] *= 1e-250;
+    }
+    double S = J[0];
+    for (int k = 1; k <= M; ++k) S += 2.0 * J[k];
+    int K = 0;
+    for (int k = 1; k <= M; ++k)
+        if (2.0 * J[k] / S > tol) K = k;
+    if (K >= M - 10) return -1;  // the start index was too small
+    c.resize(K + 1);
+    c[0] = J[0] / S;
+    double tail = 0.0;
+    for (int k = K + 1; k <= M; ++k) tail += 2.0 * J[k] / S;
+    for (int k = 1; k <= K; ++k) c[k] = 2.0 * J[k] / S;
+    // the dropped tail is added to c_0: the operator keeps constant vectors (T_k(A~) 1 = 1), so
+    // the truncated series then keeps them exactly as exp(A dt) does (P 1 = 1)
+    c[0] += tail;
+    return 0;
+}
+
+// Chebyshev store: per cell the edge coefficients of B = A dt in the difference form
+// (B t)_i = sum_j b_ij (t_j - t_i), b_ij = dt / (R_edge C_i) (electric.py:92-108), scaled by
+// 2 / rho, the series coefficients, and the cells ordered by decreasing work (terms x rows).
+static constexpr double kChebTol = 1e-10;
+
+static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int32_t* parent,
+                           const double* R, const double* C, double dt_ms) {
+    const int64_t N = p->num_rows;
+    const double dt_s = dt_ms * 1e-3;
+    std::vector<int32_t> nb(3 * N, 0), ovi;
+    std::vector<float> nc(3 * N, 0.0f), ovc;
+    p->h_cheb_K.assign(p->num_cells, 0);
+    p->h_cheb_off.assign(p->num_cells, 0);
+    p->h_rho.assign(p->num_cells, 0.0);
+    p->h_cheb_c.clear();
+    std::vector<double> coef;
+    std::vector<std::vector<std::pair<int, double>>> adj;
+    double terms_rows = 0.0;
+    int max_deg = 0;
+    int64_t edges = 0;
+    for (int64_t cell = 0; cell < p->num_cells; ++cell) {
+        const int64_t r0 = cell_ptr[cell];
+        const int n = (int)(cell_ptr[cell + 1] - r0);
+        adj.assign(n, {});
+        std::vector<double> diag(n, 0.0);
+        for (int c = 0; c < n; ++c) {
+            const int q = parent[r0 + c];
+            if (q < 0) continue;
+            NW_REQUIRE(q < n && q != c, "cell %lld: bad parent %d of segment %d", (long long)cell, q, c);
+            const double bc = dt_s / (R[r0 + c] * C[r0 + c]);  // row c, neighbour q
+            const double bq = dt_s / (R[r0 + c] * C[r0 + q]);  // row q, neighbour c
+            adj[c].push_back({q, bc});
+            adj[q].push_back({c, bq});
+            diag[c] += bc;
+            diag[q] += bq;
+            ++edges;
+        }
+        double rho = 0.0;
+        for (int i = 0; i < n; ++i) rho = std::max(rho, 2.0 * diag[i]);
+        NW_REQUIRE(exp_chebyshev(rho, kChebTol, coef) == 0,
+                   "cell %lld: no Chebyshev series for rho = %g", (long long)cell, rho);
+        p->h_rho[cell] = rho;
+        p->h_cheb_K[cell] = (int32_t)coef.size() - 1;
+        p->h_cheb_off[cell] = (int64_t)p->h_cheb_c.size();
+        p->h_cheb_c.insert(p->h_cheb_c.end(), coef.begin(), coef.end());
+        terms_rows += (double)(coef.size() - 1) * n;
+        const double scale = rho > 0.0 ? 2.0 / rho : 0.0;
+        for (int i = 0; i < n; ++i) {
+            const int64_t g = r0 + i;
+            const auto& a = adj[i];
+            max_deg = std::max<int>(max_deg, (int)a.size());
+            for (int d = 0; d < 3; ++d) nb[3 * g + d] = i;  // unused slots: the row itself, 0
+            if (a.size() <= 3) {
+                for (size_t d = 0; d < a.size(); ++d) {
+                    nb[3 * g + d] = a[d].first;
+                    nc[3 * g + d] = (float)(scale * a[d].second);
+                }
+            } else {
+                for (int d = 0; d < 2; ++d) {
+                    nb[3 * g + d] = a[d].first;
+                    nc[3 * g + d] = (float)(scale * a[d].second);
+                }
+                nb[3 * g + 2] = -(int32_t)ovi.size() - 1;
+                for (size_t d = 2; d < a.size(); ++d) {
+                    ovi.push_back(a[d].first);
+                    ovc.push_back((float)(scale * a[d].second));
+                }
+                ovi.push_back(-1);
+                ovc.push_back(0.0f);
+            }
+        }
+    }
+    if (ovi.empty()) {  // a valid (never read) overflow list
+        ovi.push_back(-1);
+        ovc.push_back(0.0f);
+    }
+    p->nnz = 2 * edges;
+    p->stored = 0;
+    p->max_row_nnz = max_deg;
+    p->mean_terms = N ? terms_rows / (double)N : 0.0;
+    NW_HIP(hipMalloc(&p->nb_idx, sizeof(int32_t) * 3 * N));
+    NW_HIP(hipMalloc(&p->nb_coef, sizeof(float) * 3 * N));
+    NW_HIP(hipMalloc(&p->ov_idx, sizeof(int32_t) * ovi.size()));
+    NW_HIP(hipMalloc(&p->ov_coef, sizeof(float) * ovc.size()));
+    NW_HIP(hipMalloc(&p->cheb_K, sizeof(int32_t) * p->num_cells));
+    NW_HIP(hipMalloc(&p->cheb_off, sizeof(int64_t) * p->num_cells));
+    NW_HIP(hipMalloc(&p->cheb_c, sizeof(double) * p->h_cheb_c.size()));
+    NW_HIP(hipMalloc(&p->cheb_v64, sizeof(double) * N));
+    NW_HIP(hipMemcpy(p->nb_idx, nb.data(), sizeof(int32_t) * 3 * N, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->nb_coef, nc.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->ov_idx, ovi.data(), sizeof(int32_t) * ovi.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->ov_coef, ovc.data(), sizeof(float) * ovc.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->cheb_K, p->h_cheb_K.data(), sizeof(int32_t) * p->num_cells,
+                     hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->cheb_off, p->h_cheb_off.data(), sizeof(int64_t) * p->num_cells,
+                     hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->cheb_c, p->h_cheb_c.data(), sizeof(double) * p->h_cheb_c.size(),
+                     hipMemcpyHostToDevice));
+    std::vector<int32_t> order(p->num_cells);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        const double wa = (double)p->h_cheb_K[a] * (double)(cell_ptr[a + 1] - cell_ptr[a]);
+        const double wb = (double)p->h_cheb_K[b] * (double)(cell_ptr[b + 1] - cell_ptr[b]);
+        return wa > wb;
+    });
+    NW_HIP(hipMalloc(&p->order, sizeof(int32_t) * p->num_cells));
+    NW_HIP(hipMemcpy(p->order, order.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
+    return NW_OK;
+}
+
 static void free_prop(nw_propagator* p) {
     if (!p) return;
+    if (p->nb_idx) (void)hipFree(p->nb_idx);
+    if (p->nb_coef) (void)hipFree(p->nb_coef);
+    if (p->ov_idx) (void)hipFree(p->ov_idx);
+    if (p->ov_coef) (void)hipFree(p->ov_coef);
+    if (p->cheb_K) (void)hipFree(p->cheb_K);
+    if (p->cheb_off) (void)hipFree(p->cheb_off);
+    if (p->cheb_c) (void)hipFree(p->cheb_c);
+    if (p->cheb_v64) (void)hipFree(p->cheb_v64);
     if (p->cell_row0) (void)hipFree(p->cell_row0);
     if (p->cell_nz0) (void)hipFree(p->cell_nz0);
     if (p->row_off) (void)hipFree(p->row_off);
@@ -439,16 +594,21 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
                  const double* R, const double* C, double dt_ms, double prune, int flags,
                  nw_propagator* p) {
     const bool sym = (flags & NW_PROPAGATOR_SYMMETRIC) != 0;
+    const bool cheb = (flags & NW_PROPAGATOR_CHEBYSHEV) != 0;
+    NW_REQUIRE(!(sym && cheb), "the symmetric and Chebyshev stores are exclusive");
     p->symmetric = sym ? 1 : 0;
+    p->chebyshev = cheb ? 1 : 0;
     auto t0 = std::chrono::steady_clock::now();
     NW_REQUIRE(num_cells > 0, "num_cells must be positive");
     NW_REQUIRE(cell_ptr[0] == 0, "cell_ptr[0] must be 0");
     NW_REQUIRE(dt_ms > 0.0 && prune >= 0.0, "time step must be positive and prune non-negative");
     int nmax = 0;
+    const int64_t limit = cheb ? (int64_t)kChebBlock * kChebRows : 65535;
     for (int64_t c = 0; c < num_cells; ++c) {
         int64_t n = cell_ptr[c + 1] - cell_ptr[c];
-        NW_REQUIRE(n >= 1 && n <= 65535, "cell %lld has %lld segments (1..65535 supported)",
-                   (long long)c, (long long)n);
+        NW_REQUIRE(n >= 1 && n <= limit, "cell %lld has %lld segments (1..%lld supported%s)",
+                   (long long)c, (long long)n, (long long)limit,
+                   cheb ? " by the Chebyshev store" : "");
         nmax = std::max<int>(nmax, (int)n);
     }
     NW_HIP(hipSetDevice(device));
@@ -458,6 +618,15 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
     p->max_cell_rows = nmax;
     p->h_cell_row0.assign(cell_ptr, cell_ptr + num_cells + 1);
     p->h_cell_nz0.assign(num_cells, 0);
+    if (cheb) {
+        NW_HIP(hipMalloc(&p->cell_row0, sizeof(int64_t) * (num_cells + 1)));
+        NW_HIP(hipMemcpy(p->cell_row0, cell_ptr, sizeof(int64_t) * (num_cells + 1),
+                         hipMemcpyHostToDevice));
+        int rc = build_chebyshev(p, cell_ptr, parent, R, C, dt_ms);
+        p->build_seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    }
     NW_HIP(hipMalloc(&p->row_off, sizeof(uint32_t) * (p->num_rows + num_cells)));
     NW_HIP(hipMalloc(&p->rowsum, sizeof(double) * p->num_rows));
     if (sym) NW_HIP(hipMalloc(&p->wgt, sizeof(double) * p->num_rows));
@@ -618,6 +787,28 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
     }
 }
 
+// Chebyshev store: x in LDS as FP64, v = sum_k c_k T_k(A~) x (cheb_rows).
+__global__ void __launch_bounds__(kChebBlock) k_half_step_cheb(HalfArgs a, ChebStore cs) {
+    extern __shared__ double lds64[];
+    const int cell = a.order[blockIdx.x];
+    const int64_t row0 = a.cell_row0[cell];
+    const int n = (int)(a.cell_row0[cell + 1] - row0);
+    const double dt_ms = a.half_dt_ms;
+    const double dt_s = dt_ms * 1e-3;
+    double* b0 = lds64;
+    double* b1 = lds64 + a.nmax;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) b0[i] = half_x(a, row0 + i, dt_ms, dt_s, true);
+    __syncthreads();
+    double out[kChebRows];
+    cheb_rows(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef,
+              cs.c + cs.off[cell], cs.K[cell], n, b0, b1, out);
+#pragma unroll
+    for (int j = 0; j < kChebRows; ++j) {
+        const int r = threadIdx.x + j * kChebBlock;
+        if (r < n) a.v[row0 + r] = (float)out[j];
+    }
+}
+
 }  // namespace nw
 
 using namespace nw;
@@ -657,7 +848,8 @@ int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_pt
                            nw_propagator** out) {
     NW_REQUIRE(out && cell_ptr_host && parent_local_host && R_host && C_host,
                "nw_propagator_build: null argument");
-    NW_REQUIRE((flags & ~NW_PROPAGATOR_SYMMETRIC) == 0, "unknown propagator flags %d", flags);
+    NW_REQUIRE((flags & ~(NW_PROPAGATOR_SYMMETRIC | NW_PROPAGATOR_CHEBYSHEV)) == 0,
+               "unknown propagator flags %d", flags);
     *out = nullptr;
     auto* p = new nw_propagator();
     int rc = build(device, num_cells, cell_ptr_host, parent_local_host, R_host, C_host,
@@ -680,11 +872,39 @@ int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
     s->max_row_nnz = p->max_row_nnz;
     s->build_seconds = p->build_seconds;
     s->symmetric = p->symmetric;
+    s->chebyshev = p->chebyshev;
+    s->mean_terms = p->mean_terms;
     s->device_bytes = p->vals_capacity * (int64_t)(sizeof(float) + sizeof(uint16_t)) +
                       (p->num_rows + p->num_cells) * (int64_t)sizeof(uint32_t) +
                       p->num_rows * (int64_t)sizeof(double) * (p->symmetric ? 2 : 1) +
                       p->num_cells * (int64_t)(2 * sizeof(int64_t) + sizeof(int32_t));
+    if (p->chebyshev)
+        s->device_bytes = p->num_rows * (int64_t)(3 * sizeof(int32_t) + 3 * sizeof(float)) +
+                          (int64_t)p->h_cheb_c.size() * (int64_t)sizeof(double) +
+                          p->num_cells * (int64_t)(2 * sizeof(int64_t) + 2 * sizeof(int32_t));
     return NW_OK;
+}
+
+int nw_propagator_cell_chebyshev(const nw_propagator* p, int64_t cell, int32_t* K_out,
+                                 double* rho_out, double* coef_host) {
+    NW_REQUIRE(p && K_out && rho_out, "nw_propagator_cell_chebyshev: null argument");
+    NW_REQUIRE(p->chebyshev, "the propagator is not in the Chebyshev store");
+    NW_REQUIRE(cell >= 0 && cell < p->num_cells, "cell index out of range");
+    *K_out = p->h_cheb_K[cell];
+    *rho_out = p->h_rho[cell];
+    if (coef_host)
+        std::memcpy(coef_host, p->h_cheb_c.data() + p->h_cheb_off[cell],
+                    sizeof(double) * (p->h_cheb_K[cell] + 1));
+    return NW_OK;
+}
+
+int nw_chebyshev_exp_coefficients(double rho, double tol, double* coef_host, int32_t capacity) {
+    NW_REQUIRE(rho >= 0.0 && tol > 0.0 && capacity >= 0, "bad argument");
+    std::vector<double> c;
+    NW_REQUIRE(exp_chebyshev(rho, tol, c) == 0, "no series for rho = %g", rho);
+    if (coef_host)
+        std::memcpy(coef_host, c.data(), sizeof(double) * std::min<size_t>(c.size(), capacity));
+    return (int)c.size();
 }
 
 int nw_propagator_destroy(nw_propagator* p) {
@@ -747,6 +967,7 @@ int nw_propagator_cell_weights(const nw_propagator* p, int64_t cell, double* w_h
 int nw_propagator_cell_export(const nw_propagator* p, int64_t cell, uint32_t* row_off_host,
                               uint16_t* col_host, float* val_host) {
     NW_REQUIRE(p && cell >= 0 && cell < p->num_cells, "cell index out of range");
+    NW_REQUIRE(!p->chebyshev, "the Chebyshev store holds no matrix to export");
     NW_HIP(hipSetDevice(p->device));
     int64_t r0 = p->h_cell_row0[cell];
     int n = (int)(p->h_cell_row0[cell + 1] - r0);
@@ -768,6 +989,13 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
     NW_REQUIRE(p && v && sum_g && drive_v && sum_i && inv_c, "nw_electric_half_step: null argument");
     HalfArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
                p->max_cell_rows, half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v, p->wgt};
+    if (p->chebyshev) {
+        const size_t lds = 2 * sizeof(double) * (size_t)p->max_cell_rows;
+        k_half_step_cheb<<<(unsigned)p->num_cells, kChebBlock, lds, (hipStream_t)stream>>>(
+            a, cheb_store(p));
+        NW_HIP(hipGetLastError());
+        return NW_OK;
+    }
     const bool sym = p->symmetric != 0;
     // general: x and the product as FP32; symmetric: y as FP32 and z as FP64
     size_t lds = (sym ? sizeof(double) + sizeof(float) : 2 * sizeof(float)) *

@@ -1027,7 +1027,8 @@ class RxD_Model:
     def __init__(self, time_step=0.1, *, temperature=37, initial_voltage=-70,
                  cytoplasmic_resistance=100, membrane_capacitance=1, extracellular_tortuosity=1.55,
                  extracellular_max_distance=20e-6, species=(), mechanisms=(), device=0,
-                 prune=EPSILON, symmetric_propagator=False, step_graph=True):
+                 prune=EPSILON, symmetric_propagator=False, step_graph=True,
+                 propagator_store=None):
         self.time_step = float(time_step)
         self.temperature = float(temperature)
         self.initial_voltage = float(initial_voltage)
@@ -1037,6 +1038,9 @@ class RxD_Model:
         # symmetric_propagator: store sqrt(C) P sqrt(C)^-1, one triangle (half the bytes per half
         # step; pairs pruned together; sums in atomic order, so not bitwise reproducible)
         self.symmetric_propagator = bool(symmetric_propagator)
+        # propagator_store: "general" | "symmetric" | "chebyshev" (overrides symmetric_propagator)
+        from .population import store_name
+        self.propagator_store = store_name(propagator_store, self.symmetric_propagator)
         # step_graph: when every callback of a step runs on the device (device mechanisms only,
         # play back / recording / traces of device attributes), advance() replays a captured
         # graph of the step's kernels instead of launching them one by one from Python
@@ -1433,7 +1437,7 @@ class RxD_Model:
         prop = self._propagators.get(key)
         if prop is None:
             prop = Propagator(self._cell_ptr, self._parent_local, self._R, self._C, time_step,
-                              self.prune, self.device, symmetric=self.symmetric_propagator)
+                              self.prune, self.device, store=self.propagator_store)
             self._propagators = {key: prop}
         return prop
 

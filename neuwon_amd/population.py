@@ -21,29 +21,49 @@ def _torch():
     return torch
 
 
-class Propagator:
-    """Owner of an ``nw_propagator`` handle (per-cell pruned CSR, FP32 values).
+STORES = ("general", "symmetric", "chebyshev")
 
-    symmetric=True stores S = W P W^-1 (w = sqrt(C)) on and above the diagonal only; a pair (i, j)
-    is then kept when max(P_ij, P_ji) >= prune.  The fused HH step, its adjoint and the generic
-    half step read that format; the species step needs the general one.
+
+def store_name(store=None, symmetric=False):
+    """The propagator store asked for by ``store`` (a name of STORES) or the older ``symmetric``
+    flag."""
+    if store is None:
+        return "symmetric" if symmetric else "general"
+    if store not in STORES:
+        raise ValueError(f"propagator store {store!r}: one of {STORES}")
+    return store
+
+
+class Propagator:
+    """Owner of an ``nw_propagator`` handle.
+
+    store="general": per-cell pruned CSR, FP32 values.  store="symmetric" (or symmetric=True)
+    stores S = W P W^-1 (w = sqrt(C)) on and above the diagonal only; a pair (i, j) is then kept
+    when max(P_ij, P_ji) >= prune.  store="chebyshev" stores no matrix: products are Chebyshev
+    series of the cell's tree operator, unpruned (prune is ignored), for cells up to 5120
+    segments.  The fused HH step and the generic half step read every store, the adjoint the
+    first two, the species step the general one.
     """
 
-    def __init__(self, cell_ptr, parent_local, R, C, half_dt_ms, prune, device=0, symmetric=False):
+    def __init__(self, cell_ptr, parent_local, R, C, half_dt_ms, prune, device=0, symmetric=False,
+                 store=None):
         lib = _lib.load()
         self.cell_ptr = np.ascontiguousarray(cell_ptr, dtype=np.int64)
         self._parent = np.ascontiguousarray(parent_local, dtype=np.int32)
         self._R = np.ascontiguousarray(R, dtype=np.float64)
         self._C = np.ascontiguousarray(C, dtype=np.float64)
         self.half_dt_ms, self.prune, self.device = float(half_dt_ms), float(prune), int(device)
-        self.symmetric = bool(symmetric)
+        self.store = store_name(store, symmetric)
+        self.symmetric = self.store == "symmetric"
+        self.chebyshev = self.store == "chebyshev"
+        flags = {"general": 0, "symmetric": _lib.NW_PROPAGATOR_SYMMETRIC,
+                 "chebyshev": _lib.NW_PROPAGATOR_CHEBYSHEV}[self.store]
         h = ctypes.c_void_p()
         with _lib.lock():
             _lib.check(lib.nw_propagator_build_ex(self.device, len(self.cell_ptr) - 1,
                                                   _lib.ptr(self.cell_ptr), _lib.ptr(self._parent),
                                                   _lib.ptr(self._R), _lib.ptr(self._C),
-                                                  self.half_dt_ms, self.prune,
-                                                  _lib.NW_PROPAGATOR_SYMMETRIC if symmetric else 0,
+                                                  self.half_dt_ms, self.prune, flags,
                                                   ctypes.byref(h)))
         self.handle = h
         st = _lib.NwPropagatorStats()
@@ -65,6 +85,17 @@ class Propagator:
         full = (full.multiply(w[None, :]).multiply(1.0 / w[:, None])).tocsr()
         full.eliminate_zeros()
         return full
+
+    def cell_chebyshev(self, cell):
+        """(rho, coefficients) of one cell's series (Chebyshev store)."""
+        K, rho = ctypes.c_int32(), ctypes.c_double()
+        lib = _lib.load()
+        _lib.check(lib.nw_propagator_cell_chebyshev(self.handle, cell, ctypes.byref(K),
+                                                    ctypes.byref(rho), None))
+        c = np.empty(K.value + 1, dtype=np.float64)
+        _lib.check(lib.nw_propagator_cell_chebyshev(self.handle, cell, ctypes.byref(K),
+                                                    ctypes.byref(rho), _lib.ptr(c)))
+        return rho.value, c
 
     def cell_weights(self, cell):
         """w of every segment of one cell (symmetric format)."""
@@ -136,12 +167,13 @@ class HHPopulation:
     stim: dict(mode='constant'|'poisson'|None, amp_na, rate_hz, duration_ms, seed) applied to the
     root (segment 0) of every cell.
     symmetric: store the propagator in the symmetric half format (half the bytes per step).
+    store: "general", "symmetric" or "chebyshev" (no stored matrix; see Propagator).
     """
 
     def __init__(self, cells, dt=0.1, celsius=6.3, v0=-70.0, cytoplasmic_resistance=100.0,
                  membrane_capacitance=1.0, gbar=None, reversal=None, prune=1e-9, gate_mode="table",
                  stim=None, ap_threshold=20.0, device=0, block_threads=0, insert_area="final",
-                 symmetric=False):
+                 symmetric=False, store=None):
         if len(cells) == 0:
             raise ValueError("HHPopulation needs at least one cell")
         torch = _torch()
@@ -160,7 +192,7 @@ class HHPopulation:
                                 cytoplasmic_resistance, membrane_capacitance)
         self.R, self.C, self.geo, self.parent_local = R, C, geo, g["parent_local"]
         self.propagator = Propagator(self.cell_ptr, g["parent_local"], R, C, 0.5 * self.dt, prune,
-                                     device, symmetric=symmetric)
+                                     device, symmetric=symmetric, store=store)
         gbar = dict(channels.HH_GBAR, **(gbar or {}))
         self.reversal = dict(channels.HH_REVERSAL, **(reversal or {}))
         # magnitude = surface area when the mechanism is inserted: after the cell is grown
@@ -257,7 +289,7 @@ class HHPopulation:
         checkpoint.save(path, {k: getattr(self, k) for k in self._STATE},
                         {"kind": "HHPopulation", "steps": self.steps, "dt": self.dt,
                          "num_segments": self.num_segments, "num_cells": self.num_cells,
-                         "symmetric": self.propagator.symmetric, "stim": self.stim or None})
+                         "store": self.propagator.store, "stim": self.stim or None})
 
     def load(self, path):
         """Restore a snapshot written by ``save`` for the same cells."""
