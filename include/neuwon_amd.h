@@ -78,7 +78,8 @@ int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_pt
                            nw_propagator** out);
 /* flags = NW_PROPAGATOR_CHEBYSHEV stores no matrix: P x = exp(B) x with B = A dt (the tree
  * operator, 2(n-1) edge coefficients per cell) is evaluated as the Chebyshev series
- * sum_k c_k T_k(2 B / rho + I) x on the spectrum bound rho = 2 max_i |B_ii| (Gershgorin), with
+ * sum_k c_k T_k(2 B / rho + I) x on rho = the spectral radius of B (B is similar to a symmetric
+ * matrix; its largest eigenvalue by bisection with inertia counts, plus a 1e-8 margin), with
  * c_0 = e^(-rho/2) I_0(rho/2), c_k = 2 e^(-rho/2) I_k(rho/2), truncated where the remaining
  * coefficients fall below 1e-10 (nw_chebyshev_exp_coefficients).  Nothing is pruned: the
  * product is the reference's exp(A dt) x at its default threshold (electric.py:114-115, machine

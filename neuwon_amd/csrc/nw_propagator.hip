@@ -459,6 +459,42 @@ static int exp_chebyshev(double rho, double tol, std::vector<double>& c) {
     return 0;
 }
 
+// Number of eigenvalues below sigma of the symmetric tree matrix M (diagonal dg, entry e[i] between
+// segment i and its parent, parents before children): the negative pivots of M - sigma I
+// eliminated from the leaves up (Sylvester's law of inertia).
+static int count_below(int n, const int32_t* parent, const double* dg, const double* e,
+                       double sigma, std::vector<double>& d) {
+    for (int i = 0; i < n; ++i) d[i] = dg[i] - sigma;
+    int neg = 0;
+    for (int i = n - 1; i >= 0; --i) {
+        double di = d[i];
+        if (di == 0.0) di = -1e-300 * (std::abs(dg[i]) + 1.0);
+        if (di < 0.0) ++neg;
+        if (parent[i] >= 0) d[parent[i]] -= e[i] * e[i] / di;
+    }
+    return neg;
+}
+
+// Spectral radius of -B (B = A dt, similar to the symmetric M = -C^1/2 B C^-1/2, which is
+// positive semidefinite): bisection between max_i M_ii (a Rayleigh quotient, below it) and the
+// Gershgorin bound 2 max_i M_ii (above it) with inertia counts, returned with a 1e-8 relative
+// margin above the last bound found to hold every eigenvalue.
+static double tree_spectral_radius(int n, const int32_t* parent, const double* dg, const double* e,
+                                   std::vector<double>& work) {
+    double dmax = 0.0;
+    for (int i = 0; i < n; ++i) dmax = std::max(dmax, dg[i]);
+    if (!(dmax > 0.0)) return 0.0;
+    double lo = dmax, hi = 2.0 * dmax * (1.0 + 1e-12);
+    work.resize(n);
+    if (count_below(n, parent, dg, e, hi, work) != n) return 2.0 * dmax * (1.0 + 1e-8);
+    for (int it = 0; it < 45 && hi - lo > 1e-10 * hi; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (count_below(n, parent, dg, e, mid, work) == n) hi = mid;
+        else lo = mid;
+    }
+    return hi * (1.0 + 1e-8);
+}
+
 // Chebyshev store: per cell the edge coefficients of B = A dt in the difference form
 // (B t)_i = sum_j b_ij (t_j - t_i), b_ij = dt / (R_edge C_i) (electric.py:92-108), scaled by
 // 2 / rho, the series coefficients, and the cells ordered by decreasing work (terms x rows).
@@ -476,6 +512,7 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     p->h_cheb_c.clear();
     std::vector<double> coef;
     std::vector<std::vector<std::pair<int, double>>> adj;
+    std::vector<double> work;
     double terms_rows = 0.0;
     int max_deg = 0;
     int64_t edges = 0;
@@ -483,21 +520,22 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
         const int64_t r0 = cell_ptr[cell];
         const int n = (int)(cell_ptr[cell + 1] - r0);
         adj.assign(n, {});
-        std::vector<double> diag(n, 0.0);
+        std::vector<double> diag(n, 0.0), off(n, 0.0);
         for (int c = 0; c < n; ++c) {
             const int q = parent[r0 + c];
             if (q < 0) continue;
-            NW_REQUIRE(q < n && q != c, "cell %lld: bad parent %d of segment %d", (long long)cell, q, c);
+            NW_REQUIRE(q < c, "cell %lld: parent %d of segment %d is not an earlier segment",
+                       (long long)cell, q, c);
             const double bc = dt_s / (R[r0 + c] * C[r0 + c]);  // row c, neighbour q
             const double bq = dt_s / (R[r0 + c] * C[r0 + q]);  // row q, neighbour c
             adj[c].push_back({q, bc});
             adj[q].push_back({c, bq});
             diag[c] += bc;
             diag[q] += bq;
+            off[c] = -dt_s / (R[r0 + c] * std::sqrt(C[r0 + c] * C[r0 + q]));
             ++edges;
         }
-        double rho = 0.0;
-        for (int i = 0; i < n; ++i) rho = std::max(rho, 2.0 * diag[i]);
+        const double rho = tree_spectral_radius(n, parent + r0, diag.data(), off.data(), work);
         NW_REQUIRE(exp_chebyshev(rho, kChebTol, coef) == 0,
                    "cell %lld: no Chebyshev series for rho = %g", (long long)cell, rho);
         p->h_rho[cell] = rho;

@@ -32,8 +32,17 @@ def test_chebyshev_single_neuron_trajectory(torch_gpu, gate_mode):
                                   stim={"mode": "constant", "amp_na": 0.5})
     st = pop.propagator.stats
     rho, coef = pop.propagator.cell_chebyshev(0)
-    _say(f"rho {rho:.0f}, {len(coef) - 1} terms, build {st['build_seconds'] * 1e3:.1f} ms")
-    assert st["chebyshev"] == 1 and st["stored"] == 0 and abs(coef.sum() - 1.0) < 1e-15
+    # rho is the spectral radius of B = A dt: the symmetric similar matrix's extreme eigenvalue
+    import scipy.linalg
+    g0, R0, C0, _ = _oracle_cell(cells[0])
+    A = O.coefficient_matrix(np.asarray(cells[0].parent, dtype=np.int64), R0, C0, 0.05).toarray()
+    w = np.sqrt(C0)
+    S = (A * w[:, None]) / w[None, :]
+    lam = -scipy.linalg.eigvalsh(0.5 * (S + S.T)).min()
+    _say(f"rho {rho:.2f} (largest eigenvalue {lam:.2f}, Gershgorin {2 * (-np.diag(A)).max():.0f}), "
+         f"{len(coef) - 1} terms, build {st['build_seconds'] * 1e3:.1f} ms")
+    assert lam <= rho <= lam * (1.0 + 1e-6)
+    assert st["chebyshev"] == 1 and st["stored"] == 0 and abs(coef.sum() - 1.0) < 1e-14
     trace = torch.zeros((1, steps), dtype=torch.float32, device=pop.dev)
     pop.advance(steps, trace=trace)
     torch.cuda.synchronize()
