@@ -34,6 +34,7 @@ from . import _lib, channels, events, geometry
 
 EPSILON = float(np.finfo(np.float64).eps)
 NULL = -1
+CHEBYSHEV_MAX_CELL = 5120   # segments per cell the Chebyshev store takes
 
 
 def _torch():
@@ -1038,9 +1039,12 @@ class RxD_Model:
         # symmetric_propagator: store sqrt(C) P sqrt(C)^-1, one triangle (half the bytes per half
         # step; pairs pruned together; sums in atomic order, so not bitwise reproducible)
         self.symmetric_propagator = bool(symmetric_propagator)
-        # propagator_store: "general" | "symmetric" | "chebyshev" (overrides symmetric_propagator)
+        # propagator_store: "general" | "symmetric" | "chebyshev" (overrides symmetric_propagator).
+        # Unset: the Chebyshev store (exp(A dt) without a stored matrix) when nothing is pruned,
+        # the reference's default threshold, and every cell fits it; otherwise the general store
         from .population import store_name
-        self.propagator_store = store_name(propagator_store, self.symmetric_propagator)
+        self.propagator_store = (None if propagator_store is None and not self.symmetric_propagator
+                                 else store_name(propagator_store, self.symmetric_propagator))
         # step_graph: when every callback of a step runs on the device (device mechanisms only,
         # play back / recording / traces of device attributes), advance() replays a captured
         # graph of the step's kernels instead of launching them one by one from Python
@@ -1436,8 +1440,12 @@ class RxD_Model:
         key = round(time_step, 12)
         prop = self._propagators.get(key)
         if prop is None:
+            store = self.propagator_store
+            if store is None:
+                fits = int(np.diff(self._cell_ptr).max(initial=0)) <= CHEBYSHEV_MAX_CELL
+                store = "chebyshev" if self.prune <= EPSILON and fits else "general"
             prop = Propagator(self._cell_ptr, self._parent_local, self._R, self._C, time_step,
-                              self.prune, self.device, store=self.propagator_store)
+                              self.prune, self.device, store=store)
             self._propagators = {key: prop}
         return prop
 
@@ -1589,8 +1597,19 @@ class RxD_Model:
                 self.clock._count(1)
                 return
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.device(self._dev), torch.cuda.graph(g):
-                self._launch_step(prop)
+            # no cyclic garbage collection while the stream is captured: finalisers of collected
+            # objects (a dropped model's propagator frees device memory, a graph or an event is
+            # destroyed) make HIP calls that are not allowed during a capture.  torch.cuda.graph
+            # collects just before the capture begins.
+            import gc
+            was_enabled = gc.isenabled()
+            gc.disable()
+            try:
+                with torch.cuda.device(self._dev), torch.cuda.graph(g):
+                    self._launch_step(prop)
+            finally:
+                if was_enabled:
+                    gc.enable()
             self._graph, self._graph_key = g, key
         self._graph.replay()
         self.input_hook._count(2)

@@ -27,29 +27,30 @@ def _compare(a, b, bitwise, what):
             assert d < 1e-3, f"{what}: {k} differs by {d}"
 
 
-@pytest.mark.parametrize("symmetric", [False, True])
-def test_hh_population_resume(torch_gpu, tmp_path, symmetric):
+@pytest.mark.parametrize("store", ["general", "symmetric", "chebyshev"])
+def test_hh_population_resume(torch_gpu, tmp_path, store):
     """2,000 steps of a Poisson-stimulated population; snapshot at step 1,000; a second population
     of the same cells loads it and runs the last 1,000 steps."""
     from neuwon_amd import population, synth
     torch = torch_gpu
     cells = synth.population(total=20_000, seed=11, size_range=(200, 2000))
     stim = {"mode": "poisson", "amp_na": 0.6, "rate_hz": 40.0, "duration_ms": 1.0, "seed": 5}
-    a = population.HHPopulation(cells, stim=stim, symmetric=symmetric)
+    symmetric = store == "symmetric"
+    a = population.HHPopulation(cells, stim=stim, store=store)
     a.advance(1000)
     f = tmp_path / "hh.npz"
     a.save(f)
     a.advance(1000)
     torch.cuda.synchronize()
     want = _hh_state(a)
-    b = population.HHPopulation(cells, symmetric=symmetric)
+    b = population.HHPopulation(cells, store=store)
     b.load(f)
     assert b.steps == 1000 and b.stim == stim
     b.advance(1000)
     torch.cuda.synchronize()
     got = _hh_state(b)
     spikes = int(want["ap_count"].sum())
-    _say(f"{'symmetric' if symmetric else 'general'} store: {spikes} spikes over 2,000 steps, "
+    _say(f"{store} store: {spikes} spikes over 2,000 steps, "
          f"max |dv| after resume {np.abs(want['v'] - got['v']).max():.2e} mV")
     assert spikes > 0
     _compare(want, got, not symmetric, "HHPopulation")

@@ -18,11 +18,11 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("symmetric", [False, True])
-def test_time_constant(torch_gpu, symmetric):
-    """test_electric.py:28-46 through Segment._advance_electric (both propagator stores)."""
+@pytest.mark.parametrize("store", [None, "symmetric", "general"])
+def test_time_constant(torch_gpu, store):
+    """test_electric.py:28-46 through Segment._advance_electric (the three propagator stores)."""
     from neuwon_amd.model import RxD_Model
-    m = RxD_Model(symmetric_propagator=symmetric)
+    m = RxD_Model(propagator_store=store)
     root = m.Neuron([0, 0, 0], 20).root
     m.advance()   # builds the model on the device
     root.voltage = 0
@@ -39,11 +39,11 @@ def test_time_constant(torch_gpu, symmetric):
     m.check()
 
 
-@pytest.mark.parametrize("symmetric", [False, True])
-def test_inject_current(torch_gpu, symmetric):
-    """test_electric.py:132-150 (both propagator stores)."""
+@pytest.mark.parametrize("store", [None, "symmetric", "general"])
+def test_inject_current(torch_gpu, store):
+    """test_electric.py:132-150 (the three propagator stores)."""
     from neuwon_amd.model import RxD_Model
-    m = RxD_Model(time_step=.01, symmetric_propagator=symmetric)
+    m = RxD_Model(time_step=.01, propagator_store=store)
     root = m.Neuron([0, 0, 0], 10).root
     init_v = root.voltage
     while m.clock() < 3:
@@ -60,12 +60,12 @@ def test_inject_current(torch_gpu, symmetric):
     assert init_Q + delta_Q == pytest.approx(new_Q, rel=1e-5)
 
 
-@pytest.mark.parametrize("symmetric", [False, True])
-def test_length_constant(torch_gpu, symmetric):
+@pytest.mark.parametrize("store", [None, "symmetric", "general"])
+def test_length_constant(torch_gpu, store):
     """test_electric.py:48-94 (case 1): attenuation 1/e at one length constant."""
     from neuwon_amd.model import RxD_Model
     diam, rm, max_len, dt, ptol = 1, 4e11, 2, .01, .05
-    m = RxD_Model(symmetric_propagator=symmetric)
+    m = RxD_Model(propagator_store=store)
     root = m.Neuron([0, 0, 0], diam).root
     section = [root] + root.add_section([3000, 0, 0], diam, max_len)
     m.advance()
@@ -87,10 +87,10 @@ def test_length_constant(torch_gpu, symmetric):
     assert ratio == pytest.approx(1 / np.e, ptol)
 
 
-def _golden_model(mechanism, symmetric=False):
+def _golden_model(mechanism, store=None):
     from neuwon_amd.model import RxD_Model, TimeSeries
     m = RxD_Model(time_step=.1, temperature=6.3, mechanisms=[mechanism],
-                  symmetric_propagator=symmetric)
+                  propagator_store=store)
     hh = m.mechanisms[mechanism]
     root = tip = m.Neuron([-1, 0, 7], 5.7).root
     hh(root)
@@ -109,12 +109,12 @@ def _golden_model(mechanism, symmetric=False):
     return np.array(x.get_data())
 
 
-@pytest.mark.parametrize("mechanism,symmetric", [("hh", False), ("hh_kinetic", False),
-                                                ("hh", True)])
-def test_model_hh_golden(torch_gpu, mechanism, symmetric):
+@pytest.mark.parametrize("mechanism,store", [("hh", None), ("hh_kinetic", None),
+                                            ("hh", "symmetric"), ("hh", "general")])
+def test_model_hh_golden(torch_gpu, mechanism, store):
     """test_model.py:35-128 through the drop-in API, against the golden trace and the oracle."""
     golden = np.loadtxt(os.path.join(GOLDEN, "hh_trace_test_model.txt"))
-    v = _golden_model(mechanism, symmetric)
+    v = _golden_model(mechanism, store)
     assert len(v) == 201
     err = np.abs(v - golden)
     _say(f"{mechanism}: max |v - golden| = {err.max():.3f} mV")
