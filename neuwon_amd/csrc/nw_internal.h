@@ -393,6 +393,15 @@ struct nw_propagator {
     int64_t* cheb_off = nullptr;   // [num_cells] first coefficient of the cell
     double* cheb_c = nullptr;      // coefficients of all cells
     double* cheb_v64 = nullptr;    // [num_rows] voltages between the launches of a step
+    // Cells whose rows have at most 3 neighbours (every binary tree) are also stored in depth-first
+    // preorder, a first child right after its parent: pre_pos[i] is the preorder place of natural
+    // row i, and rows in preorder hold their neighbours' places (pre_idx: slots 0 | 1 << 16, slot 2)
+    // and coefficients (pre_coef: slots 0, 1, 2, unused); unused slots name the row itself with a
+    // zero coefficient.  pre_cell[c] is 1 for such cells, 0 for cells read in natural order.
+    uint16_t* pre_pos = nullptr;   // [num_rows]
+    uint2* pre_idx = nullptr;      // [num_rows]
+    float4* pre_coef = nullptr;    // [num_rows]
+    uint8_t* pre_cell = nullptr;   // [num_cells]
     double mean_terms = 0.0;
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;
@@ -411,11 +420,15 @@ struct ChebStore {
     const int32_t* K;
     const int64_t* off;
     const double* c;
+    const uint16_t* pre_pos;
+    const uint2* pre_idx;
+    const float4* pre_coef;
+    const uint8_t* pre_cell;
 };
 
 inline ChebStore cheb_store(const nw_propagator* p) {
     return ChebStore{p->nb_idx, p->nb_coef, p->ov_idx, p->ov_coef, p->cheb_K, p->cheb_off,
-                     p->cheb_c};
+                     p->cheb_c, p->pre_pos, p->pre_idx, p->pre_coef, p->pre_cell};
 }
 
 // Rows per thread of the Chebyshev products (cells up to kChebBlock * kChebRows segments).
@@ -478,4 +491,133 @@ __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const
         nxt = t;
     }
     __syncthreads();
+}
+
+// Chebyshev products over cells stored in preorder (rows with at most 3 neighbours).  Each thread
+// owns rows q = threadIdx.x + j * kChebBlock, j < J, and keeps their last two terms in registers;
+// LDS holds every row's last term for its neighbours, in two buffers of kChebBuf doubles used in
+// turn.  T_k = 2 A~ T_{k-1} - T_{k-2} is formed as
+//   2 d_q T_{k-1}(q) + sum_s 2 a_qs T_{k-1}(s) - T_{k-2}(q),   d_q = 1 - sum_s a_qs (FP64),
+// the FP32 coefficients a_qs = (2 / rho) b_qs widened to FP64 and doubled exactly, so constant
+// vectors stay constant to FP64 rounding as in the difference form.
+constexpr int kChebBuf = kChebBlock * kChebRows;
+// Series coefficients staged in LDS after the two buffers (cells of more terms are read in natural
+// order); LDS of a Chebyshev launch: 2 kChebBuf + kChebCoefCap doubles.
+constexpr int kChebCoefCap = 1024;
+constexpr int kChebLds = 2 * kChebBuf + kChebCoefCap;
+
+template <int J>
+struct ChebRegs {
+    uint32_t a[J][3];   // LDS byte offsets of the neighbours' values
+    double c[J][3];     // 2 a_qs
+    double d2[J];       // 2 d_q
+    double t[J];        // T_{k-1}(q)
+    double tp[J];       // T_{k-2}(q)
+    bool last;          // (wave-uniform) the wave has a row below n in row group J - 1
+};
+
+template <int J, bool FIRST>
+__device__ __forceinline__ double cheb_row(ChebRegs<J>& R, int j, const double (&nb)[3], double ck,
+                                           double* wr, double (&out)[kChebRows]) {
+    double acc = FIRST ? R.d2[j] * R.t[j] : fma(R.d2[j], R.t[j], -R.tp[j]);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) acc = fma(R.c[j][s], nb[s], acc);
+    if (FIRST) acc *= 0.5;
+    wr[threadIdx.x + j * kChebBlock] = acc;
+    out[j] = fma(ck, acc, out[j]);
+    R.tp[j] = R.t[j];
+    R.t[j] = acc;
+    return acc;
+}
+
+// One term: neighbours' T_{k-1} from buffer RD, T_k written to the other buffer, out += c_k T_k,
+// then a barrier.  FIRST: T_1 = A~ T_0.  A wave whose rows of group J - 1 all lie past the cell
+// skips that group.
+template <int J, int RD, bool FIRST>
+__device__ __forceinline__ void cheb_term(double* lds, ChebRegs<J>& R, int k, double (&out)[kChebRows]) {
+    const char* rd = reinterpret_cast<const char*>(lds + RD * kChebBuf);
+    double* wr = lds + (1 - RD) * kChebBuf;
+    const double ck = lds[2 * kChebBuf + k];
+    double nb[J][3];
+#pragma unroll
+    for (int j = 0; j < J - 1; ++j)
+#pragma unroll
+        for (int s = 0; s < 3; ++s) nb[j][s] = *reinterpret_cast<const double*>(rd + R.a[j][s]);
+    if (R.last) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            nb[J - 1][s] = *reinterpret_cast<const double*>(rd + R.a[J - 1][s]);
+    }
+#pragma unroll
+    for (int j = 0; j < J - 1; ++j) cheb_row<J, FIRST>(R, j, nb[j], ck, wr, out);
+    if (R.last) cheb_row<J, FIRST>(R, J - 1, nb[J - 1], ck, wr, out);
+    __syncthreads();
+}
+
+// The product for the rows of one preorder cell of n rows (n <= J * kChebBlock, K < kChebCoefCap).
+// On entry lds[q] = x of preorder row q for q < n (after a barrier); on return out[j] holds the
+// result of row threadIdx.x + j * kChebBlock, and the call has ended with a barrier.
+template <int J>
+__device__ __forceinline__ void cheb_pre_rows(const uint2* __restrict__ idx, const float4* __restrict__ cf,
+                                              const double* __restrict__ c, int K, int n, double* lds,
+                                              double (&out)[kChebRows]) {
+    for (int k = threadIdx.x; k <= K; k += kChebBlock) lds[2 * kChebBuf + k] = c[k];
+    ChebRegs<J> R;
+    const int wave_row = __builtin_amdgcn_readfirstlane(threadIdx.x & ~(nw::kWave - 1));
+    R.last = wave_row + (J - 1) * kChebBlock < n;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int q = threadIdx.x + j * kChebBlock;
+        const uint32_t self = 8u * (uint32_t)q;
+        if (q < n) {
+            const uint2 w = idx[q];
+            const float4 f = cf[q];
+            R.a[j][0] = 8u * (w.x & 0xFFFFu);
+            R.a[j][1] = 8u * (w.x >> 16);
+            R.a[j][2] = 8u * w.y;
+            R.c[j][0] = 2.0 * (double)f.x;
+            R.c[j][1] = 2.0 * (double)f.y;
+            R.c[j][2] = 2.0 * (double)f.z;
+            R.d2[j] = 2.0 * (1.0 - ((double)f.x + (double)f.y + (double)f.z));
+            R.t[j] = lds[q];
+        } else {  // padding rows: zero, reading only themselves
+            R.a[j][0] = R.a[j][1] = R.a[j][2] = self;
+            R.c[j][0] = R.c[j][1] = R.c[j][2] = 0.0;
+            R.d2[j] = 0.0;
+            R.t[j] = 0.0;
+            lds[q] = 0.0;
+        }
+        R.tp[j] = 0.0;
+    }
+    __syncthreads();
+    const double c0 = lds[2 * kChebBuf];
+#pragma unroll
+    for (int j = 0; j < J; ++j) out[j] = c0 * R.t[j];
+    if (K >= 1) cheb_term<J, 0, true>(lds, R, 1, out);
+    int k = 2;
+    for (; k + 1 <= K; k += 2) {
+        cheb_term<J, 1, false>(lds, R, k, out);
+        cheb_term<J, 0, false>(lds, R, k + 1, out);
+    }
+    if (k <= K) cheb_term<J, 1, false>(lds, R, k, out);
+}
+
+// The product for one cell, preorder (PRE) or natural order, from x in lds[0..n) to out[].
+__device__ __forceinline__ void cheb_cell(const ChebStore& cs, int64_t row0, int cell, int n, bool pre,
+                                          double* lds, double (&out)[kChebRows]) {
+    const double* c = cs.c + cs.off[cell];
+    const int K = cs.K[cell];
+    if (!pre) {
+        cheb_rows(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef, c, K, n, lds,
+                  lds + kChebBuf, out);
+        return;
+    }
+    const uint2* idx = cs.pre_idx + row0;
+    const float4* cf = cs.pre_coef + row0;
+    const int J = (n + kChebBlock - 1) / kChebBlock;
+    if (J <= 1) cheb_pre_rows<1>(idx, cf, c, K, n, lds, out);
+    else if (J == 2) cheb_pre_rows<2>(idx, cf, c, K, n, lds, out);
+    else if (J == 3) cheb_pre_rows<3>(idx, cf, c, K, n, lds, out);
+    else if (J == 4) cheb_pre_rows<4>(idx, cf, c, K, n, lds, out);
+    else cheb_pre_rows<5>(idx, cf, c, K, n, lds, out);
 }

@@ -289,13 +289,13 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 template <int GATE_MODE, int HALF>
 __global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v64, int step,
                                                              int from_v32) {
-    extern __shared__ double lds64[];
-    double* b0 = lds64;
-    double* b1 = lds64 + a.nmax;
+    __shared__ double b0[kChebLds];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int nrows = (int)(a.cell_row0[cell + 1] - row0);
+    const bool pre = a.cheb.pre_cell[cell] != 0;
+    const uint16_t* pos = a.cheb.pre_pos + row0;
     const double hdt = 0.5 * (double)p.dt_ms;
     const double hdt_s = hdt * 1e-3;
     const int64_t gstep = p.step0 + step;
@@ -333,27 +333,28 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v
         const double invc = a.inv_c[gi];
         const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick) : 0.0;
         const double alpha = exp(-hdt_s * g * invc);
-        b0[i] = d - (d - v) * alpha + hdt * I * invc;
+        b0[pre ? pos[i] : i] = d - (d - v) * alpha + hdt * I * invc;
     }
     __syncthreads();
     double out[kChebRows];
-    cheb_rows(a.cheb.nb_idx + 3 * row0, a.cheb.nb_coef + 3 * row0, a.cheb.ov_idx, a.cheb.ov_coef,
-              a.cheb.c + a.cheb.off[cell], a.cheb.K[cell], nrows, b0, b1, out);
+    cheb_cell(a.cheb, row0, cell, nrows, pre, b0, out);
+    // results to LDS (slot of the row in the cell's stored order), then out in natural order
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
-        if (r < nrows) {
-            v64[row0 + r] = out[j];
-            if (HALF == 1) a.v[row0 + r] = (float)out[j];
-            b0[r] = out[j];
-        }
+        if (r < nrows) b0[r] = out[j];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+        const double vi = b0[pre ? pos[i] : i];
+        v64[row0 + i] = vi;
+        if (HALF == 1) a.v[row0 + i] = (float)vi;
     }
     if (HALF == 1) {
         const int ap_row = a.ap_local ? a.ap_local[cell] : -1;
         if (ap_row >= 0) {
-            __syncthreads();
             if (threadIdx.x == 0) {
-                const double vap = b0[ap_row];
+                const double vap = b0[pre ? pos[ap_row] : ap_row];
                 const bool over = vap >= (double)p.ap_threshold;
                 if (over && !a.ap_state[cell]) ++a.ap_count[cell];
                 a.ap_state[cell] = over ? 1 : 0;
@@ -958,15 +959,17 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
         NW_REQUIRE(prm->block_threads == 0 || prm->block_threads == kChebBlock,
                    "the Chebyshev store runs %d threads per workgroup", kChebBlock);
         NW_REQUIRE(record == nullptr, "the Chebyshev store keeps no record for the adjoint");
+        NW_REQUIRE(p->max_cell_rows <= kChebBuf, "cell of %d segments exceeds the Chebyshev store",
+                   p->max_cell_rows);
         const unsigned grid = (unsigned)p->num_cells;
         double* v64 = p->cheb_v64;
         for (int st = 0; st < n_steps; ++st) {
             if (prm->gate_mode == 1) {
-                k_hh_half_cheb<1, 0><<<grid, kChebBlock, lds, s>>>(a, v64, st, st == 0);
-                k_hh_half_cheb<1, 1><<<grid, kChebBlock, lds, s>>>(a, v64, st, 0);
+                k_hh_half_cheb<1, 0><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
+                k_hh_half_cheb<1, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
             } else {
-                k_hh_half_cheb<0, 0><<<grid, kChebBlock, lds, s>>>(a, v64, st, st == 0);
-                k_hh_half_cheb<0, 1><<<grid, kChebBlock, lds, s>>>(a, v64, st, 0);
+                k_hh_half_cheb<0, 0><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
+                k_hh_half_cheb<0, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
             }
         }
         NW_HIP(hipGetLastError());

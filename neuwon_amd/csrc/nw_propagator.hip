@@ -513,6 +513,12 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     std::vector<double> coef;
     std::vector<std::vector<std::pair<int, double>>> adj;
     std::vector<double> work;
+    // preorder copies (cells whose rows have at most 3 neighbours)
+    std::vector<uint16_t> ppos(N, 0);
+    std::vector<uint2> pidx(N, uint2{0u, 0u});
+    std::vector<float4> pcoef(N, float4{0.0f, 0.0f, 0.0f, 0.0f});
+    std::vector<uint8_t> pcell(p->num_cells, 0);
+    std::vector<int32_t> stack, place;
     double terms_rows = 0.0;
     int max_deg = 0;
     int64_t edges = 0;
@@ -544,6 +550,39 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
         p->h_cheb_c.insert(p->h_cheb_c.end(), coef.begin(), coef.end());
         terms_rows += (double)(coef.size() - 1) * n;
         const double scale = rho > 0.0 ? 2.0 / rho : 0.0;
+        int cell_deg = 0;
+        for (int i = 0; i < n; ++i) cell_deg = std::max<int>(cell_deg, (int)adj[i].size());
+        if (cell_deg <= 3 && n > 0 && (int)coef.size() <= kChebCoefCap) {
+            // depth-first preorder, children in increasing index order: a first child's place is
+            // its parent's place + 1.  adj[i] lists the parent (an earlier index) before the
+            // children, so the slots are parent, first child, second child.
+            pcell[cell] = 1;
+            place.assign(n, -1);
+            stack.assign(1, 0);
+            int next = 0;
+            while (!stack.empty()) {
+                const int i = stack.back();
+                stack.pop_back();
+                place[i] = next++;
+                for (int d = (int)adj[i].size() - 1; d >= 0; --d)
+                    if (adj[i][d].first > i) stack.push_back(adj[i][d].first);
+            }
+            NW_REQUIRE(next == n, "cell %lld is not one tree", (long long)cell);
+            for (int i = 0; i < n; ++i) {
+                const int q = place[i];
+                ppos[r0 + i] = (uint16_t)q;
+                uint32_t slot[3] = {(uint32_t)q, (uint32_t)q, (uint32_t)q};
+                float co[3] = {0.0f, 0.0f, 0.0f};
+                for (size_t d = 0; d < adj[i].size(); ++d) {
+                    slot[d] = (uint32_t)place[adj[i][d].first];
+                    co[d] = (float)(scale * adj[i][d].second);
+                }
+                pidx[r0 + q] = uint2{slot[0] | (slot[1] << 16), slot[2]};
+                pcoef[r0 + q] = float4{co[0], co[1], co[2], 0.0f};
+            }
+        } else {
+            for (int i = 0; i < n; ++i) ppos[r0 + i] = (uint16_t)i;
+        }
         for (int i = 0; i < n; ++i) {
             const int64_t g = r0 + i;
             const auto& a = adj[i];
@@ -595,13 +634,23 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
                      hipMemcpyHostToDevice));
     NW_HIP(hipMemcpy(p->cheb_c, p->h_cheb_c.data(), sizeof(double) * p->h_cheb_c.size(),
                      hipMemcpyHostToDevice));
+    NW_HIP(hipMalloc(&p->pre_pos, sizeof(uint16_t) * std::max<int64_t>(N, 1)));
+    NW_HIP(hipMalloc(&p->pre_idx, sizeof(uint2) * std::max<int64_t>(N, 1)));
+    NW_HIP(hipMalloc(&p->pre_coef, sizeof(float4) * std::max<int64_t>(N, 1)));
+    NW_HIP(hipMalloc(&p->pre_cell, std::max<int64_t>(p->num_cells, 1)));
+    NW_HIP(hipMemcpy(p->pre_pos, ppos.data(), sizeof(uint16_t) * N, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_idx, pidx.data(), sizeof(uint2) * N, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_coef, pcoef.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_cell, pcell.data(), p->num_cells, hipMemcpyHostToDevice));
+    // launch order: most work first (series terms x rows per thread, then rows)
     std::vector<int32_t> order(p->num_cells);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-        const double wa = (double)p->h_cheb_K[a] * (double)(cell_ptr[a + 1] - cell_ptr[a]);
-        const double wb = (double)p->h_cheb_K[b] * (double)(cell_ptr[b + 1] - cell_ptr[b]);
-        return wa > wb;
-    });
+    auto work_of = [&](int32_t c) {
+        const int64_t n = cell_ptr[c + 1] - cell_ptr[c];
+        return std::make_pair((int64_t)p->h_cheb_K[c] * ((n + kChebBlock - 1) / kChebBlock), n);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return work_of(a) > work_of(b); });
     NW_HIP(hipMalloc(&p->order, sizeof(int32_t) * p->num_cells));
     NW_HIP(hipMemcpy(p->order, order.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
     return NW_OK;
@@ -617,6 +666,10 @@ static void free_prop(nw_propagator* p) {
     if (p->cheb_off) (void)hipFree(p->cheb_off);
     if (p->cheb_c) (void)hipFree(p->cheb_c);
     if (p->cheb_v64) (void)hipFree(p->cheb_v64);
+    if (p->pre_pos) (void)hipFree(p->pre_pos);
+    if (p->pre_idx) (void)hipFree(p->pre_idx);
+    if (p->pre_coef) (void)hipFree(p->pre_coef);
+    if (p->pre_cell) (void)hipFree(p->pre_cell);
     if (p->cell_row0) (void)hipFree(p->cell_row0);
     if (p->cell_nz0) (void)hipFree(p->cell_nz0);
     if (p->row_off) (void)hipFree(p->row_off);
@@ -827,24 +880,26 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
 
 // Chebyshev store: x in LDS as FP64, v = sum_k c_k T_k(A~) x (cheb_rows).
 __global__ void __launch_bounds__(kChebBlock) k_half_step_cheb(HalfArgs a, ChebStore cs) {
-    extern __shared__ double lds64[];
+    __shared__ double b0[kChebLds];
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int n = (int)(a.cell_row0[cell + 1] - row0);
     const double dt_ms = a.half_dt_ms;
     const double dt_s = dt_ms * 1e-3;
-    double* b0 = lds64;
-    double* b1 = lds64 + a.nmax;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) b0[i] = half_x(a, row0 + i, dt_ms, dt_s, true);
+    const bool pre = cs.pre_cell[cell] != 0;
+    const uint16_t* pos = cs.pre_pos + row0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        b0[pre ? pos[i] : i] = half_x(a, row0 + i, dt_ms, dt_s, true);
     __syncthreads();
     double out[kChebRows];
-    cheb_rows(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef,
-              cs.c + cs.off[cell], cs.K[cell], n, b0, b1, out);
+    cheb_cell(cs, row0, cell, n, pre, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
-        if (r < n) a.v[row0 + r] = (float)out[j];
+        if (r < n) b0[r] = out[j];
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a.v[row0 + i] = (float)b0[pre ? pos[i] : i];
 }
 
 }  // namespace nw
@@ -1028,8 +1083,9 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
     HalfArgs a{p->cell_row0, p->cell_nz0, p->row_off, p->vals, p->cols, p->rowsum, p->order,
                p->max_cell_rows, half_dt_ms, v, sum_g, drive_v, sum_i, inv_c, integral_v, p->wgt};
     if (p->chebyshev) {
-        const size_t lds = 2 * sizeof(double) * (size_t)p->max_cell_rows;
-        k_half_step_cheb<<<(unsigned)p->num_cells, kChebBlock, lds, (hipStream_t)stream>>>(
+        NW_REQUIRE(p->max_cell_rows <= kChebBuf, "cell of %d segments exceeds the Chebyshev store",
+                   p->max_cell_rows);
+        k_half_step_cheb<<<(unsigned)p->num_cells, kChebBlock, 0, (hipStream_t)stream>>>(
             a, cheb_store(p));
         NW_HIP(hipGetLastError());
         return NW_OK;
