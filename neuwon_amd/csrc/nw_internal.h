@@ -393,15 +393,20 @@ struct nw_propagator {
     int64_t* cheb_off = nullptr;   // [num_cells] first coefficient of the cell
     double* cheb_c = nullptr;      // coefficients of all cells
     double* cheb_v64 = nullptr;    // [num_rows] voltages between the launches of a step
-    // Cells whose rows have at most 3 neighbours (every binary tree) are also stored in depth-first
-    // preorder, a first child right after its parent: pre_pos[i] is the preorder place of natural
-    // row i, and rows in preorder hold their neighbours' places (pre_idx: slots 0 | 1 << 16, slot 2)
-    // and coefficients (pre_coef: slots 0, 1, 2, unused); unused slots name the row itself with a
-    // zero coefficient.  pre_cell[c] is 1 for such cells, 0 for cells read in natural order.
+    // Cells whose rows have at most 3 neighbours (every binary tree) also run in depth-first
+    // preorder, a first child right after its parent, laid out in slots: thread t of the workgroup
+    // owns slots J t .. J t + J - 1 (J = ceil(slots / 1024)), slot s = J t + j lives at LDS place
+    // j * 1024 + t.  A few slots stay empty (holes) so no row needs more than two LDS reads per
+    // term.  pre_pos[i]: LDS place of natural row i.  Per LDS place of the cell (from pre_base[c]):
+    // pre_far = places of the row's two far neighbours (| second << 16; kChebHole for an empty
+    // slot), pre_coef = (2/rho) b of the neighbour in slot s - 1, in slot s + 1, and of the two far
+    // ones.  pre_base[c] = -1: the cell runs in natural order.
     uint16_t* pre_pos = nullptr;   // [num_rows]
-    uint2* pre_idx = nullptr;      // [num_rows]
-    float4* pre_coef = nullptr;    // [num_rows]
-    uint8_t* pre_cell = nullptr;   // [num_cells]
+    uint32_t* pre_far = nullptr;   // [sum of 1024 J over preorder cells]
+    float4* pre_coef = nullptr;    // [same]
+    int64_t* pre_base = nullptr;   // [num_cells]
+    int32_t* pre_size = nullptr;   // [num_cells] slots in use (rows and holes) | J << 16
+    int64_t pre_cells = 0;         // cells laid out in preorder
     double mean_terms = 0.0;
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;
@@ -421,14 +426,15 @@ struct ChebStore {
     const int64_t* off;
     const double* c;
     const uint16_t* pre_pos;
-    const uint2* pre_idx;
+    const uint32_t* pre_far;
     const float4* pre_coef;
-    const uint8_t* pre_cell;
+    const int64_t* pre_base;
+    const int32_t* pre_size;
 };
 
 inline ChebStore cheb_store(const nw_propagator* p) {
     return ChebStore{p->nb_idx, p->nb_coef, p->ov_idx, p->ov_coef, p->cheb_K, p->cheb_off,
-                     p->cheb_c, p->pre_pos, p->pre_idx, p->pre_coef, p->pre_cell};
+                     p->cheb_c, p->pre_pos, p->pre_far, p->pre_coef, p->pre_base, p->pre_size};
 }
 
 // Rows per thread of the Chebyshev products (cells up to kChebBlock * kChebRows segments).
@@ -493,131 +499,160 @@ __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const
     __syncthreads();
 }
 
-// Chebyshev products over cells stored in preorder (rows with at most 3 neighbours).  Each thread
-// owns rows q = threadIdx.x + j * kChebBlock, j < J, and keeps their last two terms in registers;
-// LDS holds every row's last term for its neighbours, in two buffers of kChebBuf doubles used in
-// turn.  T_k = 2 A~ T_{k-1} - T_{k-2} is formed as
-//   2 d_q T_{k-1}(q) + sum_s 2 a_qs T_{k-1}(s) - T_{k-2}(q),   d_q = 1 - sum_s a_qs (FP64),
-// the FP32 coefficients a_qs = (2 / rho) b_qs widened to FP64 and doubled exactly, so constant
-// vectors stay constant to FP64 rounding as in the difference form.
+// Chebyshev products over cells laid out in preorder slots.  Thread t owns the J consecutive
+// slots s = J t + j and keeps their last two terms in FP64 registers; the term of slot s is
+// written to LDS place j * 1024 + t.  T_k = 2 A~ T_{k-1} - T_{k-2} is formed as
+//   2 d T_{k-1}(s) + 2 a_l T_{k-1}(s-1) + 2 a_r T_{k-1}(s+1) + 2 a_f T_{k-1}(f) + 2 a_g T_{k-1}(g)
+//   - T_{k-2}(s),   d = 1 - (a_l + a_r + a_f + a_g) in FP64,
+// with s - 1 and s + 1 taken from the thread's own registers or, across threads of a wave, from
+// the neighbouring lane (DPP row shifts of the whole wave), and the far neighbours f, g read from
+// the LDS vector of the previous term.  The FP32 coefficients a = (2/rho) b are widened to FP64
+// and doubled exactly, so constant vectors stay constant to FP64 rounding.
 constexpr int kChebBuf = kChebBlock * kChebRows;
 // Series coefficients staged in LDS after the two buffers (cells of more terms are read in natural
 // order); LDS of a Chebyshev launch: 2 kChebBuf + kChebCoefCap doubles.
 constexpr int kChebCoefCap = 1024;
 constexpr int kChebLds = 2 * kChebBuf + kChebCoefCap;
+// LDS place of the zero that unused far slots read, in both buffers: the last slot of a cell of
+// kChebRows row groups, which the layout leaves empty.
+constexpr int kChebZero = kChebBuf - 1;
+constexpr uint32_t kChebHole = 0xFFFFFFFFu;
 
-template <int J>
-struct ChebRegs {
-    uint32_t a[J][3];   // LDS byte offsets of the neighbours' values
-    double c[J][3];     // 2 a_qs
-    double d2[J];       // 2 d_q
-    double t[J];        // T_{k-1}(q)
-    double tp[J];       // T_{k-2}(q)
-    bool last;          // (wave-uniform) the wave has a row below n in row group J - 1
-};
-
-template <int J, bool FIRST>
-__device__ __forceinline__ double cheb_row(ChebRegs<J>& R, int j, const double (&nb)[3], double ck,
-                                           double* wr, double (&out)[kChebRows]) {
-    double acc = FIRST ? R.d2[j] * R.t[j] : fma(R.d2[j], R.t[j], -R.tp[j]);
-#pragma unroll
-    for (int s = 0; s < 3; ++s) acc = fma(R.c[j][s], nb[s], acc);
-    if (FIRST) acc *= 0.5;
-    wr[threadIdx.x + j * kChebBlock] = acc;
-    out[j] = fma(ck, acc, out[j]);
-    R.tp[j] = R.t[j];
-    R.t[j] = acc;
-    return acc;
+// v of lane l - 1 (wave_shr:1; lane 0 receives 0) and of lane l + 1 (wave_shl:1; lane 63: 0).
+__device__ __forceinline__ double lane_before(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double lane_after(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// One term: neighbours' T_{k-1} from buffer RD, T_k written to the other buffer, out += c_k T_k,
-// then a barrier.  FIRST: T_1 = A~ T_0.  A wave whose rows of group J - 1 all lie past the cell
-// skips that group.
-template <int J, int RD, bool FIRST>
-__device__ __forceinline__ void cheb_term(double* lds, ChebRegs<J>& R, int k, double (&out)[kChebRows]) {
-    const char* rd = reinterpret_cast<const char*>(lds + RD * kChebBuf);
-    double* wr = lds + (1 - RD) * kChebBuf;
-    const double ck = lds[2 * kChebBuf + k];
-    double nb[J][3];
+// Registers of a thread's J slots; CT holds the doubled coefficients (FP64, or FP32 where the
+// registers of 5 slots do not leave room).
+template <int J, typename CT>
+struct ChebRegs {
+    uint32_t f[J], g[J];   // LDS byte offsets of the far neighbours
+    CT cl[J], cr[J], cf[J], cg[J];
+    double d2[J];          // 2 d
+};
+
+// One term: t holds T_{k-1} of the thread's slots, u holds T_{k-2} and receives T_k (the two
+// arrays swap roles from term to term, so no register is copied); far neighbours from buffer RD,
+// T_k to the other buffer, out += c_k T_k, then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
+// (wave-uniform): the wave holds a slot in use; a wave without one only keeps the barriers.
+template <int J, typename CT, int RD, bool FIRST>
+__device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R, const double (&t)[J],
+                                          double (&u)[J], bool active, int k, double (&out)[kChebRows]) {
+    if (active) {
+        const char* rd = reinterpret_cast<const char*>(lds + RD * kChebBuf);
+        double* wr = lds + (1 - RD) * kChebBuf;
+        const double ck = lds[2 * kChebBuf + k];
+        double vf[J], vg[J];
 #pragma unroll
-    for (int j = 0; j < J - 1; ++j)
+        for (int j = 0; j < J; ++j) {
+            vf[j] = *reinterpret_cast<const double*>(rd + R.f[j]);
+            vg[j] = *reinterpret_cast<const double*>(rd + R.g[j]);
+        }
+        const double before = lane_before(t[J - 1]);
+        const double after = lane_after(t[0]);
 #pragma unroll
-        for (int s = 0; s < 3; ++s) nb[j][s] = *reinterpret_cast<const double*>(rd + R.a[j][s]);
-    if (R.last) {
+        for (int j = 0; j < J; ++j) {
+            const double tl = j > 0 ? t[j - 1] : before;
+            const double tr = j < J - 1 ? t[j + 1] : after;
+            double acc = FIRST ? R.d2[j] * t[j] : fma(R.d2[j], t[j], -u[j]);
+            acc = fma((double)R.cl[j], tl, acc);
+            acc = fma((double)R.cr[j], tr, acc);
+            acc = fma((double)R.cf[j], vf[j], acc);
+            acc = fma((double)R.cg[j], vg[j], acc);
+            u[j] = FIRST ? 0.5 * acc : acc;
+        }
 #pragma unroll
-        for (int s = 0; s < 3; ++s)
-            nb[J - 1][s] = *reinterpret_cast<const double*>(rd + R.a[J - 1][s]);
+        for (int j = 0; j < J; ++j) {
+            wr[threadIdx.x + j * kChebBlock] = u[j];
+            out[j] = fma(ck, u[j], out[j]);
+        }
     }
-#pragma unroll
-    for (int j = 0; j < J - 1; ++j) cheb_row<J, FIRST>(R, j, nb[j], ck, wr, out);
-    if (R.last) cheb_row<J, FIRST>(R, J - 1, nb[J - 1], ck, wr, out);
     __syncthreads();
 }
 
-// The product for the rows of one preorder cell of n rows (n <= J * kChebBlock, K < kChebCoefCap).
-// On entry lds[q] = x of preorder row q for q < n (after a barrier); on return out[j] holds the
-// result of row threadIdx.x + j * kChebBlock, and the call has ended with a barrier.
-template <int J>
-__device__ __forceinline__ void cheb_pre_rows(const uint2* __restrict__ idx, const float4* __restrict__ cf,
-                                              const double* __restrict__ c, int K, int n, double* lds,
+// The product for one preorder cell of `size` slots in J row groups (size <= 1024 J,
+// K < kChebCoefCap).
+// On entry lds[j * 1024 + t] = x of the row in slot J t + j (after a barrier); on return out[j]
+// holds that row's result and the call has ended with a barrier.
+template <int J, typename CT>
+__device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, const float4* __restrict__ cf,
+                                              const double* __restrict__ c, int K, int size, double* lds,
                                               double (&out)[kChebRows]) {
     for (int k = threadIdx.x; k <= K; k += kChebBlock) lds[2 * kChebBuf + k] = c[k];
-    ChebRegs<J> R;
-    const int wave_row = __builtin_amdgcn_readfirstlane(threadIdx.x & ~(nw::kWave - 1));
-    R.last = wave_row + (J - 1) * kChebBlock < n;
+    if (threadIdx.x == 0) {
+        lds[kChebZero] = 0.0;
+        lds[kChebBuf + kChebZero] = 0.0;
+    }
+    ChebRegs<J, CT> R;
+    double x[J], y[J];
+    const int wave_slot0 = __builtin_amdgcn_readfirstlane(threadIdx.x & ~(nw::kWave - 1)) * J;
+    const bool active = wave_slot0 < size;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int q = threadIdx.x + j * kChebBlock;
-        const uint32_t self = 8u * (uint32_t)q;
-        if (q < n) {
-            const uint2 w = idx[q];
-            const float4 f = cf[q];
-            R.a[j][0] = 8u * (w.x & 0xFFFFu);
-            R.a[j][1] = 8u * (w.x >> 16);
-            R.a[j][2] = 8u * w.y;
-            R.c[j][0] = 2.0 * (double)f.x;
-            R.c[j][1] = 2.0 * (double)f.y;
-            R.c[j][2] = 2.0 * (double)f.z;
-            R.d2[j] = 2.0 * (1.0 - ((double)f.x + (double)f.y + (double)f.z));
-            R.t[j] = lds[q];
-        } else {  // padding rows: zero, reading only themselves
-            R.a[j][0] = R.a[j][1] = R.a[j][2] = self;
-            R.c[j][0] = R.c[j][1] = R.c[j][2] = 0.0;
+        const uint32_t w = far[q];
+        if (w != kChebHole) {
+            const float4 a = cf[q];
+            R.f[j] = 8u * (w & 0xFFFFu);
+            R.g[j] = 8u * (w >> 16);
+            R.cl[j] = (CT)(2.0f * a.x);
+            R.cr[j] = (CT)(2.0f * a.y);
+            R.cf[j] = (CT)(2.0f * a.z);
+            R.cg[j] = (CT)(2.0f * a.w);
+            R.d2[j] = 2.0 * (1.0 - (((double)a.x + (double)a.y) + ((double)a.z + (double)a.w)));
+            x[j] = lds[q];
+        } else {  // hole or padding: zero, read by nobody through LDS
+            R.f[j] = R.g[j] = 8u * kChebZero;
+            R.cl[j] = R.cr[j] = R.cf[j] = R.cg[j] = (CT)0;
             R.d2[j] = 0.0;
-            R.t[j] = 0.0;
-            lds[q] = 0.0;
+            x[j] = 0.0;
         }
-        R.tp[j] = 0.0;
+        y[j] = 0.0;
     }
     __syncthreads();
     const double c0 = lds[2 * kChebBuf];
 #pragma unroll
-    for (int j = 0; j < J; ++j) out[j] = c0 * R.t[j];
-    if (K >= 1) cheb_term<J, 0, true>(lds, R, 1, out);
+    for (int j = 0; j < J; ++j) out[j] = c0 * x[j];
+    // x = T_0, then y = T_1, x = T_2, y = T_3, ...
+    if (K >= 1) cheb_term<J, CT, 0, true>(lds, R, x, y, active, 1, out);
     int k = 2;
     for (; k + 1 <= K; k += 2) {
-        cheb_term<J, 1, false>(lds, R, k, out);
-        cheb_term<J, 0, false>(lds, R, k + 1, out);
+        cheb_term<J, CT, 1, false>(lds, R, y, x, active, k, out);
+        cheb_term<J, CT, 0, false>(lds, R, x, y, active, k + 1, out);
     }
-    if (k <= K) cheb_term<J, 1, false>(lds, R, k, out);
+    if (k <= K) cheb_term<J, CT, 1, false>(lds, R, y, x, active, k, out);
 }
 
-// The product for one cell, preorder (PRE) or natural order, from x in lds[0..n) to out[].
-__device__ __forceinline__ void cheb_cell(const ChebStore& cs, int64_t row0, int cell, int n, bool pre,
-                                          double* lds, double (&out)[kChebRows]) {
+// The product for one cell, preorder slots (base >= 0) or natural order, from x in LDS (at the
+// row's LDS place) to out[] (the result of the row at LDS place threadIdx.x + j * 1024).  Returns
+// the number of LDS places in use: out[j] is defined where threadIdx.x + j * 1024 is below it.
+__device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int cell, int n, int64_t base,
+                                         double* lds, double (&out)[kChebRows]) {
     const double* c = cs.c + cs.off[cell];
     const int K = cs.K[cell];
-    if (!pre) {
+    if (base < 0) {
         cheb_rows(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef, c, K, n, lds,
                   lds + kChebBuf, out);
-        return;
+        return n;
     }
-    const uint2* idx = cs.pre_idx + row0;
-    const float4* cf = cs.pre_coef + row0;
-    const int J = (n + kChebBlock - 1) / kChebBlock;
-    if (J <= 1) cheb_pre_rows<1>(idx, cf, c, K, n, lds, out);
-    else if (J == 2) cheb_pre_rows<2>(idx, cf, c, K, n, lds, out);
-    else if (J == 3) cheb_pre_rows<3>(idx, cf, c, K, n, lds, out);
-    else if (J == 4) cheb_pre_rows<4>(idx, cf, c, K, n, lds, out);
-    else cheb_pre_rows<5>(idx, cf, c, K, n, lds, out);
+    const uint32_t* far = cs.pre_far + base;
+    const float4* cf = cs.pre_coef + base;
+    const int size = cs.pre_size[cell] & 0xFFFF;
+    const int J = cs.pre_size[cell] >> 16;
+    if (J <= 1) cheb_pre_rows<1, double>(far, cf, c, K, size, lds, out);
+    else if (J == 2) cheb_pre_rows<2, double>(far, cf, c, K, size, lds, out);
+    else if (J == 3) cheb_pre_rows<3, double>(far, cf, c, K, size, lds, out);
+    else if (J == 4) cheb_pre_rows<4, double>(far, cf, c, K, size, lds, out);
+    else cheb_pre_rows<5, float>(far, cf, c, K, size, lds, out);
+    return J * kChebBlock;
 }

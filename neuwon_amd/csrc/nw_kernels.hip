@@ -294,7 +294,8 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int nrows = (int)(a.cell_row0[cell + 1] - row0);
-    const bool pre = a.cheb.pre_cell[cell] != 0;
+    const int64_t pre_base = a.cheb.pre_base[cell];
+    const bool pre = pre_base >= 0;
     const uint16_t* pos = a.cheb.pre_pos + row0;
     const double hdt = 0.5 * (double)p.dt_ms;
     const double hdt_s = hdt * 1e-3;
@@ -337,12 +338,12 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v
     }
     __syncthreads();
     double out[kChebRows];
-    cheb_cell(a.cheb, row0, cell, nrows, pre, b0, out);
-    // results to LDS (slot of the row in the cell's stored order), then out in natural order
+    const int places = cheb_cell(a.cheb, row0, cell, nrows, pre_base, b0, out);
+    // results to LDS (the row's place), then out in natural order
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
-        if (r < nrows) b0[r] = out[j];
+        if (r < places) b0[r] = out[j];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nrows; i += kChebBlock) {

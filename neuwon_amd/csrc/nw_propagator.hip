@@ -495,6 +495,90 @@ static double tree_spectral_radius(int n, const int32_t* parent, const double* d
     return hi * (1.0 + 1e-8);
 }
 
+// Preorder slot layout of one cell whose rows have at most three neighbours (adj[i]: the parent,
+// an earlier index, first, then the children in increasing order; coefficients already scaled by
+// 2 / rho).  Rows go in depth-first preorder, a first child right after its parent, into slots
+// s = J t + j (thread t, row group j, LDS place j * 1024 + t).  A row reaches the rows in slots
+// s - 1 and s + 1 without LDS (own registers, or the neighbouring lane of the same wave) unless
+// s is the first slot of a wave (s - 1 is in another wave) or the last (s + 1 is); every other
+// neighbour is "far" and read from LDS.  A row would need three far reads only in the last slot
+// of a wave, as a second child with two children: such a row moves to the next slot, leaving a
+// hole.  Returns the slots in use, or -1 when the layout does not fit (more than kChebRows row
+// groups, or the zero place kChebZero taken); sets J, fills pos (LDS place of every natural row)
+// and the 1024 J places of far/coef.
+static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, double>>>& adj,
+                           double scale, int& J_out, std::vector<uint16_t>& pos,
+                           std::vector<uint32_t>& far, std::vector<float4>& coef) {
+    std::vector<int32_t> place(n, -1), order(n), stack(1, 0);
+    int next = 0;
+    while (!stack.empty()) {
+        const int i = stack.back();
+        stack.pop_back();
+        place[i] = next;
+        order[next++] = i;
+        for (int d = (int)adj[i].size() - 1; d >= 0; --d)
+            if (adj[i][d].first > i) stack.push_back(adj[i][d].first);
+    }
+    if (next != n) return -1;
+    auto parent_of = [&](int i) { return (!adj[i].empty() && adj[i][0].first < i) ? adj[i][0].first : -1; };
+    std::vector<int32_t> slot(n);
+    int J = (n + kChebBlock - 1) / kChebBlock, size = 0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        const int W = kWave * J;
+        int s = -1;
+        for (int q = 0; q < n; ++q) {
+            int sn = s + 1;
+            if (q > 0 && sn % W == W - 1) {
+                const int i = order[q];
+                const int par = parent_of(i);
+                const int nch = (int)adj[i].size() - (par >= 0 ? 1 : 0);
+                const bool left_near = par >= 0 && place[par] == q - 1;
+                const int nfar = (par >= 0 && !left_near) + (nch >= 1) + (nch >= 2);
+                if (nfar > 2) ++sn;
+            }
+            slot[q] = sn;
+            s = sn;
+        }
+        size = s + 1;
+        // more row groups never need more holes, so J only grows
+        const int Jn = (size + kChebBlock - 1) / kChebBlock;
+        if (Jn <= J) break;
+        J = Jn;
+    }
+    if (size > J * kChebBlock || J > kChebRows || (J == kChebRows && size > kChebZero)) return -1;
+    J_out = J;
+    const size_t base = far.size();
+    far.resize(base + (size_t)J * kChebBlock, kChebHole);
+    coef.resize(base + (size_t)J * kChebBlock, float4{0.0f, 0.0f, 0.0f, 0.0f});
+    auto place_of_slot = [&](int s) { return (uint32_t)((s % J) * kChebBlock + s / J); };
+    for (int q = 0; q < n; ++q) {
+        const int i = order[q];
+        const int s = slot[q], t = s / J, j = s % J, lane = t % kWave;
+        const uint32_t pl = place_of_slot(s);
+        pos[i] = (uint16_t)pl;
+        float c4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t fp[2] = {(uint32_t)kChebZero, (uint32_t)kChebZero};
+        int nfar = 0;
+        for (const auto& e : adj[i]) {
+            const int sm = slot[place[e.first]];
+            const float a = (float)(scale * e.second);
+            if (sm == s - 1 && !(j == 0 && lane == 0)) {
+                c4[0] = a;
+            } else if (sm == s + 1 && !(j == J - 1 && lane == kWave - 1)) {
+                c4[1] = a;
+            } else {
+                if (nfar >= 2) return -1;
+                fp[nfar] = place_of_slot(sm);
+                c4[2 + nfar] = a;
+                ++nfar;
+            }
+        }
+        far[base + pl] = fp[0] | (fp[1] << 16);
+        coef[base + pl] = float4{c4[0], c4[1], c4[2], c4[3]};
+    }
+    return size;
+}
+
 // Chebyshev store: per cell the edge coefficients of B = A dt in the difference form
 // (B t)_i = sum_j b_ij (t_j - t_i), b_ij = dt / (R_edge C_i) (electric.py:92-108), scaled by
 // 2 / rho, the series coefficients, and the cells ordered by decreasing work (terms x rows).
@@ -513,12 +597,12 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     std::vector<double> coef;
     std::vector<std::vector<std::pair<int, double>>> adj;
     std::vector<double> work;
-    // preorder copies (cells whose rows have at most 3 neighbours)
+    // preorder slot layouts (cells whose rows have at most 3 neighbours)
     std::vector<uint16_t> ppos(N, 0);
-    std::vector<uint2> pidx(N, uint2{0u, 0u});
-    std::vector<float4> pcoef(N, float4{0.0f, 0.0f, 0.0f, 0.0f});
-    std::vector<uint8_t> pcell(p->num_cells, 0);
-    std::vector<int32_t> stack, place;
+    std::vector<uint32_t> pfar;
+    std::vector<float4> pcoef;
+    std::vector<int64_t> pbase(p->num_cells, -1);
+    std::vector<int32_t> psize(p->num_cells, 0);
     double terms_rows = 0.0;
     int max_deg = 0;
     int64_t edges = 0;
@@ -552,36 +636,23 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
         const double scale = rho > 0.0 ? 2.0 / rho : 0.0;
         int cell_deg = 0;
         for (int i = 0; i < n; ++i) cell_deg = std::max<int>(cell_deg, (int)adj[i].size());
+        for (int i = 0; i < n; ++i) ppos[r0 + i] = (uint16_t)i;
         if (cell_deg <= 3 && n > 0 && (int)coef.size() <= kChebCoefCap) {
-            // depth-first preorder, children in increasing index order: a first child's place is
-            // its parent's place + 1.  adj[i] lists the parent (an earlier index) before the
-            // children, so the slots are parent, first child, second child.
-            pcell[cell] = 1;
-            place.assign(n, -1);
-            stack.assign(1, 0);
-            int next = 0;
-            while (!stack.empty()) {
-                const int i = stack.back();
-                stack.pop_back();
-                place[i] = next++;
-                for (int d = (int)adj[i].size() - 1; d >= 0; --d)
-                    if (adj[i][d].first > i) stack.push_back(adj[i][d].first);
+            std::vector<std::vector<std::pair<int, double>>> sadj(adj);
+            for (auto& row : sadj)
+                for (auto& e : row) e.second *= scale;
+            std::vector<uint16_t> cpos(n);
+            const size_t mark = pfar.size();
+            int J = 0;
+            const int size = preorder_layout(n, sadj, 1.0, J, cpos, pfar, pcoef);
+            if (size > 0) {
+                pbase[cell] = (int64_t)mark;
+                psize[cell] = size | (J << 16);
+                for (int i = 0; i < n; ++i) ppos[r0 + i] = cpos[i];
+            } else {
+                pfar.resize(mark);
+                pcoef.resize(mark);
             }
-            NW_REQUIRE(next == n, "cell %lld is not one tree", (long long)cell);
-            for (int i = 0; i < n; ++i) {
-                const int q = place[i];
-                ppos[r0 + i] = (uint16_t)q;
-                uint32_t slot[3] = {(uint32_t)q, (uint32_t)q, (uint32_t)q};
-                float co[3] = {0.0f, 0.0f, 0.0f};
-                for (size_t d = 0; d < adj[i].size(); ++d) {
-                    slot[d] = (uint32_t)place[adj[i][d].first];
-                    co[d] = (float)(scale * adj[i][d].second);
-                }
-                pidx[r0 + q] = uint2{slot[0] | (slot[1] << 16), slot[2]};
-                pcoef[r0 + q] = float4{co[0], co[1], co[2], 0.0f};
-            }
-        } else {
-            for (int i = 0; i < n; ++i) ppos[r0 + i] = (uint16_t)i;
         }
         for (int i = 0; i < n; ++i) {
             const int64_t g = r0 + i;
@@ -634,14 +705,22 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
                      hipMemcpyHostToDevice));
     NW_HIP(hipMemcpy(p->cheb_c, p->h_cheb_c.data(), sizeof(double) * p->h_cheb_c.size(),
                      hipMemcpyHostToDevice));
+    if (pfar.empty()) {  // valid (never read) arrays
+        pfar.push_back(kChebHole);
+        pcoef.push_back(float4{0.0f, 0.0f, 0.0f, 0.0f});
+    }
     NW_HIP(hipMalloc(&p->pre_pos, sizeof(uint16_t) * std::max<int64_t>(N, 1)));
-    NW_HIP(hipMalloc(&p->pre_idx, sizeof(uint2) * std::max<int64_t>(N, 1)));
-    NW_HIP(hipMalloc(&p->pre_coef, sizeof(float4) * std::max<int64_t>(N, 1)));
-    NW_HIP(hipMalloc(&p->pre_cell, std::max<int64_t>(p->num_cells, 1)));
+    NW_HIP(hipMalloc(&p->pre_far, sizeof(uint32_t) * pfar.size()));
+    NW_HIP(hipMalloc(&p->pre_coef, sizeof(float4) * pcoef.size()));
+    NW_HIP(hipMalloc(&p->pre_base, sizeof(int64_t) * std::max<int64_t>(p->num_cells, 1)));
+    NW_HIP(hipMalloc(&p->pre_size, sizeof(int32_t) * std::max<int64_t>(p->num_cells, 1)));
     NW_HIP(hipMemcpy(p->pre_pos, ppos.data(), sizeof(uint16_t) * N, hipMemcpyHostToDevice));
-    NW_HIP(hipMemcpy(p->pre_idx, pidx.data(), sizeof(uint2) * N, hipMemcpyHostToDevice));
-    NW_HIP(hipMemcpy(p->pre_coef, pcoef.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
-    NW_HIP(hipMemcpy(p->pre_cell, pcell.data(), p->num_cells, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_far, pfar.data(), sizeof(uint32_t) * pfar.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_coef, pcoef.data(), sizeof(float4) * pcoef.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_base, pbase.data(), sizeof(int64_t) * p->num_cells, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_size, psize.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
+    p->pre_cells = 0;
+    for (int64_t c = 0; c < p->num_cells; ++c) p->pre_cells += pbase[c] >= 0;
     // launch order: most work first (series terms x rows per thread, then rows)
     std::vector<int32_t> order(p->num_cells);
     std::iota(order.begin(), order.end(), 0);
@@ -667,9 +746,10 @@ static void free_prop(nw_propagator* p) {
     if (p->cheb_c) (void)hipFree(p->cheb_c);
     if (p->cheb_v64) (void)hipFree(p->cheb_v64);
     if (p->pre_pos) (void)hipFree(p->pre_pos);
-    if (p->pre_idx) (void)hipFree(p->pre_idx);
+    if (p->pre_far) (void)hipFree(p->pre_far);
     if (p->pre_coef) (void)hipFree(p->pre_coef);
-    if (p->pre_cell) (void)hipFree(p->pre_cell);
+    if (p->pre_base) (void)hipFree(p->pre_base);
+    if (p->pre_size) (void)hipFree(p->pre_size);
     if (p->cell_row0) (void)hipFree(p->cell_row0);
     if (p->cell_nz0) (void)hipFree(p->cell_nz0);
     if (p->row_off) (void)hipFree(p->row_off);
@@ -886,17 +966,18 @@ __global__ void __launch_bounds__(kChebBlock) k_half_step_cheb(HalfArgs a, ChebS
     const int n = (int)(a.cell_row0[cell + 1] - row0);
     const double dt_ms = a.half_dt_ms;
     const double dt_s = dt_ms * 1e-3;
-    const bool pre = cs.pre_cell[cell] != 0;
+    const int64_t pre_base = cs.pre_base[cell];
+    const bool pre = pre_base >= 0;
     const uint16_t* pos = cs.pre_pos + row0;
     for (int i = threadIdx.x; i < n; i += blockDim.x)
         b0[pre ? pos[i] : i] = half_x(a, row0 + i, dt_ms, dt_s, true);
     __syncthreads();
     double out[kChebRows];
-    cheb_cell(cs, row0, cell, n, pre, b0, out);
+    const int places = cheb_cell(cs, row0, cell, n, pre_base, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
-        if (r < n) b0[r] = out[j];
+        if (r < places) b0[r] = out[j];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) a.v[row0 + i] = (float)b0[pre ? pos[i] : i];

@@ -127,6 +127,37 @@ def test_chebyshev_population_against_oracle(torch_gpu):
     assert np.abs(co.v - v_end).max() < 2e-2
 
 
+def test_chebyshev_cell_sizes_across_row_groups(torch_gpu):
+    """Cells of 1, 2, 3, 63, 64, 65, 1023, 1024, 1025 and 2049 segments in one population (one,
+    two and three row groups per thread, rows on either side of the wave and workgroup
+    boundaries, layouts with empty slots), 0.5 nA constant current at every root, 200 steps:
+    every segment against the FP64 oracle with the unpruned propagator."""
+    import scipy.sparse
+    from neuwon_amd import population, synth
+    torch = torch_gpu
+    rng = np.random.default_rng(21)
+    sizes = [1, 2, 3, 63, 64, 65, 1023, 1024, 1025, 2049]
+    cells = [synth.random_cell(n, rng) for n in sizes]
+    pop = population.HHPopulation(cells, store="chebyshev", stim={"mode": "constant", "amp_na": 0.5})
+    pop.advance(200)
+    torch.cuda.synchronize()
+    v = pop.v.cpu().numpy().astype(np.float64)
+    blocks = []
+    for c in range(pop.num_cells):
+        r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
+        blocks.append(population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1],
+                                                     pop.C[r0:r1], 0.05, 0.0))
+    P = scipy.sparse.block_diag(blocks, format="csr")
+    co = O.CHHPopulation(pop.cell_ptr, P, pop.C, pop.g_bar, dt=pop.dt, celsius=pop.celsius,
+                         gate_mode="table", stim=("constant", 0.5e-9))
+    co.advance(200, threads=len(os.sched_getaffinity(0)))
+    rel = np.abs(v - co.v) / np.abs(co.v)
+    per = [float("%.1e" % rel[int(pop.cell_ptr[c]):int(pop.cell_ptr[c + 1])].max())
+           for c in range(pop.num_cells)]
+    _say(f"sizes {sizes}: max rel per cell {per}, spikes {pop.ap_count.cpu().numpy().tolist()}")
+    assert rel.max() < 1e-5
+
+
 def test_chebyshev_large_cell(torch_gpu):
     """A 4600-segment cell (rows in the fifth register row of a 1024-thread block), 0.05 nA
     constant current, 300 steps, every segment against the FP64 oracle."""
