@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per compartment-step of the step kernel on this workload, per propagator format, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs scaled by what the same counters read for a
 # plain 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); None where not measured
-MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3, "chebyshev": 157.5}
+MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3, "chebyshev": 151.0}
 # Kept entries per row of the propagator pruned at 1e-9 (pair rule), this seeded 10M workload:
 # the canonical algorithmic count of the Chebyshev store, which keeps no matrix, uses it
 NNZ_BAR_PRUNED_1E9 = 567.99
@@ -48,8 +48,8 @@ def format_bytes(fmt, nnz_bar):
     per-segment mechanism state and constants."""
     if fmt == "symmetric":
         return 2.0 * (6.0 * (nnz_bar + 1.0) / 2.0 + 20.0) + 68.0
-    if fmt == "chebyshev":   # per half step the preorder place, 3 neighbour places and FP32
-        return 2.0 * (2.0 + 8.0 + 16.0 + 16.0) + 68.0   # coefficients, FP64 v in/out
+    if fmt == "chebyshev":   # per half step the LDS place, 2 far places and 4 FP32 coefficients,
+        return 2.0 * (2.0 + 4.0 + 16.0 + 16.0) + 68.0   # FP64 v in/out
     return 2.0 * (6.0 * nnz_bar + 12.0) + 68.0
 
 
@@ -359,11 +359,11 @@ def main():
                      "measured_traffic_GBs": None if traffic is None else traffic * units_per_s / 1e9},
     }
     if pop.propagator.chebyshev:
-        # the series runs out of LDS: per row and term three neighbour slots read (a slot a row
-        # does not use reads the row itself) and the new term written, 8 B each; the row's own
-        # last two terms stay in registers
+        # the series runs out of LDS: per row and term two far-neighbour slots read (an unused
+        # slot reads a zero) and the new term written, 8 B each; the row's own last two terms and
+        # its r +- 1 neighbours' terms come from registers
         terms = st["mean_terms"]
-        lds_b = 2.0 * terms * 8.0 * 4.0
+        lds_b = 2.0 * terms * 8.0 * 3.0
         out["roofline"].update({"lds_bytes_per_cmpt_step": lds_b,
                                 "lds_GBs": lds_b * N * K / (kern_ms / 1e3) / 1e9,
                                 "lds_peak_GBs": 256 * 128 * 2.4,
