@@ -7,6 +7,10 @@ rate multipliers theta = (ta_m, tb_m, ta_h, tb_h, ta_n, tb_n) scaling the HH alp
 (hh.mod:62-86).  The forward pass keeps the state every `checkpoint_every` steps; the reverse pass
 recomputes each segment with nw_hh_step writing a per-step record and runs nw_hh_adjoint over it.
 The reference has no differentiation; this is the build's own extension of its step.
+
+The forward passes (the loss and the recomputation) run on the Chebyshev store by default (no
+stored matrix, unpruned, the fastest step); the reverse pass needs a stored matrix and runs on
+the symmetric (default) or general store, pruned at `prune`.
 """
 import ctypes
 
@@ -18,15 +22,29 @@ from . import _lib, channels, population
 class HHAdjoint:
     """Batched cells with HH on every segment and a constant current at every root.
 
-    symmetric: propagator store of the forward and reverse passes (the symmetric one moves half
-    the bytes; see population.Propagator)."""
+    symmetric: store of the reverse pass, symmetric (half the bytes) or general (see
+    population.Propagator).  forward_store: store of the forward passes, "chebyshev" (default
+    when every cell has at most 5120 segments), "symmetric" or "general"; the reverse pass's
+    store when it names the same one."""
 
     def __init__(self, cells, dt=0.05, celsius=6.3, stim_amp_na=0.5, prune=1e-9,
-                 checkpoint_every=100, v0=-70.0, device=0, symmetric=True):
+                 checkpoint_every=100, v0=-70.0, device=0, symmetric=True, forward_store=None):
+        reverse_store = "symmetric" if symmetric else "general"
+        if forward_store is None:
+            fits = max(len(c) for c in cells) <= population.CHEBYSHEV_MAX_CELL
+            forward_store = "chebyshev" if fits else reverse_store
         self.pop = population.HHPopulation(cells, dt=dt, celsius=celsius, v0=v0, prune=prune,
                                            gate_mode="table", device=device,
                                            stim={"mode": "constant", "amp_na": stim_amp_na},
-                                           symmetric=symmetric)
+                                           store=forward_store)
+        pop = self.pop
+        if forward_store == reverse_store:
+            self.reverse = pop.propagator
+        else:
+            self.reverse = population.Propagator(pop.cell_ptr, pop.parent_local, pop.R, pop.C,
+                                                 0.5 * pop.dt, prune, device=device,
+                                                 store=reverse_store)
+        self.forward_store = forward_store
         self.torch = self.pop.torch
         self.dev = self.pop.dev
         self.K = int(checkpoint_every)
@@ -103,7 +121,7 @@ class HHAdjoint:
             prm.conductances_valid = 1 if s0 > 0 else 0
             s = torch.cuda.current_stream(self.dev)
             _lib.check(_lib.load().nw_hh_adjoint(
-                pop.propagator.handle, ctypes.byref(prm), n, P(self._record), P(self.table6),
+                self.reverse.handle, ctypes.byref(prm), n, P(self._record), P(self.table6),
                 P(pop.g_na), P(pop.g_k), P(pop.g_l), P(pop.inv_c), P(self.soma), P(self.target),
                 self.T, P(lam[0]), P(lam[1]), P(lam[2]), P(lam[3]), P(grads[0]), P(grads[1]),
                 P(grads[2]), P(gth), P(loss), ctypes.c_void_p(s.cuda_stream)))

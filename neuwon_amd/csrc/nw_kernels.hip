@@ -303,10 +303,22 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v
     const int64_t tick = 2 * gstep + 1 + HALF;
     const int stim_row = a.stim_local ? a.stim_local[cell] : -1;
     const bool g_valid = HALF == 1 || p.conductances_valid || step > 0;
+    const int64_t NR = a.num_rows;
     for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
         const int64_t gi = row0 + i;
         const double v = from_v32 ? (double)a.v[gi] : v64[gi];
         double g = 0.0, d = 0.0;
+        if (a.record) {  // the record of k_hh_step: v, m, h, n at the start, v after half A
+            float* rec = a.record + (int64_t)step * 5 * NR + gi;
+            if (HALF == 0) {
+                rec[0] = (float)v;
+                rec[NR] = a.m[gi];
+                rec[2 * NR] = a.h[gi];
+                rec[3 * NR] = a.n[gi];
+            } else {
+                rec[4 * NR] = (float)v;
+            }
+        }
         if (HALF == 1) {
             double m = a.m[gi], h = a.h[gi], n = a.n[gi];
             if (GATE_MODE == 1) {
@@ -346,10 +358,19 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v
         if (r < places) b0[r] = out[j];
     }
     __syncthreads();
+    const bool closes = HALF == 1 && a.record && step == a.n_steps - 1;
     for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+        const int64_t gi = row0 + i;
         const double vi = b0[pre ? pos[i] : i];
-        v64[row0 + i] = vi;
-        if (HALF == 1) a.v[row0 + i] = (float)vi;
+        v64[gi] = vi;
+        if (HALF == 1) a.v[gi] = (float)vi;
+        if (closes) {  // the state after the last step closes the record
+            float* rec = a.record + (int64_t)a.n_steps * 5 * NR + gi;
+            rec[0] = (float)vi;
+            rec[NR] = a.m[gi];
+            rec[2 * NR] = a.h[gi];
+            rec[3 * NR] = a.n[gi];
+        }
     }
     if (HALF == 1) {
         const int ap_row = a.ap_local ? a.ap_local[cell] : -1;
@@ -959,7 +980,6 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
     if (p->chebyshev) {
         NW_REQUIRE(prm->block_threads == 0 || prm->block_threads == kChebBlock,
                    "the Chebyshev store runs %d threads per workgroup", kChebBlock);
-        NW_REQUIRE(record == nullptr, "the Chebyshev store keeps no record for the adjoint");
         NW_REQUIRE(p->max_cell_rows <= kChebBuf, "cell of %d segments exceeds the Chebyshev store",
                    p->max_cell_rows);
         const unsigned grid = (unsigned)p->num_cells;

@@ -31,10 +31,10 @@ import os
 import numpy as np
 
 from . import _lib, channels, events, geometry
+from .population import CHEBYSHEV_MAX_CELL
 
 EPSILON = float(np.finfo(np.float64).eps)
 NULL = -1
-CHEBYSHEV_MAX_CELL = 5120   # segments per cell the Chebyshev store takes
 
 
 def _torch():

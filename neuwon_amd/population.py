@@ -34,6 +34,9 @@ def store_name(store=None, symmetric=False):
     return store
 
 
+CHEBYSHEV_MAX_CELL = 5120   # segments per cell the Chebyshev store takes
+
+
 class Propagator:
     """Owner of an ``nw_propagator`` handle.
 
