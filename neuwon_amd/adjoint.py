@@ -8,9 +8,9 @@ rate multipliers theta = (ta_m, tb_m, ta_h, tb_h, ta_n, tb_n) scaling the HH alp
 recomputes each segment with nw_hh_step writing a per-step record and runs nw_hh_adjoint over it.
 The reference has no differentiation; this is the build's own extension of its step.
 
-The forward passes (the loss and the recomputation) run on the Chebyshev store by default (no
-stored matrix, unpruned, the fastest step); the reverse pass needs a stored matrix and runs on
-the symmetric (default) or general store, pruned at `prune`.
+Both passes run on the Chebyshev store by default (no stored matrix, unpruned, the fastest step;
+the reverse pass applies P^T lambda = C P (C^-1 lambda) with the same series); the symmetric and
+general stores (pruned at `prune`) remain available for either pass.
 """
 import ctypes
 
@@ -22,17 +22,19 @@ from . import _lib, channels, population
 class HHAdjoint:
     """Batched cells with HH on every segment and a constant current at every root.
 
-    symmetric: store of the reverse pass, symmetric (half the bytes) or general (see
-    population.Propagator).  forward_store: store of the forward passes, "chebyshev" (default
-    when every cell has at most 5120 segments), "symmetric" or "general"; the reverse pass's
-    store when it names the same one."""
+    forward_store / reverse_store: propagator store of the forward passes (the loss and the
+    recomputation with the record) and of the reverse pass: "chebyshev" (the default for both
+    when every cell has at most 5120 segments), "symmetric" or "general" (see
+    population.Propagator).  symmetric: the stored format chosen where a store is not named and
+    the cells do not fit the Chebyshev store (symmetric: half the bytes)."""
 
     def __init__(self, cells, dt=0.05, celsius=6.3, stim_amp_na=0.5, prune=1e-9,
-                 checkpoint_every=100, v0=-70.0, device=0, symmetric=True, forward_store=None):
-        reverse_store = "symmetric" if symmetric else "general"
-        if forward_store is None:
-            fits = max(len(c) for c in cells) <= population.CHEBYSHEV_MAX_CELL
-            forward_store = "chebyshev" if fits else reverse_store
+                 checkpoint_every=100, v0=-70.0, device=0, symmetric=True, forward_store=None,
+                 reverse_store=None):
+        fits = max(len(c) for c in cells) <= population.CHEBYSHEV_MAX_CELL
+        default = "chebyshev" if fits else ("symmetric" if symmetric else "general")
+        forward_store = forward_store or default
+        reverse_store = reverse_store or default
         self.pop = population.HHPopulation(cells, dt=dt, celsius=celsius, v0=v0, prune=prune,
                                            gate_mode="table", device=device,
                                            stim={"mode": "constant", "amp_na": stim_amp_na},

@@ -630,6 +630,121 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
 }
 
 // ----------------------------------------------------------------------------------------------
+// Reverse pass on the Chebyshev store: one launch per half step, in reverse order (HALF 1: the
+// loss of step k, P^T through the second half step and the mechanism; HALF 0: P^T through the
+// first half step).  P^T lambda = C P (C^-1 lambda) (B = C^-1 G with G symmetric), the same series
+// as the forward product.  The voltage adjoint passes between launches as FP64 (lam64); a call
+// reads it from lam_v at its first launch and leaves it there after its last.
+// ----------------------------------------------------------------------------------------------
+template <int HALF>
+__global__ void __launch_bounds__(kChebBlock) k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64,
+                                                            int k, int first, int last) {
+    __shared__ double b0[kChebLds];
+    __shared__ double red[7][kChebBlock / kWave];
+    const nw_hh_params& p = a.prm;
+    const int cell = a.order[blockIdx.x];
+    const int64_t row0 = a.cell_row0[cell];
+    const int nrows = (int)(a.cell_row0[cell + 1] - row0);
+    const int64_t pre_base = cs.pre_base[cell];
+    const bool pre = pre_base >= 0;
+    const uint16_t* pos = cs.pre_pos + row0;
+    const int64_t NR = a.num_rows;
+    const int64_t gstep = p.step0 + k;
+    const float* rec = a.record + (int64_t)k * 5 * NR;
+    const float* nxt = rec + 5 * NR;  // state at the start of the following step
+    const int soma = a.soma_local[cell];
+    double loss = 0.0;
+    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+        const int64_t gi = row0 + i;
+        double lam = first ? (double)a.lam_v[gi] : lam64[gi];
+        if (HALF == 1 && i == soma) {
+            const double r = (double)nxt[gi] - (double)a.target[(int64_t)cell * a.target_stride + gstep];
+            loss = r * r;
+            lam += 2.0 * r;
+        }
+        b0[pre ? pos[i] : i] = lam * (double)a.inv_c[gi];
+    }
+    __syncthreads();
+    double out[kChebRows];
+    const int places = cheb_cell(cs, row0, cell, nrows, pre_base, b0, out);
+#pragma unroll
+    for (int j = 0; j < kChebRows; ++j) {
+        const int r = threadIdx.x + j * kChebBlock;
+        if (r < places) b0[r] = out[j];
+    }
+    __syncthreads();
+    const double hs = 0.5 * (double)p.dt_ms * 1e-3;
+    const double vmin = p.vmin;
+    const double frq = (double)(p.num_nodes - 1) / ((double)p.vmax - (double)p.vmin);
+    const double e_na = p.e_na, e_k = p.e_k, e_l = p.e_l;
+    double th[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+        const int64_t gi = row0 + i;
+        const double invc = a.inv_c[gi];
+        const double lx = b0[pre ? pos[i] : i] / invc;  // (P^T lambda)_i
+        double lam_out = lx;
+        if (HALF == 1) {  // half B and the mechanism
+            const double v1 = rec[4 * NR + gi];
+            const double mN = nxt[NR + gi], hN = nxt[2 * NR + gi], nN = nxt[3 * NR + gi];
+            double lm = a.lam_m[gi], lh = a.lam_h[gi], ln = a.lam_n[gi];
+            double gr_na = 0, gr_k = 0, gr_l = 0;
+            lam_out = half_adjoint(lx, v1, mN, hN, nN, a.g_na[gi], a.g_k[gi], a.g_l[gi], e_na, e_k,
+                                   e_l, hs, invc, lm, lh, ln, gr_na, gr_k, gr_l);
+            const double g0[3] = {rec[NR + gi], rec[2 * NR + gi], rec[3 * NR + gi]};
+            double lg[3] = {lm, lh, ln};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                GateLookup L;
+                lookup6(a.table6 + (int64_t)q * p.num_nodes * 6, p.num_nodes, v1, vmin, frq, L);
+                lam_out += (L.slope[0] + L.slope[1] * g0[q]) * lg[q];
+                th[2 * q] += (L.val[2] + L.val[3] * g0[q]) * lg[q];
+                th[2 * q + 1] += (L.val[4] + L.val[5] * g0[q]) * lg[q];
+                lg[q] *= L.val[1];
+            }
+            a.lam_m[gi] = (float)lg[0];
+            a.lam_h[gi] = (float)lg[1];
+            a.lam_n[gi] = (float)lg[2];
+            a.grad_na[gi] += gr_na;
+            a.grad_k[gi] += gr_k;
+            a.grad_l[gi] += gr_l;
+        } else if (gstep > 0 || p.conductances_valid) {  // half A with the gates at the start
+            double lm = a.lam_m[gi], lh = a.lam_h[gi], ln = a.lam_n[gi];
+            double gr_na = 0, gr_k = 0, gr_l = 0;
+            lam_out = half_adjoint(lx, rec[gi], rec[NR + gi], rec[2 * NR + gi], rec[3 * NR + gi],
+                                   a.g_na[gi], a.g_k[gi], a.g_l[gi], e_na, e_k, e_l, hs, invc, lm,
+                                   lh, ln, gr_na, gr_k, gr_l);
+            a.lam_m[gi] = (float)lm;
+            a.lam_h[gi] = (float)lh;
+            a.lam_n[gi] = (float)ln;
+            a.grad_na[gi] += gr_na;
+            a.grad_k[gi] += gr_k;
+            a.grad_l[gi] += gr_l;
+        }
+        lam64[gi] = lam_out;
+        if (last) a.lam_v[gi] = (float)lam_out;
+    }
+    if (HALF == 1) {  // the block's rate-parameter gradients and loss (fixed order: deterministic)
+        const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double t = wave_sum(th[q]);
+            if (lane == 0) red[q][wave] = t;
+        }
+        const double l = wave_sum(loss);
+        if (lane == 0) red[6][wave] = l;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 0; q < 7; ++q) {
+                double t = 0.0;
+                for (int w = 0; w < kChebBlock / kWave; ++w) t += red[q][w];
+                if (q < 6) a.grad_theta[(int64_t)cell * 6 + q] += t;
+                else a.loss[cell] += t;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
 // HH mechanism instances (generic staggered path): thread per instance, voltage gathered through
 // the instance's segment; writes the three conductances interleaved (na, k, l).
 // ----------------------------------------------------------------------------------------------
@@ -1029,7 +1144,6 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
                    target && lam_v && lam_m && lam_h && lam_n && grad_na && grad_k && grad_l &&
                    grad_theta && loss,
                "nw_hh_adjoint: null argument");
-    NW_REQUIRE(!p->chebyshev, "nw_hh_adjoint reads a stored propagator (general or symmetric)");
     NW_REQUIRE(prm->gate_mode == 0, "the adjoint is written for the table gate mode");
     NW_REQUIRE(prm->stim_mode == 0 || prm->stim_mode == 1, "adjoint supports constant stimulus only");
     NW_REQUIRE(n_steps >= 0 && prm->num_nodes >= 2, "bad arguments");
@@ -1038,6 +1152,19 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
               p->max_cell_rows, n_steps, *prm, record, p->num_rows, table6, g_na, g_k, g_l, inv_c,
               soma_local, target, target_stride, lam_v, lam_m, lam_h, lam_n, grad_na, grad_k,
               grad_l, grad_theta, loss, p->wgt};
+    if (p->chebyshev) {
+        NW_REQUIRE(p->max_cell_rows <= kChebBuf, "cell of %d segments exceeds the Chebyshev store",
+                   p->max_cell_rows);
+        const ChebStore cs = cheb_store(p);
+        const unsigned grid = (unsigned)p->num_cells;
+        hipStream_t s = (hipStream_t)stream;
+        for (int k = n_steps - 1; k >= 0; --k) {
+            k_hh_adj_cheb<1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, k == n_steps - 1, 0);
+            k_hh_adj_cheb<0><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);
+        }
+        NW_HIP(hipGetLastError());
+        return NW_OK;
+    }
     const bool sym = p->symmetric != 0;
     const size_t lds = (sym ? sizeof(double) + 2 * sizeof(float) : 3 * sizeof(float)) *
                        (size_t)p->max_cell_rows;

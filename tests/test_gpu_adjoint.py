@@ -15,15 +15,16 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _store_pruned(prob, c, half_dt):
-    """FP64 propagator of cell c pruned as the reverse pass's store prunes it: entry rule for the
-    general store, pair rule (either of P_ij, P_ji reaches the threshold) for the symmetric one."""
+def _store_operator(prob, c, half_dt, prop):
+    """FP64 propagator of cell c as store `prop` applies it: unpruned for the Chebyshev store,
+    entry rule for the general store, pair rule (either of P_ij, P_ji reaches the threshold) for
+    the symmetric one."""
     import scipy.sparse
     pop = prob.pop
     r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
-    prune = prob.reverse.prune
+    prune = 0.0 if prop.chebyshev else prop.prune
     parent = pop.parent_local[r0:r1].astype(np.int64)
-    if not prob.reverse.symmetric:
+    if not prop.symmetric:
         return O.propagator(parent, pop.R[r0:r1], pop.C[r0:r1], half_dt, prune=prune)
     full = O.propagator(parent, pop.R[r0:r1], pop.C[r0:r1], half_dt, prune=0.0).toarray()
     keep = np.maximum(np.abs(full), np.abs(full.T)) >= prune
@@ -31,22 +32,11 @@ def _store_pruned(prob, c, half_dt):
     return scipy.sparse.csr_matrix(np.where(keep, full, 0.0))
 
 
-def _forward_operator(prob, c, half_dt):
-    """FP64 propagator of cell c as the forward passes apply it: unpruned on the Chebyshev store,
-    else pruned as the store prunes it."""
-    if prob.forward_store != "chebyshev":
-        return _store_pruned(prob, c, half_dt)
-    pop = prob.pop
-    r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
-    return O.propagator(pop.parent_local[r0:r1].astype(np.int64), pop.R[r0:r1], pop.C[r0:r1],
-                        half_dt, prune=0.0)
-
-
 def _oracle_problem(prob, c, stim_amp, dt):
-    """The FP64 oracle adjoint of one cell, fed the propagator the store holds."""
+    """The FP64 oracle adjoint of one cell, fed the propagator of the reverse pass's store."""
     pop = prob.pop
     r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
-    P = _store_pruned(prob, c, 0.5 * dt)
+    P = _store_operator(prob, c, 0.5 * dt, prob.reverse)
     return r0, r1, O.HHAdjointProblem(P, pop.C[r0:r1], stim_amp, dt=dt)
 
 
@@ -58,20 +48,21 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("symmetric,forward_store",
-                         [(False, "general"), (True, "symmetric"), (True, "chebyshev"),
-                          (False, "chebyshev")])
-def test_adjoint_matches_oracle(torch_gpu, symmetric, forward_store):
-    """Reverse pass on the symmetric or general store, forward passes (loss and recomputation
-    with the record) on the same store or on the Chebyshev one (unpruned): per-segment and rate
-    gradients and the loss against the FP64 oracle adjoint fed the reverse store's propagator."""
+@pytest.mark.parametrize("forward_store,reverse_store",
+                         [("general", "general"), ("symmetric", "symmetric"),
+                          ("chebyshev", "symmetric"), ("chebyshev", "general"),
+                          ("chebyshev", "chebyshev")])
+def test_adjoint_matches_oracle(torch_gpu, forward_store, reverse_store):
+    """Forward passes (loss and recomputation with the record) and reverse pass on each store
+    pairing: per-segment and rate gradients and the loss against the FP64 oracle adjoint fed the
+    reverse store's propagator (unpruned for the Chebyshev store)."""
     from neuwon_amd import adjoint, synth
     torch = torch_gpu
     dt, T, amp = 0.05, 300, 0.1e-9
     cells = synth.uniform_population(num_cells=4, n=90, seed=6)
     prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100,
-                             symmetric=symmetric, forward_store=forward_store)
-    assert prob.pop.propagator.store == forward_store
+                             forward_store=forward_store, reverse_store=reverse_store)
+    assert prob.pop.propagator.store == forward_store and prob.reverse.store == reverse_store
     pop = prob.pop
     gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
     rng = np.random.default_rng(7)
@@ -123,8 +114,8 @@ def test_autograd_function(torch_gpu):
     assert np.allclose(params[3].grad.cpu().numpy(), gr["theta"], rtol=1e-9)
 
 
-@pytest.mark.parametrize("symmetric", [False, True])
-def test_adjoint_workload_1024x500(torch_gpu, symmetric):
+@pytest.mark.parametrize("reverse_store", ["general", "symmetric", "chebyshev"])
+def test_adjoint_workload_1024x500(torch_gpu, reverse_store):
     """1024 cells x 500 compartments, 2000 steps at dt 0.05 ms, checkpoints every 100 steps:
     gradients per second and a per-cell check against the FP64 oracle adjoint."""
     from neuwon_amd import adjoint, synth
@@ -133,10 +124,10 @@ def test_adjoint_workload_1024x500(torch_gpu, symmetric):
     t0 = time.time()
     cells = synth.uniform_population(num_cells=1024, n=500, seed=4)
     prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=100,
-                             symmetric=symmetric)
+                             reverse_store=reverse_store)
     pop = prob.pop
     _say(f"setup {time.time() - t0:.1f}s, forward store {prob.forward_store}, reverse store "
-         f"nnz/row {prob.reverse.mean_nnz_per_row:.1f}")
+         f"{reverse_store}")
     rng = np.random.default_rng(4)
     gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
     hidden = _hidden(gb, rng)
@@ -149,7 +140,7 @@ def test_adjoint_workload_1024x500(torch_gpu, symmetric):
     loss, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), target)
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
-    _say(f"{prob.forward_store} forward, {'symmetric' if symmetric else 'general'} reverse: "
+    _say(f"{prob.forward_store} forward, {reverse_store} reverse: "
          f"one gradient of {pop.num_segments} compartments x {T} steps: {el:.2f}s "
          f"({1.0 / el:.3f} gradients/s), loss {loss:.4e}, "
          f"target v_soma max {float(target.max()):.1f} mV")
@@ -166,8 +157,7 @@ def test_adjoint_workload_1024x500(torch_gpu, symmetric):
 def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
     """Central finite differences of the FP64 C oracle (h = 1e-6 relative) on 32 cells of
     500 compartments, 2000 steps at dt 0.05 ms, against the GPU gradient with the default stores
-    (forward passes on the Chebyshev store, reverse pass on the symmetric one pruned at 1e-9; the
-    oracle's loss runs the forward passes' operator, the unpruned propagator)."""
+    (both passes on the Chebyshev store; the oracle's loss runs the same unpruned propagator)."""
     import os
     import scipy.sparse
     from neuwon_amd import adjoint, synth
@@ -184,8 +174,8 @@ def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
     target = prob.trace.clone()
     _, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), target)
     tg = target.cpu().numpy().astype(np.float64)
-    P = scipy.sparse.block_diag([_forward_operator(prob, c, 0.5 * dt) for c in range(ncell)],
-                                format="csr")
+    P = scipy.sparse.block_diag([_store_operator(prob, c, 0.5 * dt, prob.pop.propagator)
+                                 for c in range(ncell)], format="csr")
     roots = pop.cell_ptr[:-1]
     threads = len(os.sched_getaffinity(0))
 

@@ -214,8 +214,8 @@ def test_chebyshev_model_with_a_many_branched_soma(torch_gpu):
 
 
 def test_chebyshev_refusals(torch_gpu):
-    """The adjoint and the species step need a stored matrix; cells above 5120 segments and the
-    matrix export are refused with a message."""
+    """The species step needs a stored matrix; cells above 5120 segments and the matrix export
+    are refused with a message."""
     import ctypes
     from neuwon_amd import _lib, population, synth
     pop = population.HHPopulation(synth.single_cell(300, seed=2), store="chebyshev")
@@ -228,7 +228,3 @@ def test_chebyshev_refusals(torch_gpu):
     rc = _lib.load().nw_ca_step(pop.propagator.handle, ctypes.byref(p), 1, _lib.ptr(pop.v),
                                 _lib.ptr(pop.m), _lib.ptr(pop.h), None)
     assert rc != 0 and b"general format" in _lib.load().nw_last_error()
-    with pytest.raises(RuntimeError, match="stored propagator"):
-        _lib.check(_lib.load().nw_hh_adjoint(pop.propagator.handle, ctypes.byref(pop.params), 1,
-                                             *([_lib.ptr(pop.v)] * 8), 0,
-                                             *([_lib.ptr(pop.v)] * 9), None))
