@@ -407,6 +407,7 @@ struct nw_propagator {
     int64_t* pre_base = nullptr;   // [num_cells]
     int32_t* pre_size = nullptr;   // [num_cells] slots in use (rows and holes) | J << 16
     int64_t pre_cells = 0;         // cells laid out in preorder
+    int32_t cheb_maxj = 0;         // row groups per thread the store's cells need (1..kChebRows)
     double mean_terms = 0.0;
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;
@@ -446,17 +447,18 @@ constexpr int kChebRows = 5;
 // On entry buf0[i] = x_i for every row of the cell (after a barrier); on return the thread's
 // rows hold their result in `out[j]` and the call has ended with a barrier, so both LDS buffers
 // are free.  K = 0 returns x.
+template <int ROWS>
 __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const float* __restrict__ nbc,
                                           const int32_t* __restrict__ ovi, const float* __restrict__ ovc,
                                           const double* __restrict__ c, int K, int n, double* buf0,
                                           double* buf1, double (&out)[kChebRows]) {
     // neighbours 0 and 1 (local rows < 65536) packed in one register, neighbour 2 (or the
-    // overflow marker) in another
-    uint32_t i01[kChebRows];
-    int32_t i2[kChebRows];
-    float cf[kChebRows][3];
+    // overflow marker) in another; ROWS rows per thread (n <= ROWS * kChebBlock)
+    uint32_t i01[ROWS];
+    int32_t i2[ROWS];
+    float cf[ROWS][3];
 #pragma unroll
-    for (int j = 0; j < kChebRows; ++j) {
+    for (int j = 0; j < ROWS; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
         out[j] = 0.0;
         i01[j] = r < n ? ((uint32_t)nbi[3 * r] | ((uint32_t)nbi[3 * r + 1] << 16)) : 0u;
@@ -472,7 +474,7 @@ __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const
     for (int k = 1; k <= K; ++k) {
         const double ck = c[k];
 #pragma unroll
-        for (int j = 0; j < kChebRows; ++j) {
+        for (int j = 0; j < ROWS; ++j) {
             const int r = threadIdx.x + j * kChebBlock;
             if (r < n) {
                 const double tc = cur[r];
@@ -508,14 +510,15 @@ __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const
 // the neighbouring lane (DPP row shifts of the whole wave), and the far neighbours f, g read from
 // the LDS vector of the previous term.  The FP32 coefficients a = (2/rho) b are widened to FP64
 // and doubled exactly, so constant vectors stay constant to FP64 rounding.
+//
+// A launch is built for at most MAXJ row groups per thread (MAXJ = 1 or kChebRows): its LDS holds
+// two term vectors of BUF = MAXJ * 1024 doubles and the series coefficients (cells of more terms
+// are read in natural order).  The one-group launch needs 24 KB and at most 64 VGPRs, so two
+// workgroups share a CU.  Each cell's layout leaves its last place (J * 1024 - 1) empty: unused
+// far slots read the zero kept there in both vectors.
 constexpr int kChebBuf = kChebBlock * kChebRows;
-// Series coefficients staged in LDS after the two buffers (cells of more terms are read in natural
-// order); LDS of a Chebyshev launch: 2 kChebBuf + kChebCoefCap doubles.
 constexpr int kChebCoefCap = 1024;
-constexpr int kChebLds = 2 * kChebBuf + kChebCoefCap;
-// LDS place of the zero that unused far slots read, in both buffers: the last slot of a cell of
-// kChebRows row groups, which the layout leaves empty.
-constexpr int kChebZero = kChebBuf - 1;
+constexpr int cheb_lds(int maxj) { return 2 * maxj * kChebBlock + kChebCoefCap; }
 constexpr uint32_t kChebHole = 0xFFFFFFFFu;
 
 // v of lane l - 1 (wave_shr:1; lane 0 receives 0) and of lane l + 1 (wave_shl:1; lane 63: 0).
@@ -545,13 +548,13 @@ struct ChebRegs {
 // arrays swap roles from term to term, so no register is copied); far neighbours from buffer RD,
 // T_k to the other buffer, out += c_k T_k, then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
 // (wave-uniform): the wave holds a slot in use; a wave without one only keeps the barriers.
-template <int J, typename CT, int RD, bool FIRST>
+template <int J, typename CT, int RD, bool FIRST, int BUF>
 __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R, const double (&t)[J],
                                           double (&u)[J], bool active, int k, double (&out)[kChebRows]) {
     if (active) {
-        const char* rd = reinterpret_cast<const char*>(lds + RD * kChebBuf);
-        double* wr = lds + (1 - RD) * kChebBuf;
-        const double ck = lds[2 * kChebBuf + k];
+        const char* rd = reinterpret_cast<const char*>(lds + RD * BUF);
+        double* wr = lds + (1 - RD) * BUF;
+        const double ck = lds[2 * BUF + k];
         double vf[J], vg[J];
 #pragma unroll
         for (int j = 0; j < J; ++j) {
@@ -584,14 +587,15 @@ __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R,
 // K < kChebCoefCap).
 // On entry lds[j * 1024 + t] = x of the row in slot J t + j (after a barrier); on return out[j]
 // holds that row's result and the call has ended with a barrier.
-template <int J, typename CT>
+template <int J, typename CT, int BUF>
 __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, const float4* __restrict__ cf,
                                               const double* __restrict__ c, int K, int size, double* lds,
                                               double (&out)[kChebRows]) {
-    for (int k = threadIdx.x; k <= K; k += kChebBlock) lds[2 * kChebBuf + k] = c[k];
+    constexpr int zero = J * kChebBlock - 1;
+    for (int k = threadIdx.x; k <= K; k += kChebBlock) lds[2 * BUF + k] = c[k];
     if (threadIdx.x == 0) {
-        lds[kChebZero] = 0.0;
-        lds[kChebBuf + kChebZero] = 0.0;
+        lds[zero] = 0.0;
+        lds[BUF + zero] = 0.0;
     }
     ChebRegs<J, CT> R;
     double x[J], y[J];
@@ -612,7 +616,7 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
             R.d2[j] = 2.0 * (1.0 - (((double)a.x + (double)a.y) + ((double)a.z + (double)a.w)));
             x[j] = lds[q];
         } else {  // hole or padding: zero, read by nobody through LDS
-            R.f[j] = R.g[j] = 8u * kChebZero;
+            R.f[j] = R.g[j] = 8u * zero;
             R.cl[j] = R.cr[j] = R.cf[j] = R.cg[j] = (CT)0;
             R.d2[j] = 0.0;
             x[j] = 0.0;
@@ -620,39 +624,46 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
         y[j] = 0.0;
     }
     __syncthreads();
-    const double c0 = lds[2 * kChebBuf];
+    const double c0 = lds[2 * BUF];
 #pragma unroll
     for (int j = 0; j < J; ++j) out[j] = c0 * x[j];
     // x = T_0, then y = T_1, x = T_2, y = T_3, ...
-    if (K >= 1) cheb_term<J, CT, 0, true>(lds, R, x, y, active, 1, out);
+    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, 1, out);
     int k = 2;
     for (; k + 1 <= K; k += 2) {
-        cheb_term<J, CT, 1, false>(lds, R, y, x, active, k, out);
-        cheb_term<J, CT, 0, false>(lds, R, x, y, active, k + 1, out);
+        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
+        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, k + 1, out);
     }
-    if (k <= K) cheb_term<J, CT, 1, false>(lds, R, y, x, active, k, out);
+    if (k <= K) cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
 }
 
 // The product for one cell, preorder slots (base >= 0) or natural order, from x in LDS (at the
-// row's LDS place) to out[] (the result of the row at LDS place threadIdx.x + j * 1024).  Returns
-// the number of LDS places in use: out[j] is defined where threadIdx.x + j * 1024 is below it.
+// row's LDS place) to out[] (the result of the row at LDS place threadIdx.x + j * 1024), in a
+// launch built for MAXJ row groups (the store's cells need no more).  Returns the number of LDS
+// places in use: out[j] is defined where threadIdx.x + j * 1024 is below it.
+template <int MAXJ>
 __device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int cell, int n, int64_t base,
                                          double* lds, double (&out)[kChebRows]) {
+    constexpr int BUF = MAXJ * kChebBlock;
     const double* c = cs.c + cs.off[cell];
     const int K = cs.K[cell];
     if (base < 0) {
-        cheb_rows(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef, c, K, n, lds,
-                  lds + kChebBuf, out);
+        cheb_rows<MAXJ>(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef, c, K,
+                        n, lds, lds + BUF, out);
         return n;
     }
     const uint32_t* far = cs.pre_far + base;
     const float4* cf = cs.pre_coef + base;
     const int size = cs.pre_size[cell] & 0xFFFF;
     const int J = cs.pre_size[cell] >> 16;
-    if (J <= 1) cheb_pre_rows<1, double>(far, cf, c, K, size, lds, out);
-    else if (J == 2) cheb_pre_rows<2, double>(far, cf, c, K, size, lds, out);
-    else if (J == 3) cheb_pre_rows<3, double>(far, cf, c, K, size, lds, out);
-    else if (J == 4) cheb_pre_rows<4, double>(far, cf, c, K, size, lds, out);
-    else cheb_pre_rows<5, float>(far, cf, c, K, size, lds, out);
+    if constexpr (MAXJ == 1) {
+        cheb_pre_rows<1, double, BUF>(far, cf, c, K, size, lds, out);
+    } else {
+        if (J <= 1) cheb_pre_rows<1, double, BUF>(far, cf, c, K, size, lds, out);
+        else if (J == 2) cheb_pre_rows<2, double, BUF>(far, cf, c, K, size, lds, out);
+        else if (J == 3) cheb_pre_rows<3, double, BUF>(far, cf, c, K, size, lds, out);
+        else if (J == 4) cheb_pre_rows<4, double, BUF>(far, cf, c, K, size, lds, out);
+        else cheb_pre_rows<5, float, BUF>(far, cf, c, K, size, lds, out);
+    }
     return J * kChebBlock;
 }

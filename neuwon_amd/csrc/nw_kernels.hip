@@ -286,10 +286,10 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 // of 1024-thread blocks.  Voltages pass between launches as FP64 (v64); the FP32 voltage array
 // is written after every step.
 // ----------------------------------------------------------------------------------------------
-template <int GATE_MODE, int HALF>
-__global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v64, int step,
-                                                             int from_v32) {
-    __shared__ double b0[kChebLds];
+template <int GATE_MODE, int HALF, int MAXJ>
+__global__ void __launch_bounds__(kChebBlock) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
+    __shared__ double b0[cheb_lds(MAXJ)];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_half_cheb(HHArgs a, double* v
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell(a.cheb, row0, cell, nrows, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ>(a.cheb, row0, cell, nrows, pre_base, b0, out);
     // results to LDS (the row's place), then out in natural order
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
@@ -636,10 +636,10 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
 // as the forward product.  The voltage adjoint passes between launches as FP64 (lam64); a call
 // reads it from lam_v at its first launch and leaves it there after its last.
 // ----------------------------------------------------------------------------------------------
-template <int HALF>
-__global__ void __launch_bounds__(kChebBlock) k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64,
-                                                            int k, int first, int last) {
-    __shared__ double b0[kChebLds];
+template <int HALF, int MAXJ>
+__global__ void __launch_bounds__(kChebBlock) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last) {
+    __shared__ double b0[cheb_lds(MAXJ)];
     __shared__ double red[7][kChebBlock / kWave];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
@@ -666,7 +666,7 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_adj_cheb(AdjArgs a, ChebStore
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell(cs, row0, cell, nrows, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ>(cs, row0, cell, nrows, pre_base, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
@@ -692,8 +692,8 @@ __global__ void __launch_bounds__(kChebBlock) k_hh_adj_cheb(AdjArgs a, ChebStore
                                    e_l, hs, invc, lm, lh, ln, gr_na, gr_k, gr_l);
             const double g0[3] = {rec[NR + gi], rec[2 * NR + gi], rec[3 * NR + gi]};
             double lg[3] = {lm, lh, ln};
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
+#pragma unroll 1
+            for (int q = 0; q < 3; ++q) {  // one gate at a time: fewer live registers
                 GateLookup L;
                 lookup6(a.table6 + (int64_t)q * p.num_nodes * 6, p.num_nodes, v1, vmin, frq, L);
                 lam_out += (L.slope[0] + L.slope[1] * g0[q]) * lg[q];
@@ -1099,13 +1099,22 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
                    p->max_cell_rows);
         const unsigned grid = (unsigned)p->num_cells;
         double* v64 = p->cheb_v64;
+        const bool one = p->cheb_maxj == 1;
         for (int st = 0; st < n_steps; ++st) {
-            if (prm->gate_mode == 1) {
-                k_hh_half_cheb<1, 0><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
-                k_hh_half_cheb<1, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
+            if (one) {
+                if (prm->gate_mode == 1) {
+                    k_hh_half_cheb<1, 0, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
+                    k_hh_half_cheb<1, 1, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
+                } else {
+                    k_hh_half_cheb<0, 0, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
+                    k_hh_half_cheb<0, 1, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
+                }
+            } else if (prm->gate_mode == 1) {
+                k_hh_half_cheb<1, 0, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
+                k_hh_half_cheb<1, 1, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
             } else {
-                k_hh_half_cheb<0, 0><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
-                k_hh_half_cheb<0, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
+                k_hh_half_cheb<0, 0, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
+                k_hh_half_cheb<0, 1, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
             }
         }
         NW_HIP(hipGetLastError());
@@ -1158,9 +1167,18 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
         const ChebStore cs = cheb_store(p);
         const unsigned grid = (unsigned)p->num_cells;
         hipStream_t s = (hipStream_t)stream;
+        const bool one = p->cheb_maxj == 1;
         for (int k = n_steps - 1; k >= 0; --k) {
-            k_hh_adj_cheb<1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, k == n_steps - 1, 0);
-            k_hh_adj_cheb<0><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);
+            if (one) {
+                k_hh_adj_cheb<1, 1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k,
+                                                                k == n_steps - 1, 0);
+                k_hh_adj_cheb<0, 1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);
+            } else {
+                k_hh_adj_cheb<1, kChebRows><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k,
+                                                                        k == n_steps - 1, 0);
+                k_hh_adj_cheb<0, kChebRows><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, 0,
+                                                                        k == 0);
+            }
         }
         NW_HIP(hipGetLastError());
         return NW_OK;

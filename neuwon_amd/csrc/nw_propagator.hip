@@ -503,9 +503,9 @@ static double tree_spectral_radius(int n, const int32_t* parent, const double* d
 // s is the first slot of a wave (s - 1 is in another wave) or the last (s + 1 is); every other
 // neighbour is "far" and read from LDS.  A row would need three far reads only in the last slot
 // of a wave, as a second child with two children: such a row moves to the next slot, leaving a
-// hole.  Returns the slots in use, or -1 when the layout does not fit (more than kChebRows row
-// groups, or the zero place kChebZero taken); sets J, fills pos (LDS place of every natural row)
-// and the 1024 J places of far/coef.
+// hole.  The last place, J * 1024 - 1, stays empty: it holds the zero that unused far slots read.
+// Returns the slots in use, or -1 when the layout needs more than kChebRows row groups; sets J,
+// fills pos (LDS place of every natural row) and the 1024 J places of far/coef.
 static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, double>>>& adj,
                            double scale, int& J_out, std::vector<uint16_t>& pos,
                            std::vector<uint32_t>& far, std::vector<float4>& coef) {
@@ -522,7 +522,7 @@ static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, d
     if (next != n) return -1;
     auto parent_of = [&](int i) { return (!adj[i].empty() && adj[i][0].first < i) ? adj[i][0].first : -1; };
     std::vector<int32_t> slot(n);
-    int J = (n + kChebBlock - 1) / kChebBlock, size = 0;
+    int J = n / kChebBlock + 1, size = 0;
     for (int attempt = 0; attempt < 4; ++attempt) {
         const int W = kWave * J;
         int s = -1;
@@ -541,11 +541,12 @@ static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, d
         }
         size = s + 1;
         // more row groups never need more holes, so J only grows
-        const int Jn = (size + kChebBlock - 1) / kChebBlock;
+        const int Jn = size / kChebBlock + 1;  // room for the zero place after the last slot
         if (Jn <= J) break;
         J = Jn;
     }
-    if (size > J * kChebBlock || J > kChebRows || (J == kChebRows && size > kChebZero)) return -1;
+    if (size > J * kChebBlock - 1 || J > kChebRows) return -1;
+    const uint32_t zero = (uint32_t)(J * kChebBlock - 1);
     J_out = J;
     const size_t base = far.size();
     far.resize(base + (size_t)J * kChebBlock, kChebHole);
@@ -557,7 +558,7 @@ static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, d
         const uint32_t pl = place_of_slot(s);
         pos[i] = (uint16_t)pl;
         float c4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        uint32_t fp[2] = {(uint32_t)kChebZero, (uint32_t)kChebZero};
+        uint32_t fp[2] = {zero, zero};
         int nfar = 0;
         for (const auto& e : adj[i]) {
             const int sm = slot[place[e.first]];
@@ -720,7 +721,13 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     NW_HIP(hipMemcpy(p->pre_base, pbase.data(), sizeof(int64_t) * p->num_cells, hipMemcpyHostToDevice));
     NW_HIP(hipMemcpy(p->pre_size, psize.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
     p->pre_cells = 0;
-    for (int64_t c = 0; c < p->num_cells; ++c) p->pre_cells += pbase[c] >= 0;
+    p->cheb_maxj = 1;
+    for (int64_t c = 0; c < p->num_cells; ++c) {
+        p->pre_cells += pbase[c] >= 0;
+        const int64_t n = cell_ptr[c + 1] - cell_ptr[c];
+        const int J = pbase[c] >= 0 ? (psize[c] >> 16) : (int)((n + kChebBlock - 1) / kChebBlock);
+        p->cheb_maxj = std::max(p->cheb_maxj, J);
+    }
     // launch order: most work first (series terms x rows per thread, then rows)
     std::vector<int32_t> order(p->num_cells);
     std::iota(order.begin(), order.end(), 0);
@@ -959,8 +966,10 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
 }
 
 // Chebyshev store: x in LDS as FP64, v = sum_k c_k T_k(A~) x (cheb_rows).
-__global__ void __launch_bounds__(kChebBlock) k_half_step_cheb(HalfArgs a, ChebStore cs) {
-    __shared__ double b0[kChebLds];
+template <int MAXJ>
+__global__ void __launch_bounds__(kChebBlock) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+k_half_step_cheb(HalfArgs a, ChebStore cs) {
+    __shared__ double b0[cheb_lds(MAXJ)];
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int n = (int)(a.cell_row0[cell + 1] - row0);
@@ -973,7 +982,7 @@ __global__ void __launch_bounds__(kChebBlock) k_half_step_cheb(HalfArgs a, ChebS
         b0[pre ? pos[i] : i] = half_x(a, row0 + i, dt_ms, dt_s, true);
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell(cs, row0, cell, n, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ>(cs, row0, cell, n, pre_base, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
@@ -1166,8 +1175,12 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
     if (p->chebyshev) {
         NW_REQUIRE(p->max_cell_rows <= kChebBuf, "cell of %d segments exceeds the Chebyshev store",
                    p->max_cell_rows);
-        k_half_step_cheb<<<(unsigned)p->num_cells, kChebBlock, 0, (hipStream_t)stream>>>(
-            a, cheb_store(p));
+        if (p->cheb_maxj == 1)
+            k_half_step_cheb<1><<<(unsigned)p->num_cells, kChebBlock, 0, (hipStream_t)stream>>>(
+                a, cheb_store(p));
+        else
+            k_half_step_cheb<kChebRows><<<(unsigned)p->num_cells, kChebBlock, 0,
+                                          (hipStream_t)stream>>>(a, cheb_store(p));
         NW_HIP(hipGetLastError());
         return NW_OK;
     }
