@@ -551,8 +551,17 @@ class DBClass:
         self.columns = {}
         self.class_attributes = {}
         self.size = 0
-        self.row_of = np.zeros(0, dtype=np.int64)  # creation id -> row
+        self._row_of = np.zeros(16, dtype=np.int64)  # creation id -> row, grown by doubling
         database.classes[name] = self
+
+    @property
+    def row_of(self):
+        """Row of every object by creation id (the identity until the model is built)."""
+        return self._row_of[:self.size]
+
+    @row_of.setter
+    def row_of(self, rows):
+        self._row_of = np.asarray(rows, dtype=np.int64)
 
     def get_name(self):
         return self.name
@@ -587,8 +596,12 @@ class DBClass:
         for col in self.columns.values():
             col.append()
         cid = self.size
+        if cid >= len(self._row_of):  # amortised O(1): objects are created one at a time
+            grown = np.empty(max(16, 2 * len(self._row_of)), dtype=np.int64)
+            grown[:cid] = self._row_of[:cid]
+            self._row_of = grown
+        self._row_of[cid] = cid
         self.size += 1
-        self.row_of = np.append(self.row_of, cid)
         return cid
 
 
