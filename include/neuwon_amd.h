@@ -83,9 +83,12 @@ int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_pt
  * c_0 = e^(-rho/2) I_0(rho/2), c_k = 2 e^(-rho/2) I_k(rho/2), truncated where the remaining
  * coefficients fall below 1e-10 (nw_chebyshev_exp_coefficients).  Nothing is pruned: the
  * product is the reference's exp(A dt) x at its default threshold (electric.py:114-115, machine
- * epsilon) to about 1e-10 of the input's magnitude.  Each row keeps up to 3 neighbours (more
- * in an overflow list); cells up to 5120 segments.  Read by nw_hh_step and
- * nw_electric_half_step; nw_hh_adjoint and nw_ca_step refuse it. */
+ * epsilon) to about 1e-10 of the input's magnitude.  Cells whose rows have at most 3
+ * neighbours run in depth-first preorder slots (the rows next to a row's slot from registers or
+ * the neighbouring lane, two far neighbours from LDS); other cells keep up to 3 neighbours per
+ * row in natural order (more in an overflow list); cells up to 5120 segments.  Read by
+ * nw_hh_step (which also writes the adjoint's record), nw_electric_half_step and nw_hh_adjoint
+ * (P^T y = C P (C^-1 y), the same series); nw_ca_step refuses it. */
 int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* out);
 /* Chebyshev store: the spectrum bound rho and series coefficients of one cell
  * (coef_host may be NULL: only *K_out, the index of the last kept coefficient, and *rho_out). */
@@ -254,7 +257,9 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
  * Reverse pass of the fused step (no counterpart in the reference, which has no differentiation:
  * it is the adjoint of nw_hh_step with table gates and a constant stimulus).  `record` is the
  * record nw_hh_step wrote for the same n_steps (num_rows-long arrays v, m, h, n at the start of
- * every step and v after its first half step, 5 per step, then v, m, h, n after the last step).
+ * every step and v after its first half step, 5 per step, then v, m, h, n after the last step),
+ * with any store; `p` may be any store too (the Chebyshev store applies P^T y = C P (C^-1 y) as
+ * the same series, one launch per half step).
  * Loss: sum over the steps of (v[soma_local[cell]] after the step - target[cell*stride + step])^2.
  * lam_v/lam_m/lam_h/lam_n (FP32, per segment) carry the adjoint of the state from the following
  * launch (zeros before the last one); grad_na/k/l (FP64, per segment) and grad_theta (FP64,
