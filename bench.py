@@ -190,10 +190,11 @@ def precision(pop, cells, args, v_ref, trace_ref, r_end, total_steps):
     """Voltage error of the GPU against the FP64 oracle on the sampled cells: (1) the full state
     of the timed population after all its steps; (2) every cell's root voltage after every step,
     from the same cells advanced again as a population of their own in one launch (cells are
-    independent; the rerun differs from the timed run only by the launch split, which moves the
-    per-cell reference voltage, and the order of the symmetric store's atomics: rerun_max_abs_diff
-    reports by how much), split into subthreshold samples (relative) and samples within 2 ms of a
-    spike (absolute)."""
+    independent; the rerun differs from the timed run only by the launch split — the general
+    store moves its per-cell reference voltage, the Chebyshev store rounds the voltage to FP32
+    between calls — and the order of the symmetric store's atomics: rerun_max_abs_diff reports by
+    how much), split into subthreshold samples (relative) and samples within 2 ms of a spike
+    (absolute)."""
     import torch
     from neuwon_amd import population
     v_gpu = pop.v.cpu().numpy().astype(np.float64)[:r_end]

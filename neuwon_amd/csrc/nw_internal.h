@@ -407,6 +407,7 @@ struct nw_propagator {
     int64_t* pre_base = nullptr;   // [num_cells]
     int32_t* pre_size = nullptr;   // [num_cells] slots in use (rows and holes) | J << 16
     int64_t pre_cells = 0;         // cells laid out in preorder
+    int64_t pre_places = 0;        // entries of pre_far / pre_coef
     int32_t cheb_maxj = 0;         // row groups per thread the store's cells need (1..kChebRows)
     double mean_terms = 0.0;
     // host mirrors

@@ -711,6 +711,7 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
         pcoef.push_back(float4{0.0f, 0.0f, 0.0f, 0.0f});
     }
     NW_HIP(hipMalloc(&p->pre_pos, sizeof(uint16_t) * std::max<int64_t>(N, 1)));
+    p->pre_places = (int64_t)pfar.size();
     NW_HIP(hipMalloc(&p->pre_far, sizeof(uint32_t) * pfar.size()));
     NW_HIP(hipMalloc(&p->pre_coef, sizeof(float4) * pcoef.size()));
     NW_HIP(hipMalloc(&p->pre_base, sizeof(int64_t) * std::max<int64_t>(p->num_cells, 1)));
@@ -1061,10 +1062,12 @@ int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
                       (p->num_rows + p->num_cells) * (int64_t)sizeof(uint32_t) +
                       p->num_rows * (int64_t)sizeof(double) * (p->symmetric ? 2 : 1) +
                       p->num_cells * (int64_t)(2 * sizeof(int64_t) + sizeof(int32_t));
-    if (p->chebyshev)
-        s->device_bytes = p->num_rows * (int64_t)(3 * sizeof(int32_t) + 3 * sizeof(float)) +
+    if (p->chebyshev)  // natural-order neighbours, preorder slots, series, FP64 voltages
+        s->device_bytes = p->num_rows * (int64_t)(3 * sizeof(int32_t) + 3 * sizeof(float) +
+                                                  sizeof(uint16_t) + sizeof(double)) +
+                          p->pre_places * (int64_t)(sizeof(uint32_t) + sizeof(float4)) +
                           (int64_t)p->h_cheb_c.size() * (int64_t)sizeof(double) +
-                          p->num_cells * (int64_t)(2 * sizeof(int64_t) + 2 * sizeof(int32_t));
+                          p->num_cells * (int64_t)(3 * sizeof(int64_t) + 3 * sizeof(int32_t));
     return NW_OK;
 }
 
