@@ -304,49 +304,104 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     const int stim_row = a.stim_local ? a.stim_local[cell] : -1;
     const bool g_valid = HALF == 1 || p.conductances_valid || step > 0;
     const int64_t NR = a.num_rows;
-    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
-        const int64_t gi = row0 + i;
-        const double v = from_v32 ? (double)a.v[gi] : v64[gi];
-        double g = 0.0, d = 0.0;
-        if (a.record) {  // the record of k_hh_step: v, m, h, n at the start, v after half A
-            float* rec = a.record + (int64_t)step * 5 * NR + gi;
-            if (HALF == 0) {
-                rec[0] = (float)v;
-                rec[NR] = a.m[gi];
-                rec[2 * NR] = a.h[gi];
-                rec[3 * NR] = a.n[gi];
-            } else {
-                rec[4 * NR] = (float)v;
+    // The thread's rows i = threadIdx.x + j * 1024 (j < MAXJ) in groups of at most three, each in
+    // phases so that the loads of the group are in flight together: the per-segment values, then
+    // (table gates) the gate table entries at each row's voltage, then the arithmetic.
+    constexpr int R = MAXJ;
+    constexpr int G = R < 3 ? R : 3;
+#pragma unroll
+    for (int j0 = 0; j0 < R; j0 += G) {
+        double v[G];
+        float fm[G], fh[G], fn[G], fgna[G], fgk[G], fgl[G], fic[G];
+        int place[G];
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) {
+            const int j = j0 + jj;
+            const int i = threadIdx.x + j * kChebBlock;
+            if (j < R && i < nrows) {
+                const int64_t gi = row0 + i;
+                v[jj] = from_v32 ? (double)a.v[gi] : v64[gi];
+                fm[jj] = a.m[gi];
+                fh[jj] = a.h[gi];
+                fn[jj] = a.n[gi];
+                fgna[jj] = a.g_na[gi];
+                fgk[jj] = a.g_k[gi];
+                fgl[jj] = a.g_l[gi];
+                fic[jj] = a.inv_c[gi];
+                place[jj] = pre ? (int)pos[i] : i;
             }
         }
-        if (HALF == 1) {
-            double m = a.m[gi], h = a.h[gi], n = a.n[gi];
-            if (GATE_MODE == 1) {
-                hh_exact(v, (double)p.q10, (double)p.dt_ms, m, h, n);
-            } else {
-                const int nb = p.num_nodes - 1;
-                const float2* tm = reinterpret_cast<const float2*>(a.table);
-                int b;
-                double f;
-                bucket_of(v, (double)p.vmin, (double)nb / ((double)p.vmax - (double)p.vmin), nb, b, f);
-                m = gate_advance(tm, b, f, m);
-                h = gate_advance(tm + p.num_nodes, b, f, h);
-                n = gate_advance(tm + 2 * p.num_nodes, b, f, n);
+        float2 tab[G][3][2];
+        double frac[G];
+        if (HALF == 1 && GATE_MODE == 0) {
+            const int nb = p.num_nodes - 1;
+            const double frq = (double)nb / ((double)p.vmax - (double)p.vmin);
+            const float2* tm = reinterpret_cast<const float2*>(a.table);
+#pragma unroll
+            for (int jj = 0; jj < G; ++jj) {
+                const int j = j0 + jj;
+                const int i = threadIdx.x + j * kChebBlock;
+                if (j < R && i < nrows) {
+                    int bk;
+                    bucket_of(v[jj], (double)p.vmin, frq, nb, bk, frac[jj]);
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        tab[jj][q][0] = tm[q * p.num_nodes + bk];
+                        tab[jj][q][1] = tm[q * p.num_nodes + bk + 1];
+                    }
+                }
             }
-            const float m32 = (float)m, h32 = (float)h, n32 = (float)n;
-            a.m[gi] = m32;
-            a.h[gi] = h32;
-            a.n[gi] = n32;
-            hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], m32, h32, n32, p.e_na, p.e_k, p.e_l,
-                           g, d);
-        } else if (g_valid) {
-            hh_conductance(a.g_na[gi], a.g_k[gi], a.g_l[gi], a.m[gi], a.h[gi], a.n[gi], p.e_na,
-                           p.e_k, p.e_l, g, d);
         }
-        const double invc = a.inv_c[gi];
-        const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick) : 0.0;
-        const double alpha = exp(-hdt_s * g * invc);
-        b0[pre ? pos[i] : i] = d - (d - v) * alpha + hdt * I * invc;
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) {
+            const int j = j0 + jj;
+            const int i = threadIdx.x + j * kChebBlock;
+            if (j >= R || i >= nrows) continue;
+            const int64_t gi = row0 + i;
+            double g = 0.0, d = 0.0;
+            if (a.record) {  // the record of k_hh_step: v, m, h, n at the start, v after half A
+                float* rec = a.record + (int64_t)step * 5 * NR + gi;
+                if (HALF == 0) {
+                    rec[0] = (float)v[jj];
+                    rec[NR] = fm[jj];
+                    rec[2 * NR] = fh[jj];
+                    rec[3 * NR] = fn[jj];
+                } else {
+                    rec[4 * NR] = (float)v[jj];
+                }
+            }
+            if (HALF == 1) {
+                double m = fm[jj], h = fh[jj], n = fn[jj];
+                if (GATE_MODE == 1) {
+                    hh_exact(v[jj], (double)p.q10, (double)p.dt_ms, m, h, n);
+                } else {  // open' = a + b open, (a, b) interpolated between the two table nodes
+                    double* x3[3] = {&m, &h, &n};
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const float2 t0 = tab[jj][q][0], t1 = tab[jj][q][1];
+                        const double f = frac[jj];
+                        const double ga = fma(f, (double)t1.x - (double)t0.x, (double)t0.x);
+                        const double gb = fma(f, (double)t1.y - (double)t0.y, (double)t0.y);
+                        *x3[q] = fma(gb, *x3[q], ga);
+                    }
+                }
+                fm[jj] = (float)m;
+                fh[jj] = (float)h;
+                fn[jj] = (float)n;
+                a.m[gi] = fm[jj];
+                a.h[gi] = fh[jj];
+                a.n[gi] = fn[jj];
+                hh_conductance(fgna[jj], fgk[jj], fgl[jj], fm[jj], fh[jj], fn[jj], p.e_na, p.e_k,
+                               p.e_l, g, d);
+            } else if (g_valid) {
+                hh_conductance(fgna[jj], fgk[jj], fgl[jj], fm[jj], fh[jj], fn[jj], p.e_na, p.e_k,
+                               p.e_l, g, d);
+            }
+            const double invc = fic[jj];
+            const double I = (i == stim_row) ? (double)stim_current(p, (uint32_t)cell, tick) : 0.0;
+            const double alpha = exp(-hdt_s * g * invc);
+            b0[place[jj]] = d - (d - v[jj]) * alpha + hdt * I * invc;
+        }
     }
     __syncthreads();
     double out[kChebRows];
@@ -359,9 +414,12 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     }
     __syncthreads();
     const bool closes = HALF == 1 && a.record && step == a.n_steps - 1;
-    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = threadIdx.x + j * kChebBlock;
+        if (i >= nrows) continue;
         const int64_t gi = row0 + i;
-        const double vi = b0[pre ? pos[i] : i];
+        const double vi = b0[pre ? (int)pos[i] : i];
         v64[gi] = vi;
         if (HALF == 1) a.v[gi] = (float)vi;
         if (closes) {  // the state after the last step closes the record
