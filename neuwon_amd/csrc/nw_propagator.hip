@@ -906,18 +906,22 @@ struct HalfArgs {
 // FP32 storage, FP64 arithmetic: x is formed in FP64 and staged in LDS as an FP32 offset from the
 // cell's reference voltage (x of its first segment); the row products accumulate in FP64
 // (spmv_rows).
-__device__ __forceinline__ double half_x(const HalfArgs& a, int64_t gi, double dt_ms, double dt_s,
-                                         bool write_integral) {
-    const double invc = a.inv_c[gi];
-    const double d = a.drive_v[gi];
-    const double v = a.v[gi];
-    const double dv = dt_ms * (double)a.sum_i[gi] * invc;
-    const double expo = -dt_s * (double)a.sum_g[gi] * invc;
+__device__ __forceinline__ double half_x_of(const HalfArgs& a, int64_t gi, double v, double d,
+                                            double sum_i, double sum_g, double invc, double dt_ms,
+                                            double dt_s, bool write_integral) {
+    const double dv = dt_ms * sum_i * invc;
+    const double expo = -dt_s * sum_g * invc;
     const double alpha = exp(expo);
     const double diff = d - v;
     if (write_integral && a.integral_v)
         a.integral_v[gi] = (float)(dt_s * d - expo * diff * alpha + 0.5 * dt_s * dv);
     return d - diff * alpha + dv;
+}
+
+__device__ __forceinline__ double half_x(const HalfArgs& a, int64_t gi, double dt_ms, double dt_s,
+                                         bool write_integral) {
+    return half_x_of(a, gi, a.v[gi], a.drive_v[gi], a.sum_i[gi], a.sum_g[gi], a.inv_c[gi], dt_ms,
+                     dt_s, write_integral);
 }
 
 // SYM = false: x - vref and the product in LDS as FP32 (spmv_rows).  SYM = true (symmetric store):
@@ -979,8 +983,34 @@ k_half_step_cheb(HalfArgs a, ChebStore cs) {
     const int64_t pre_base = cs.pre_base[cell];
     const bool pre = pre_base >= 0;
     const uint16_t* pos = cs.pre_pos + row0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        b0[pre ? pos[i] : i] = half_x(a, row0 + i, dt_ms, dt_s, true);
+    // the thread's rows (i = threadIdx.x + j * 1024, j < MAXJ) in groups of at most three whose
+    // loads are issued together, then the arithmetic
+    constexpr int G = MAXJ < 3 ? MAXJ : 3;
+#pragma unroll
+    for (int j0 = 0; j0 < MAXJ; j0 += G) {
+        float fv[G], fd[G], fi[G], fg[G], fc[G];
+        int place[G];
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) {
+            const int i = threadIdx.x + (j0 + jj) * kChebBlock;
+            if (j0 + jj < MAXJ && i < n) {
+                const int64_t gi = row0 + i;
+                fv[jj] = a.v[gi];
+                fd[jj] = a.drive_v[gi];
+                fi[jj] = a.sum_i[gi];
+                fg[jj] = a.sum_g[gi];
+                fc[jj] = a.inv_c[gi];
+                place[jj] = pre ? (int)pos[i] : i;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) {
+            const int i = threadIdx.x + (j0 + jj) * kChebBlock;
+            if (j0 + jj >= MAXJ || i >= n) continue;
+            b0[place[jj]] = half_x_of(a, row0 + i, fv[jj], fd[jj], fi[jj], fg[jj], fc[jj], dt_ms,
+                                      dt_s, true);
+        }
+    }
     __syncthreads();
     double out[kChebRows];
     const int places = cheb_cell<MAXJ>(cs, row0, cell, n, pre_base, b0, out);
