@@ -62,6 +62,10 @@ typedef struct nw_propagator_stats {
     int32_t symmetric;       /* 1: symmetric format (see nw_propagator_build_ex)                  */
     int32_t chebyshev;       /* 1: no stored matrix, a Chebyshev series of the tree operator      */
     double  mean_terms;      /* Chebyshev store: series terms per row and product (row-weighted)  */
+    int32_t workgroup_threads; /* Chebyshev store: threads per workgroup of its launches: 512 when
+                                  every cell fits one row group of 512 slots, else 1024; 0 for the
+                                  stored formats                                                  */
+    int32_t row_groups;      /* Chebyshev store: row groups per thread its launches are built for */
 } nw_propagator_stats;
 
 int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_host,

@@ -22,7 +22,7 @@ class NwPropagatorStats(ctypes.Structure):
     _fields_ = [("num_cells", c_i64), ("num_rows", c_i64), ("nnz", c_i64), ("stored", c_i64),
                 ("max_cell_rows", c_i32), ("max_row_nnz", c_i32), ("build_seconds", c_f64),
                 ("device_bytes", c_i64), ("symmetric", c_i32), ("chebyshev", c_i32),
-                ("mean_terms", c_f64)]
+                ("mean_terms", c_f64), ("workgroup_threads", c_i32), ("row_groups", c_i32)]
 
 
 class NwHHParams(ctypes.Structure):

@@ -409,6 +409,7 @@ struct nw_propagator {
     int64_t pre_cells = 0;         // cells laid out in preorder
     int64_t pre_places = 0;        // entries of pre_far / pre_coef
     int32_t cheb_maxj = 0;         // row groups per thread the store's cells need (1..kChebRows)
+    int32_t cheb_small = 0;        // every cell preorder, one row group of <= kChebSmallBlock slots
     double mean_terms = 0.0;
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;
@@ -442,6 +443,9 @@ inline ChebStore cheb_store(const nw_propagator* p) {
 // Rows per thread of the Chebyshev products (cells up to kChebBlock * kChebRows segments).
 constexpr int kChebBlock = 1024;
 constexpr int kChebRows = 5;
+// Workgroup of the launches over stores of small cells (every cell preorder, one row group, at
+// most this many slots): four workgroups per CU instead of two 1024-thread ones half idle.
+constexpr int kChebSmallBlock = 512;
 
 // One Chebyshev product v = sum_k c_k T_k(A~) x for the rows this thread owns (row
 // threadIdx.x + j * kChebBlock), A~ t = t + sum_j coef_ij (t_j - t_i) over the row's neighbours.
@@ -588,12 +592,12 @@ __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R,
 // K < kChebCoefCap).
 // On entry lds[j * 1024 + t] = x of the row in slot J t + j (after a barrier); on return out[j]
 // holds that row's result and the call has ended with a barrier.
-template <int J, typename CT, int BUF>
+template <int J, typename CT, int BUF, int NT>
 __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, const float4* __restrict__ cf,
                                               const double* __restrict__ c, int K, int size, double* lds,
                                               double (&out)[kChebRows]) {
     constexpr int zero = J * kChebBlock - 1;
-    for (int k = threadIdx.x; k <= K; k += kChebBlock) lds[2 * BUF + k] = c[k];
+    for (int k = threadIdx.x; k <= K; k += NT) lds[2 * BUF + k] = c[k];
     if (threadIdx.x == 0) {
         lds[zero] = 0.0;
         lds[BUF + zero] = 0.0;
@@ -640,31 +644,37 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
 
 // The product for one cell, preorder slots (base >= 0) or natural order, from x in LDS (at the
 // row's LDS place) to out[] (the result of the row at LDS place threadIdx.x + j * 1024), in a
-// launch built for MAXJ row groups (the store's cells need no more).  Returns the number of LDS
-// places in use: out[j] is defined where threadIdx.x + j * 1024 is below it.
-template <int MAXJ>
+// launch built for MAXJ row groups (the store's cells need no more) and NT threads per
+// workgroup.  NT = kChebSmallBlock serves stores whose cells are all preorder in one row group of
+// at most NT slots (nw_propagator::cheb_small): the LDS places stay those of the 1024-thread
+// layout, threads NT..1023 would own only empty slots and do not exist.  Returns the number of
+// LDS places in use: out[j] is defined where threadIdx.x + j * 1024 is below it.
+template <int MAXJ, int NT = kChebBlock>
 __device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int cell, int n, int64_t base,
                                          double* lds, double (&out)[kChebRows]) {
+    static_assert(NT == kChebBlock || MAXJ == 1, "small workgroups serve one row group");
     constexpr int BUF = MAXJ * kChebBlock;
     const double* c = cs.c + cs.off[cell];
     const int K = cs.K[cell];
-    if (base < 0) {
-        cheb_rows<MAXJ>(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef, c, K,
-                        n, lds, lds + BUF, out);
-        return n;
+    if constexpr (NT == kChebBlock) {
+        if (base < 0) {
+            cheb_rows<MAXJ>(cs.nb_idx + 3 * row0, cs.nb_coef + 3 * row0, cs.ov_idx, cs.ov_coef, c,
+                            K, n, lds, lds + BUF, out);
+            return n;
+        }
     }
     const uint32_t* far = cs.pre_far + base;
     const float4* cf = cs.pre_coef + base;
     const int size = cs.pre_size[cell] & 0xFFFF;
     const int J = cs.pre_size[cell] >> 16;
     if constexpr (MAXJ == 1) {
-        cheb_pre_rows<1, double, BUF>(far, cf, c, K, size, lds, out);
+        cheb_pre_rows<1, double, BUF, NT>(far, cf, c, K, size, lds, out);
     } else {
-        if (J <= 1) cheb_pre_rows<1, double, BUF>(far, cf, c, K, size, lds, out);
-        else if (J == 2) cheb_pre_rows<2, double, BUF>(far, cf, c, K, size, lds, out);
-        else if (J == 3) cheb_pre_rows<3, double, BUF>(far, cf, c, K, size, lds, out);
-        else if (J == 4) cheb_pre_rows<4, double, BUF>(far, cf, c, K, size, lds, out);
-        else cheb_pre_rows<5, float, BUF>(far, cf, c, K, size, lds, out);
+        if (J <= 1) cheb_pre_rows<1, double, BUF, NT>(far, cf, c, K, size, lds, out);
+        else if (J == 2) cheb_pre_rows<2, double, BUF, NT>(far, cf, c, K, size, lds, out);
+        else if (J == 3) cheb_pre_rows<3, double, BUF, NT>(far, cf, c, K, size, lds, out);
+        else if (J == 4) cheb_pre_rows<4, double, BUF, NT>(far, cf, c, K, size, lds, out);
+        else cheb_pre_rows<5, float, BUF, NT>(far, cf, c, K, size, lds, out);
     }
     return J * kChebBlock;
 }

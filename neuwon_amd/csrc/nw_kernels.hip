@@ -286,8 +286,8 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 // of 1024-thread blocks.  Voltages pass between launches as FP64 (v64); the FP32 voltage array
 // is written after every step.
 // ----------------------------------------------------------------------------------------------
-template <int GATE_MODE, int HALF, int MAXJ>
-__global__ void __launch_bounds__(kChebBlock) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     __shared__ double b0[cheb_lds(MAXJ)];
     const nw_hh_params& p = a.prm;
@@ -405,7 +405,7 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell<MAXJ>(a.cheb, row0, cell, nrows, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ, NT>(a.cheb, row0, cell, nrows, pre_base, b0, out);
     // results to LDS (the row's place), then out in natural order
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
@@ -694,11 +694,11 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
 // as the forward product.  The voltage adjoint passes between launches as FP64 (lam64); a call
 // reads it from lam_v at its first launch and leaves it there after its last.
 // ----------------------------------------------------------------------------------------------
-template <int HALF, int MAXJ>
-__global__ void __launch_bounds__(kChebBlock) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+template <int HALF, int MAXJ, int NT = kChebBlock>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last) {
     __shared__ double b0[cheb_lds(MAXJ)];
-    __shared__ double red[7][kChebBlock / kWave];
+    __shared__ double red[7][NT / kWave];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -712,7 +712,7 @@ k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last
     const float* nxt = rec + 5 * NR;  // state at the start of the following step
     const int soma = a.soma_local[cell];
     double loss = 0.0;
-    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+    for (int i = threadIdx.x; i < nrows; i += NT) {
         const int64_t gi = row0 + i;
         double lam = first ? (double)a.lam_v[gi] : lam64[gi];
         if (HALF == 1 && i == soma) {
@@ -724,7 +724,7 @@ k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell<MAXJ>(cs, row0, cell, nrows, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ, NT>(cs, row0, cell, nrows, pre_base, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
@@ -736,7 +736,7 @@ k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last
     const double frq = (double)(p.num_nodes - 1) / ((double)p.vmax - (double)p.vmin);
     const double e_na = p.e_na, e_k = p.e_k, e_l = p.e_l;
     double th[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < nrows; i += kChebBlock) {
+    for (int i = threadIdx.x; i < nrows; i += NT) {
         const int64_t gi = row0 + i;
         const double invc = a.inv_c[gi];
         const double lx = b0[pre ? pos[i] : i] / invc;  // (P^T lambda)_i
@@ -794,7 +794,7 @@ k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last
         if (threadIdx.x == 0) {
             for (int q = 0; q < 7; ++q) {
                 double t = 0.0;
-                for (int w = 0; w < kChebBlock / kWave; ++w) t += red[q][w];
+                for (int w = 0; w < NT / kWave; ++w) t += red[q][w];
                 if (q < 6) a.grad_theta[(int64_t)cell * 6 + q] += t;
                 else a.loss[cell] += t;
             }
@@ -1158,8 +1158,17 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
         const unsigned grid = (unsigned)p->num_cells;
         double* v64 = p->cheb_v64;
         const bool one = p->cheb_maxj == 1;
+        constexpr int SB = kChebSmallBlock;
         for (int st = 0; st < n_steps; ++st) {
-            if (one) {
+            if (p->cheb_small) {
+                if (prm->gate_mode == 1) {
+                    k_hh_half_cheb<1, 0, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, st == 0);
+                    k_hh_half_cheb<1, 1, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, 0);
+                } else {
+                    k_hh_half_cheb<0, 0, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, st == 0);
+                    k_hh_half_cheb<0, 1, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, 0);
+                }
+            } else if (one) {
                 if (prm->gate_mode == 1) {
                     k_hh_half_cheb<1, 0, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
                     k_hh_half_cheb<1, 1, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
@@ -1226,8 +1235,13 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
         const unsigned grid = (unsigned)p->num_cells;
         hipStream_t s = (hipStream_t)stream;
         const bool one = p->cheb_maxj == 1;
+        constexpr int SB = kChebSmallBlock;
         for (int k = n_steps - 1; k >= 0; --k) {
-            if (one) {
+            if (p->cheb_small) {
+                k_hh_adj_cheb<1, 1, SB><<<grid, SB, 0, s>>>(a, cs, p->cheb_v64, k,
+                                                            k == n_steps - 1, 0);
+                k_hh_adj_cheb<0, 1, SB><<<grid, SB, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);
+            } else if (one) {
                 k_hh_adj_cheb<1, 1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k,
                                                                 k == n_steps - 1, 0);
                 k_hh_adj_cheb<0, 1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);

@@ -723,11 +723,13 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     NW_HIP(hipMemcpy(p->pre_size, psize.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
     p->pre_cells = 0;
     p->cheb_maxj = 1;
+    p->cheb_small = 1;
     for (int64_t c = 0; c < p->num_cells; ++c) {
         p->pre_cells += pbase[c] >= 0;
         const int64_t n = cell_ptr[c + 1] - cell_ptr[c];
         const int J = pbase[c] >= 0 ? (psize[c] >> 16) : (int)((n + kChebBlock - 1) / kChebBlock);
         p->cheb_maxj = std::max(p->cheb_maxj, J);
+        if (pbase[c] < 0 || J != 1 || (psize[c] & 0xFFFF) > kChebSmallBlock) p->cheb_small = 0;
     }
     // launch order: most work first (series terms x rows per thread, then rows)
     std::vector<int32_t> order(p->num_cells);
@@ -971,8 +973,8 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
 }
 
 // Chebyshev store: x in LDS as FP64, v = sum_k c_k T_k(A~) x (cheb_rows).
-template <int MAXJ>
-__global__ void __launch_bounds__(kChebBlock) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+template <int MAXJ, int NT = kChebBlock>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_half_step_cheb(HalfArgs a, ChebStore cs) {
     __shared__ double b0[cheb_lds(MAXJ)];
     const int cell = a.order[blockIdx.x];
@@ -1013,14 +1015,14 @@ k_half_step_cheb(HalfArgs a, ChebStore cs) {
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell<MAXJ>(cs, row0, cell, n, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ, NT>(cs, row0, cell, n, pre_base, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
         if (r < places) b0[r] = out[j];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) a.v[row0 + i] = (float)b0[pre ? pos[i] : i];
+    for (int i = threadIdx.x; i < n; i += NT) a.v[row0 + i] = (float)b0[pre ? pos[i] : i];
 }
 
 }  // namespace nw
@@ -1088,6 +1090,8 @@ int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
     s->symmetric = p->symmetric;
     s->chebyshev = p->chebyshev;
     s->mean_terms = p->mean_terms;
+    s->workgroup_threads = p->chebyshev ? (p->cheb_small ? kChebSmallBlock : kChebBlock) : 0;
+    s->row_groups = p->chebyshev ? (p->cheb_maxj == 1 ? 1 : kChebRows) : 0;
     s->device_bytes = p->vals_capacity * (int64_t)(sizeof(float) + sizeof(uint16_t)) +
                       (p->num_rows + p->num_cells) * (int64_t)sizeof(uint32_t) +
                       p->num_rows * (int64_t)sizeof(double) * (p->symmetric ? 2 : 1) +
@@ -1208,7 +1212,10 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
     if (p->chebyshev) {
         NW_REQUIRE(p->max_cell_rows <= kChebBuf, "cell of %d segments exceeds the Chebyshev store",
                    p->max_cell_rows);
-        if (p->cheb_maxj == 1)
+        if (p->cheb_small)
+            k_half_step_cheb<1, kChebSmallBlock><<<(unsigned)p->num_cells, kChebSmallBlock, 0,
+                                                   (hipStream_t)stream>>>(a, cheb_store(p));
+        else if (p->cheb_maxj == 1)
             k_half_step_cheb<1><<<(unsigned)p->num_cells, kChebBlock, 0, (hipStream_t)stream>>>(
                 a, cheb_store(p));
         else

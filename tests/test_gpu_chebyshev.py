@@ -155,7 +155,66 @@ def test_chebyshev_cell_sizes_across_row_groups(torch_gpu):
     per = [float("%.1e" % rel[int(pop.cell_ptr[c]):int(pop.cell_ptr[c + 1])].max())
            for c in range(pop.num_cells)]
     _say(f"sizes {sizes}: max rel per cell {per}, spikes {pop.ap_count.cpu().numpy().tolist()}")
+    assert pop.propagator.stats["workgroup_threads"] == 1024
+    assert pop.propagator.stats["row_groups"] == 5
     assert rel.max() < 1e-5
+
+
+def test_chebyshev_small_cells_half_workgroups(torch_gpu):
+    """A population whose cells all fit one row group of 512 slots runs 512-thread workgroups
+    (four per CU): cells of 1, 2, 63, 64, 65, 127, 128, 300 and 500 segments, 0.5 nA constant
+    current at every root, 200 steps.  The same cells beside a 600-segment cell run 1024-thread
+    workgroups: every voltage bit for bit the same (each thread does the same arithmetic on the
+    same LDS places).  Against the FP64 oracle with the unpruned propagator: spike counts, root
+    traces at the FP32-storage floor of strongly driven small cells (subthreshold relative 1e-4;
+    2e-2 mV near spikes, the population bound) and every segment at the end within 2e-3 mV."""
+    import scipy.sparse
+    from neuwon_amd import population, synth
+    torch = torch_gpu
+    rng = np.random.default_rng(22)
+    sizes = [1, 2, 63, 64, 65, 127, 128, 300, 500]
+    cells = [synth.random_cell(n, rng) for n in sizes]
+    steps = 200
+    runs = []
+    for extra, threads in (([], 512), ([synth.random_cell(600, rng)], 1024)):
+        pop = population.HHPopulation(cells + extra, store="chebyshev",
+                                      stim={"mode": "constant", "amp_na": 0.5})
+        assert pop.propagator.stats["workgroup_threads"] == threads
+        assert pop.propagator.stats["row_groups"] == 1
+        trace = torch.zeros((pop.num_cells, steps), dtype=torch.float32, device=pop.dev)
+        pop.advance(steps, trace=trace)
+        torch.cuda.synchronize()
+        runs.append((pop, trace.cpu().numpy().astype(np.float64), pop.v.cpu().numpy().astype(np.float64)))
+    pop, tr, v = runs[0]
+    n = int(pop.cell_ptr[-1])
+    dv = np.abs(v - runs[1][2][:n]).max()
+    blocks = []
+    for c in range(pop.num_cells):
+        r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
+        blocks.append(population.cell_propagator_f64(pop.parent_local[r0:r1], pop.R[r0:r1],
+                                                     pop.C[r0:r1], 0.05, 0.0))
+    P = scipy.sparse.block_diag(blocks, format="csr")
+    co = O.CHHPopulation(pop.cell_ptr, P, pop.C, pop.g_bar, dt=pop.dt, celsius=pop.celsius,
+                         gate_mode="table", stim=("constant", 0.5e-9))
+    ref = np.empty((pop.num_cells, steps))
+    roots = pop.cell_ptr[:-1]
+    v0 = co.v[roots].copy()
+    for k in range(steps):
+        co.advance(1, threads=len(os.sched_getaffinity(0)))
+        ref[:, k] = co.v[roots]
+    # the initial voltage heads each trace, so a spike in the first step counts as a crossing
+    errs = [voltage_error(np.r_[v0[c], tr[c]], np.r_[v0[c], ref[c]], dt=0.1)
+            for c in range(pop.num_cells)]
+    _say(f"sizes {sizes}: 512 vs 1024 threads max |dv| {dv:.2e} mV; subthreshold rel per cell "
+         f"{[float('%.1e' % e['rel_subthreshold']) for e in errs]}, near spikes "
+         f"{max(e['abs_spiking_mV'] for e in errs):.2e} mV, spikes "
+         f"{pop.ap_count.cpu().numpy().tolist()}, end max |dv| {np.abs(v - co.v).max():.2e} mV")
+    assert dv == 0.0
+    assert np.array_equal(pop.ap_count.cpu().numpy(), [e["spikes"] for e in errs])
+    for e in errs:
+        assert e["rel_subthreshold"] < 1e-4
+        assert e["abs_spiking_mV"] < 2e-2
+    assert np.abs(v - co.v).max() < 2e-3
 
 
 def test_chebyshev_large_cell(torch_gpu):
