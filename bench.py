@@ -26,7 +26,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per compartment-step of the step kernel on this workload, per propagator format, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs scaled by what the same counters read for a
 # plain 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); None where not measured
-MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3, "chebyshev": 151.0}
+MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3, "chebyshev": 151.0,
+                                  "rational": None}
 # Kept entries per row of the propagator pruned at 1e-9 (pair rule), this seeded 10M workload:
 # the canonical algorithmic count of the Chebyshev store, which keeps no matrix, uses it
 NNZ_BAR_PRUNED_1E9 = 567.99
@@ -50,6 +51,8 @@ def format_bytes(fmt, nnz_bar):
         return 2.0 * (6.0 * (nnz_bar + 1.0) / 2.0 + 20.0) + 68.0
     if fmt == "chebyshev":   # per half step the LDS place, 2 far places and 4 FP32 coefficients,
         return 2.0 * (2.0 + 4.0 + 16.0 + 16.0) + 68.0   # FP64 v in/out
+    if fmt == "rational":    # per half step the level place, link, 2 FP32 edge coefficients, five
+        return 2.0 * (2.0 + 4.0 + 8.0 + 5 * 16.0 + 16.0) + 68.0   # FP64 pivots, FP64 v in/out
     return 2.0 * (6.0 * nnz_bar + 12.0) + 68.0
 
 
@@ -60,12 +63,16 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--compartments", type=float, default=1e7)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--dt", type=float, default=0.1, help="time step (ms) of the population")
     ap.add_argument("--launch-steps", type=int, default=0,
                     help="steps per kernel launch (0: all timed steps in one launch)")
     ap.add_argument("--block-threads", type=int, default=0)
-    ap.add_argument("--format", choices=("symmetric", "general", "chebyshev"), default="chebyshev",
-                    help="propagator store: symmetric half (S = W P W^-1), general CSR, or "
-                         "chebyshev (no stored matrix: series of the tree operator, unpruned)")
+    ap.add_argument("--format", choices=("symmetric", "general", "chebyshev", "rational"),
+                    default="chebyshev",
+                    help="propagator store: symmetric half (S = W P W^-1), general CSR, "
+                         "chebyshev (no stored matrix: series of the tree operator, unpruned) or "
+                         "rational (no stored matrix: five-pole rational approximation, exact "
+                         "tree solves, unpruned)")
     ap.add_argument("--cpu-sample-cells", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -224,7 +231,7 @@ def precision(pop, cells, args, v_ref, trace_ref, r_end, total_steps):
                           "spike of the oracle trace"}
 
 
-def adjoint_rate(seed=4):
+def adjoint_rate(seed=4, store=None):
     """Gradients per second of the trajectory loss through the fused step: 1024 cells x 500
     compartments, 2000 steps at dt 0.05 ms, checkpoints every 100 steps, targets from a forward run
     with hidden per-segment conductances (logU(0.5, 2) x the defaults).  One warm-up gradient, then
@@ -233,7 +240,8 @@ def adjoint_rate(seed=4):
     from neuwon_amd import adjoint, synth
     T, amp = 2000, 0.5
     cells = synth.uniform_population(num_cells=1024, n=500, seed=seed)
-    prob = adjoint.HHAdjoint(cells, dt=0.05, stim_amp_na=amp, checkpoint_every=100)
+    prob = adjoint.HHAdjoint(cells, dt=0.05, stim_amp_na=amp, checkpoint_every=100,
+                             forward_store=store, reverse_store=store)
     pop = prob.pop
     rng = np.random.default_rng(seed)
     gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
@@ -252,6 +260,7 @@ def adjoint_rate(seed=4):
     el = (time.perf_counter() - t0) / 2
     log(f"adjoint: {el:.3f} s per gradient")
     return {"gradients_per_s": 1.0 / el, "seconds_per_gradient": el,
+            "store": prob.forward_store,
             "config": "1024 cells x 500 compartments, 2000 steps at dt 0.05 ms, checkpoints every "
                       "100 steps; gradients of sum (v_soma - target)^2 w.r.t. per-segment gNa, gK, "
                       "gL and 6 HH rate multipliers"}
@@ -286,7 +295,7 @@ def main():
     cells = synth.population(total=int(args.compartments), seed=args.seed)
     log(f"generated {len(cells)} cells in {time.time() - t0:.1f}s")
     t0 = time.time()
-    pop = population.HHPopulation(cells, dt=0.1, celsius=6.3, prune=1e-9, gate_mode="table",
+    pop = population.HHPopulation(cells, dt=args.dt, celsius=6.3, prune=1e-9, gate_mode="table",
                                   stim={"mode": "poisson", "amp_na": 0.5, "rate_hz": 5.0,
                                         "duration_ms": 1.0, "seed": args.seed},
                                   block_threads=args.block_threads, store=args.format)
@@ -340,11 +349,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded random-tree cells, Poisson stimulus)",
-        "config": {"workload": ("batched HH population, FP32 state, exp(A dt) applied as a Chebyshev "
+        "config": {"workload": ("batched HH population, FP32 state, exp(A dt) applied as a five-pole "
+                                "rational approximation, each pole an exact solve over the cell's "
+                                "tree (no stored matrix, unpruned)" if pop.propagator.rational else
+                                "batched HH population, FP32 state, exp(A dt) applied as a Chebyshev "
                                 "series of each cell's tree operator (no stored matrix, unpruned)"
                                 if pop.propagator.chebyshev else
                                 "batched HH population, FP32 state, FP64-built propagator pruned 1e-9"),
-                   "propagator_format": args.format, "compartments": N, "cells": pop.num_cells, "dt_ms": 0.1,
+                   "propagator_format": args.format, "compartments": N, "cells": pop.num_cells, "dt_ms": args.dt,
                    "nnz_per_row": round(nnz_bar, 2), "steps_per_launch": L,
                    "propagator_build_s": round(st["build_seconds"], 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -359,7 +371,15 @@ def main():
                      "stored_propagator_bytes_per_cmpt_step": stored_bytes_step,
                      "measured_traffic_GBs": None if traffic is None else traffic * units_per_s / 1e9},
     }
-    if pop.propagator.chebyshev:
+    if pop.propagator.rational:
+        # per pole and row: leaves to root a child's q read (16 B, one child per row on average)
+        # and q written; root to leaves the row's q and its parent's z read and z written
+        lds_b = 2.0 * 5 * 5 * 16.0
+        out["roofline"].update({"lds_bytes_per_cmpt_step": lds_b,
+                                "lds_GBs": lds_b * N * K / (kern_ms / 1e3) / 1e9,
+                                "lds_peak_GBs": 256 * 128 * 2.4,
+                                "tree_levels_per_row": st["mean_levels"]})
+    elif pop.propagator.chebyshev:
         # the series runs out of LDS: per row and term two far-neighbour slots read (an unused
         # slot reads a zero) and the new term written, 8 B each; the row's own last two terms and
         # its r +- 1 neighbours' terms come from registers
@@ -374,10 +394,11 @@ def main():
         out["cpu_baseline"], v_ref, trace_ref, r_end = cpu_baseline(pop, args, W + K)
         out["precision"] = precision(pop, cells, args, v_ref, trace_ref, r_end, W + K)
         log(f"precision: {out['precision']}")
+    pop_store_free = pop.propagator.chebyshev
     if not args.no_adjoint:
         del pop
         torch.cuda.empty_cache()
-        out["adjoint"] = adjoint_rate()
+        out["adjoint"] = adjoint_rate(store=args.format if pop_store_free else None)
     print(json.dumps(out), flush=True)
 
 

@@ -49,6 +49,7 @@ typedef struct nw_propagator nw_propagator;
 
 #define NW_PROPAGATOR_SYMMETRIC 1
 #define NW_PROPAGATOR_CHEBYSHEV 2
+#define NW_PROPAGATOR_RATIONAL 4
 
 typedef struct nw_propagator_stats {
     int64_t num_cells;
@@ -66,6 +67,9 @@ typedef struct nw_propagator_stats {
                                   every cell fits one row group of 512 slots, else 1024; 0 for the
                                   stored formats                                                  */
     int32_t row_groups;      /* Chebyshev store: row groups per thread its launches are built for */
+    int32_t rational;        /* 1: the rational store (NW_PROPAGATOR_RATIONAL)                    */
+    int32_t max_levels;      /* rational store: levels of the deepest cell's tree                 */
+    double  mean_levels;     /* rational store: levels per cell, weighted by segments             */
 } nw_propagator_stats;
 
 int nw_propagator_build(int device, int64_t num_cells, const int64_t* cell_ptr_host,
@@ -92,7 +96,18 @@ int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_pt
  * the neighbouring lane, two far neighbours from LDS); other cells keep up to 3 neighbours per
  * row in natural order (more in an overflow list); cells up to 5120 segments.  Read by
  * nw_hh_step (which also writes the adjoint's record), nw_electric_half_step and nw_hh_adjoint
- * (P^T y = C P (C^-1 y), the same series); nw_ca_step refuses it. */
+ * (P^T y = C P (C^-1 y), the same series); nw_ca_step refuses it.
+ * flags = NW_PROPAGATOR_RATIONAL stores no matrix either: exp(B) x is evaluated as
+ * a0 x + 2 Re sum_j alpha_j (B - theta_j I)^-1 x, a type (10, 10) rational approximation of exp
+ * on (-inf, 0] with five poles in the upper half plane (maximum error 3e-10; r(0) = 1 exactly, so
+ * constant vectors are kept; nw_rational_coefficients).  Each shifted system is solved exactly
+ * by elimination over the cell's tree: rows in level order (the root, its children, their
+ * children, ...), a parent's children adjacent; leaves to root then root to leaves, one
+ * workgroup barrier per level; the pivots 1 / d_i of every pole are built once in FP64 from the
+ * FP32 edge coefficients (their row sums are zero) and read from memory.  Nothing is pruned (the
+ * reference's default threshold, electric.py:114-115).  Cells up to 5120 segments, at most 63
+ * children per segment, at most 4095 levels.  Read by nw_hh_step, nw_electric_half_step and
+ * nw_hh_adjoint; nw_ca_step refuses it. */
 int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* out);
 /* Chebyshev store: the spectrum bound rho and series coefficients of one cell
  * (coef_host may be NULL: only *K_out, the index of the last kept coefficient, and *rho_out). */
@@ -105,6 +120,10 @@ int nw_propagator_cell_chebyshev(const nw_propagator* p, int64_t cell, int32_t* 
  * kept exactly).  Writes at most `capacity` values; returns the number of coefficients (K + 1), or
  * a negative error code. */
 int nw_chebyshev_exp_coefficients(double rho, double tol, double* coef_host, int32_t capacity);
+/* The rational store's approximation: out[0] = a0, then per pole theta_re, theta_im, alpha_re,
+ * alpha_im (1 + 4 * 5 doubles).  No counterpart in the reference (it forms expm(A dt) with scipy,
+ * electric.py:111-112). */
+int nw_rational_coefficients(double* out, int32_t capacity);
 /* Symmetric format only: the per-segment w of one cell. */
 int nw_propagator_cell_weights(const nw_propagator* p, int64_t cell, double* w_host);
 int nw_propagator_destroy(nw_propagator* p);

@@ -16,13 +16,15 @@ c_i32, c_i64, c_u32, c_f32, c_f64, c_vp = (ctypes.c_int32, ctypes.c_int64, ctype
 
 NW_PROPAGATOR_SYMMETRIC = 1
 NW_PROPAGATOR_CHEBYSHEV = 2
+NW_PROPAGATOR_RATIONAL = 4
 
 
 class NwPropagatorStats(ctypes.Structure):
     _fields_ = [("num_cells", c_i64), ("num_rows", c_i64), ("nnz", c_i64), ("stored", c_i64),
                 ("max_cell_rows", c_i32), ("max_row_nnz", c_i32), ("build_seconds", c_f64),
                 ("device_bytes", c_i64), ("symmetric", c_i32), ("chebyshev", c_i32),
-                ("mean_terms", c_f64), ("workgroup_threads", c_i32), ("row_groups", c_i32)]
+                ("mean_terms", c_f64), ("workgroup_threads", c_i32), ("row_groups", c_i32),
+                ("rational", c_i32), ("max_levels", c_i32), ("mean_levels", c_f64)]
 
 
 class NwHHParams(ctypes.Structure):
@@ -73,6 +75,7 @@ SIGNATURES = {
     "nw_propagator_cell_chebyshev": (c_i32, [c_vp, c_i64, ctypes.POINTER(c_i32),
                                              ctypes.POINTER(c_f64), c_vp]),
     "nw_chebyshev_exp_coefficients": (c_i32, [c_f64, c_f64, c_vp, c_i32]),
+    "nw_rational_coefficients": (c_i32, [c_vp, c_i32]),
     "nw_propagator_destroy": (c_i32, [c_vp]),
     "nw_propagator_cell_f64": (c_i32, [c_i32, c_i32, c_vp, c_vp, c_vp, c_f64, c_f64,
                                        ctypes.POINTER(c_i64), c_vp, c_vp, c_vp]),

@@ -410,6 +410,18 @@ struct nw_propagator {
     int64_t pre_places = 0;        // entries of pre_far / pre_coef
     int32_t cheb_maxj = 0;         // row groups per thread the store's cells need (1..kChebRows)
     int32_t cheb_small = 0;        // every cell preorder, one row group of <= kChebSmallBlock slots
+    // Rational store (also chebyshev = 1: its launches, LDS places and pre_pos / pre_size): rows
+    // of a cell in level order (root, its children, theirs, ...; a parent's children adjacent),
+    // pre_pos[i] = level place of natural row i, pre_base[c] = 0 for every cell.
+    int rational = 0;
+    uint32_t* rat_link = nullptr;  // [num_rows, level order] parent | first child << 13 | children << 26
+    float2* rat_b = nullptr;       // [num_rows, level order] (B[parent][row], B[row][parent])
+    double2* rat_invd = nullptr;   // [kRatPoles][num_rows, level order] 1 / pivot of (B - theta I)
+    uint16_t* rat_lev = nullptr;   // level starts of every cell (levels + 1 each)
+    int64_t* rat_lev_off = nullptr;  // [num_cells] first level start of the cell
+    int32_t* rat_depth = nullptr;  // [num_cells] levels
+    int32_t rat_max_levels = 0;
+    double rat_mean_levels = 0.0;
     double mean_terms = 0.0;
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;
@@ -433,12 +445,37 @@ struct ChebStore {
     const float4* pre_coef;
     const int64_t* pre_base;
     const int32_t* pre_size;
+    int32_t rational;
+    int64_t num_rows;
+    const uint32_t* rat_link;
+    const float2* rat_b;
+    const double2* rat_invd;
+    const uint16_t* rat_lev;
+    const int64_t* rat_lev_off;
+    const int32_t* rat_depth;
 };
 
 inline ChebStore cheb_store(const nw_propagator* p) {
-    return ChebStore{p->nb_idx, p->nb_coef, p->ov_idx, p->ov_coef, p->cheb_K, p->cheb_off,
-                     p->cheb_c, p->pre_pos, p->pre_far, p->pre_coef, p->pre_base, p->pre_size};
+    return ChebStore{p->nb_idx,   p->nb_coef,  p->ov_idx,    p->ov_coef,     p->cheb_K,
+                     p->cheb_off, p->cheb_c,   p->pre_pos,   p->pre_far,     p->pre_coef,
+                     p->pre_base, p->pre_size, p->rational,  p->num_rows,    p->rat_link,
+                     p->rat_b,    p->rat_invd, p->rat_lev,   p->rat_lev_off, p->rat_depth};
 }
+
+// The rational store's approximation exp(x) ~ a0 + 2 Re sum_j alpha_j / (x - theta_j) on
+// (-inf, 0]: type (10, 10), poles from the AAA algorithm on exp(9 (t - 1) / (t + 1)), t in
+// [-1, 1), residues by least squares, a0 moved so that r(0) = 1; maximum error 2.9e-10.
+// Layout: a0, then theta_re, theta_im, alpha_re, alpha_im per pole.
+constexpr int kRatPoles = 5;
+#define NW_RAT_COEFFICIENTS                                                                    \
+    2.802151606726937e-11,                                                                     \
+    -5.056635951881228, 11.421609290485412, -0.006437906767520651, 0.0005054310548602677,      \
+    -0.8025731342599113, 8.606156657901737, 0.2987978122522727, 0.02160797576074506,           \
+    1.803811562314832, 6.052904140355052, -2.7893832883191694, -1.363533660934761,             \
+    3.369248005747136, 3.602670710938092, 7.700048093610149, 9.67073157332556,                 \
+    4.11129462003477, 1.1966884849799233, -5.203024560624244, -22.951290497759175
+constexpr double kRatHost[1 + 4 * kRatPoles] = {NW_RAT_COEFFICIENTS};
+static __constant__ double kRatDev[1 + 4 * kRatPoles] = {NW_RAT_COEFFICIENTS};
 
 // Rows per thread of the Chebyshev products (cells up to kChebBlock * kChebRows segments).
 constexpr int kChebBlock = 1024;
@@ -649,10 +686,130 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
 // at most NT slots (nw_propagator::cheb_small): the LDS places stay those of the 1024-thread
 // layout, threads NT..1023 would own only empty slots and do not exist.  Returns the number of
 // LDS places in use: out[j] is defined where threadIdx.x + j * 1024 is below it.
-template <int MAXJ, int NT = kChebBlock>
+// The rational store's product for one cell (the contract of cheb_cell): x at the rows' level
+// places in LDS (FP64, after a barrier); thread t owns the places t + j * 1024.  Per pole, the
+// system (B - theta I) z = x by elimination over the tree: leaves to root
+//   q_i = (x_i - sum_children B[i][c] q_c) / d_i,
+// then root to leaves z_i = q_i - (B[i][parent] / d_i) z_parent, one level per barrier, q and z
+// as FP64 complex in LDS (16 B per row); out = a0 x + 2 Re sum_j alpha_j z_j.  The level starts
+// sit in LDS after the two term vectors.  Ends with a barrier; returns the places in use (n).
+template <int MAXJ, int NT>
+__device__ __forceinline__ int rat_cell(const ChebStore& cs, int64_t row0, int cell, int n, double* lds,
+                                        double (&out)[kChebRows]) {
+    double2* q = reinterpret_cast<double2*>(lds);
+    uint16_t* lev = reinterpret_cast<uint16_t*>(lds + 2 * MAXJ * kChebBlock);
+    const int D = cs.rat_depth[cell];
+    const uint16_t* glev = cs.rat_lev + cs.rat_lev_off[cell];
+    for (int l = threadIdx.x; l <= D; l += NT) lev[l] = glev[l];
+    const uint32_t* link = cs.rat_link + row0;
+    const float2* bb = cs.rat_b + row0;
+    double xt[MAXJ];
+    uint32_t lk[MAXJ];
+    int lvl[MAXJ];
+    float bdn[MAXJ], bc0[MAXJ], bc1[MAXJ];
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) {
+        const int k = threadIdx.x + j * kChebBlock;
+        xt[j] = 0.0;
+        lk[j] = 0u;
+        bdn[j] = bc0[j] = bc1[j] = 0.0f;
+        if (k < n) {
+            xt[j] = lds[k];
+            lk[j] = link[k];
+            bdn[j] = bb[k].y;
+            const int fc = (int)((lk[j] >> 13) & 0x1FFFu), cnt = (int)(lk[j] >> 26);
+            if (cnt > 0) bc0[j] = bb[fc].x;
+            if (cnt > 1) bc1[j] = bb[fc + 1].x;
+        }
+        out[j] = kRatDev[0] * xt[j];
+    }
+    __syncthreads();
+    // the level of every owned row (-1: no row), by bisection of the level starts
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) {
+        const int k = threadIdx.x + j * kChebBlock;
+        int a = 0, b = D;  // lev[a] <= k < lev[b]
+        while (b - a > 1) {
+            const int m = (a + b) >> 1;
+            if ((int)lev[m] <= k) a = m;
+            else b = m;
+        }
+        lvl[j] = k < n ? a : -1;
+    }
+    const double2* gid = cs.rat_invd + row0;
+    for (int pj = 0; pj < kRatPoles; ++pj) {
+        const double al_re2 = 2.0 * kRatDev[3 + 4 * pj], al_im2 = 2.0 * kRatDev[4 + 4 * pj];
+        double2 id[MAXJ];
+#pragma unroll
+        for (int j = 0; j < MAXJ; ++j) {
+            const int k = threadIdx.x + j * kChebBlock;
+            id[j] = k < n ? gid[(int64_t)pj * cs.num_rows + k] : make_double2(0.0, 0.0);
+        }
+        // leaves to root
+        for (int l = D - 1; l >= 0; --l) {
+#pragma unroll
+            for (int j = 0; j < MAXJ; ++j) {
+                const int k = threadIdx.x + j * kChebBlock;
+                if (lvl[j] == l) {
+                    double re = xt[j], im = 0.0;
+                    const int fc = (int)((lk[j] >> 13) & 0x1FFFu), cnt = (int)(lk[j] >> 26);
+                    if (cnt > 0) {
+                        const double2 c0 = q[fc];
+                        re = fma(-(double)bc0[j], c0.x, re);
+                        im = fma(-(double)bc0[j], c0.y, im);
+                    }
+                    if (cnt > 1) {
+                        const double2 c1 = q[fc + 1];
+                        re = fma(-(double)bc1[j], c1.x, re);
+                        im = fma(-(double)bc1[j], c1.y, im);
+                    }
+                    for (int c = 2; c < cnt; ++c) {  // more than two children (a soma)
+                        const double b = bb[fc + c].x;
+                        const double2 cc = q[fc + c];
+                        re = fma(-b, cc.x, re);
+                        im = fma(-b, cc.y, im);
+                    }
+                    q[k] = make_double2(fma(re, id[j].x, -im * id[j].y), fma(re, id[j].y, im * id[j].x));
+                }
+            }
+            __syncthreads();
+        }
+        // root to leaves
+        for (int l = 0; l < D; ++l) {
+#pragma unroll
+            for (int j = 0; j < MAXJ; ++j) {
+                const int k = threadIdx.x + j * kChebBlock;
+                if (lvl[j] == l) {
+                    double2 z = q[k];
+                    if (l > 0) {
+                        const double2 zp = q[lk[j] & 0x1FFFu];
+                        const double er = (double)bdn[j] * id[j].x, ei = (double)bdn[j] * id[j].y;
+                        z.x -= fma(er, zp.x, -ei * zp.y);
+                        z.y -= fma(er, zp.y, ei * zp.x);
+                    }
+                    q[k] = z;
+                    out[j] = fma(al_re2, z.x, out[j]);
+                    out[j] = fma(-al_im2, z.y, out[j]);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    return n;
+}
+
+// KIND selects the products a kernel is built with: kProdAny (the store's flag decides at run
+// time), kProdRational or kProdChebyshev (launches for one store only: no registers or code of
+// the other product).
+constexpr int kProdAny = 0, kProdRational = 1, kProdChebyshev = 2;
+template <int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int cell, int n, int64_t base,
                                          double* lds, double (&out)[kChebRows]) {
     static_assert(NT == kChebBlock || MAXJ == 1, "small workgroups serve one row group");
+    if constexpr (KIND == kProdRational) {
+        return rat_cell<MAXJ, NT>(cs, row0, cell, n, lds, out);
+    } else {
+    if (KIND == kProdAny && cs.rational) return rat_cell<MAXJ, NT>(cs, row0, cell, n, lds, out);
     constexpr int BUF = MAXJ * kChebBlock;
     const double* c = cs.c + cs.off[cell];
     const int K = cs.K[cell];
@@ -677,4 +834,5 @@ __device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int 
         else cheb_pre_rows<5, float, BUF, NT>(far, cf, c, K, size, lds, out);
     }
     return J * kChebBlock;
+    }
 }

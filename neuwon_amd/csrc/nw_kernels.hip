@@ -14,6 +14,8 @@
 // HBM (they are a few percent of the propagator's bytes).
 #include <algorithm>
 
+#include <type_traits>
+
 #include "nw_internal.h"
 
 namespace nw {
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 // of 1024-thread blocks.  Voltages pass between launches as FP64 (v64); the FP32 voltage array
 // is written after every step.
 // ----------------------------------------------------------------------------------------------
-template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock>
+template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     __shared__ double b0[cheb_lds(MAXJ)];
@@ -405,7 +407,7 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell<MAXJ, NT>(a.cheb, row0, cell, nrows, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ, NT, KIND>(a.cheb, row0, cell, nrows, pre_base, b0, out);
     // results to LDS (the row's place), then out in natural order
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
@@ -1157,33 +1159,30 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
                    p->max_cell_rows);
         const unsigned grid = (unsigned)p->num_cells;
         double* v64 = p->cheb_v64;
-        const bool one = p->cheb_maxj == 1;
-        constexpr int SB = kChebSmallBlock;
-        for (int st = 0; st < n_steps; ++st) {
-            if (p->cheb_small) {
-                if (prm->gate_mode == 1) {
-                    k_hh_half_cheb<1, 0, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, st == 0);
-                    k_hh_half_cheb<1, 1, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, 0);
-                } else {
-                    k_hh_half_cheb<0, 0, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, st == 0);
-                    k_hh_half_cheb<0, 1, 1, SB><<<grid, SB, 0, s>>>(a, v64, st, 0);
-                }
-            } else if (one) {
-                if (prm->gate_mode == 1) {
-                    k_hh_half_cheb<1, 0, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
-                    k_hh_half_cheb<1, 1, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
-                } else {
-                    k_hh_half_cheb<0, 0, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
-                    k_hh_half_cheb<0, 1, 1><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
-                }
-            } else if (prm->gate_mode == 1) {
-                k_hh_half_cheb<1, 0, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
-                k_hh_half_cheb<1, 1, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
-            } else {
-                k_hh_half_cheb<0, 0, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, st == 0);
-                k_hh_half_cheb<0, 1, kChebRows><<<grid, kChebBlock, 0, s>>>(a, v64, st, 0);
+        // one launch per half step, built for the store's product and the largest cell
+        auto launch = [&](auto gate, auto maxj, auto nt, auto kind) {
+            constexpr int G = decltype(gate)::value, MJ = decltype(maxj)::value;
+            constexpr int T = decltype(nt)::value, KD = decltype(kind)::value;
+            for (int st = 0; st < n_steps; ++st) {
+                k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, s>>>(a, v64, st, st == 0);
+                k_hh_half_cheb<G, 1, MJ, T, KD><<<grid, T, 0, s>>>(a, v64, st, 0);
             }
-        }
+        };
+        auto by_shape = [&](auto gate, auto kind) {
+            using I = std::integral_constant<int, 1>;
+            using I5 = std::integral_constant<int, kChebRows>;
+            using TS = std::integral_constant<int, kChebSmallBlock>;
+            using TB = std::integral_constant<int, kChebBlock>;
+            if (p->cheb_small) launch(gate, I{}, TS{}, kind);
+            else if (p->cheb_maxj == 1) launch(gate, I{}, TB{}, kind);
+            else launch(gate, I5{}, TB{}, kind);
+        };
+        auto by_kind = [&](auto gate) {
+            if (p->rational) by_shape(gate, std::integral_constant<int, kProdRational>{});
+            else by_shape(gate, std::integral_constant<int, kProdChebyshev>{});
+        };
+        if (prm->gate_mode == 1) by_kind(std::integral_constant<int, 1>{});
+        else by_kind(std::integral_constant<int, 0>{});
         NW_HIP(hipGetLastError());
         return NW_OK;
     }

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <numeric>
 
@@ -745,6 +746,172 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     return NW_OK;
 }
 
+// Rational store: per cell the rows in level order, the two FP32 edge coefficients of every
+// row, and per pole the FP64 pivots of (B - theta I) eliminated from the leaves up.
+static int build_rational(nw_propagator* p, const int64_t* cell_ptr, const int32_t* parent,
+                          const double* R, const double* C, double dt_ms) {
+    const int64_t N = p->num_rows;
+    const double dt_s = dt_ms * 1e-3;
+    std::vector<uint16_t> ppos(std::max<int64_t>(N, 1), 0);
+    std::vector<uint32_t> link(std::max<int64_t>(N, 1), 0u);
+    std::vector<float2> bcoef(std::max<int64_t>(N, 1), float2{0.0f, 0.0f});
+    std::vector<double2> invd((size_t)kRatPoles * std::max<int64_t>(N, 1), double2{0.0, 0.0});
+    std::vector<uint16_t> levs;
+    std::vector<int64_t> lev_off(p->num_cells, 0), pbase(p->num_cells, 0);
+    std::vector<int32_t> depth(p->num_cells, 0), psize(p->num_cells, 0);
+    std::vector<int32_t> order_lv, first, count, nat_of;
+    std::vector<std::vector<int32_t>> kids;
+    double lev_rows = 0.0;
+    int max_levels = 0, max_deg = 0;
+    int64_t edges = 0;
+    for (int64_t cell = 0; cell < p->num_cells; ++cell) {
+        const int64_t r0 = cell_ptr[cell];
+        const int n = (int)(cell_ptr[cell + 1] - r0);
+        kids.assign(n, {});
+        for (int c = 1; c < n; ++c) {
+            const int q = parent[r0 + c];
+            NW_REQUIRE(q >= 0 && q < c, "cell %lld: segment %d needs an earlier parent (one root, "
+                       "segment 0)", (long long)cell, c);
+            kids[q].push_back(c);
+        }
+        NW_REQUIRE(parent[r0] < 0, "cell %lld: segment 0 must be the root", (long long)cell);
+        // level order: breadth first, a parent's children adjacent in segment order
+        nat_of.assign(1, 0);
+        std::vector<int32_t> place(n, 0);
+        lev_off[cell] = (int64_t)levs.size();
+        levs.push_back(0);
+        int lo = 0, hi = 1;
+        while (lo < hi) {
+            for (int k = lo; k < hi; ++k)
+                for (int c : kids[nat_of[k]]) {
+                    place[c] = (int)nat_of.size();
+                    nat_of.push_back(c);
+                }
+            levs.push_back((uint16_t)hi);
+            lo = hi;
+            hi = (int)nat_of.size();
+        }
+        NW_REQUIRE((int)nat_of.size() == n, "cell %lld: the parents do not form one tree",
+                   (long long)cell);
+        const int D = (int)(levs.size() - lev_off[cell]) - 1;
+        NW_REQUIRE(D <= 4095, "cell %lld: %d levels (the rational store holds 4095)",
+                   (long long)cell, D);
+        depth[cell] = D;
+        max_levels = std::max(max_levels, D);
+        lev_rows += (double)D * n;
+        std::vector<double> bup(n, 0.0), bdn(n, 0.0), diag(n, 0.0);
+        for (int k = 0; k < n; ++k) {
+            const int i = nat_of[k];
+            ppos[r0 + i] = (uint16_t)k;
+            const int cnt = (int)kids[i].size();
+            NW_REQUIRE(cnt <= 63, "cell %lld: segment %d has %d children (the rational store holds "
+                       "63)", (long long)cell, i, cnt);
+            max_deg = std::max(max_deg, cnt + (i > 0 ? 1 : 0));
+            const int fc = cnt ? place[kids[i][0]] : 0;
+            const int par = i > 0 ? place[parent[r0 + i]] : 0;
+            link[r0 + k] = (uint32_t)par | ((uint32_t)fc << 13) | ((uint32_t)cnt << 26);
+            if (i > 0) {
+                const int q = parent[r0 + i];
+                // B[parent][row] and B[row][parent] (electric.py:92-108, times the half step)
+                const float up = (float)(dt_s / (R[r0 + i] * C[r0 + q]));
+                const float dn = (float)(dt_s / (R[r0 + i] * C[r0 + i]));
+                bcoef[r0 + k] = float2{up, dn};
+                bup[k] = up;
+                bdn[k] = dn;
+                ++edges;
+            }
+        }
+        // zero row sums from the FP32 coefficients: B[i][i] = -(B[i][parent] + sum B[i][child])
+        for (int k = 0; k < n; ++k) {
+            const uint32_t L = link[r0 + k];
+            const int fc = (int)((L >> 13) & 0x1FFFu), cnt = (int)(L >> 26);
+            double d = -bdn[k];
+            for (int c = fc; c < fc + cnt; ++c) d -= bup[c];
+            diag[k] = d;
+        }
+        for (int pj = 0; pj < kRatPoles; ++pj) {
+            const std::complex<double> th(kRatHost[1 + 4 * pj], kRatHost[2 + 4 * pj]);
+            std::vector<std::complex<double>> iv(n);
+            for (int k = n - 1; k >= 0; --k) {
+                const uint32_t L = link[r0 + k];
+                const int fc = (int)((L >> 13) & 0x1FFFu), cnt = (int)(L >> 26);
+                std::complex<double> d = diag[k] - th;
+                for (int c = fc; c < fc + cnt; ++c) d -= bup[c] * bdn[c] * iv[c];
+                iv[k] = 1.0 / d;
+                invd[(size_t)pj * N + r0 + k] = double2{iv[k].real(), iv[k].imag()};
+            }
+        }
+        const int J = (n + kChebBlock - 1) / kChebBlock;
+        psize[cell] = n | (J << 16);
+    }
+    p->nnz = 2 * edges;
+    p->stored = 0;
+    p->max_row_nnz = max_deg;
+    p->mean_terms = 0.0;
+    p->rat_max_levels = max_levels;
+    p->rat_mean_levels = N ? lev_rows / (double)N : 0.0;
+    // the Chebyshev arrays the kernels address but never read for this store
+    p->h_cheb_K.assign(p->num_cells, 0);
+    p->h_cheb_off.assign(p->num_cells, 0);
+    p->h_cheb_c.assign(1, 1.0);
+    p->h_rho.assign(p->num_cells, 0.0);
+    NW_HIP(hipMalloc(&p->nb_idx, sizeof(int32_t) * 3));
+    NW_HIP(hipMalloc(&p->nb_coef, sizeof(float) * 3));
+    NW_HIP(hipMalloc(&p->ov_idx, sizeof(int32_t)));
+    NW_HIP(hipMalloc(&p->ov_coef, sizeof(float)));
+    NW_HIP(hipMalloc(&p->cheb_K, sizeof(int32_t) * p->num_cells));
+    NW_HIP(hipMalloc(&p->cheb_off, sizeof(int64_t) * p->num_cells));
+    NW_HIP(hipMalloc(&p->cheb_c, sizeof(double)));
+    NW_HIP(hipMalloc(&p->pre_far, sizeof(uint32_t)));
+    NW_HIP(hipMalloc(&p->pre_coef, sizeof(float4)));
+    NW_HIP(hipMemcpy(p->cheb_K, p->h_cheb_K.data(), sizeof(int32_t) * p->num_cells,
+                     hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->cheb_off, p->h_cheb_off.data(), sizeof(int64_t) * p->num_cells,
+                     hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->cheb_c, p->h_cheb_c.data(), sizeof(double), hipMemcpyHostToDevice));
+    NW_HIP(hipMalloc(&p->cheb_v64, sizeof(double) * std::max<int64_t>(N, 1)));
+    NW_HIP(hipMalloc(&p->pre_pos, sizeof(uint16_t) * ppos.size()));
+    NW_HIP(hipMalloc(&p->pre_base, sizeof(int64_t) * p->num_cells));
+    NW_HIP(hipMalloc(&p->pre_size, sizeof(int32_t) * p->num_cells));
+    NW_HIP(hipMalloc(&p->rat_link, sizeof(uint32_t) * link.size()));
+    NW_HIP(hipMalloc(&p->rat_b, sizeof(float2) * bcoef.size()));
+    NW_HIP(hipMalloc(&p->rat_invd, sizeof(double2) * invd.size()));
+    NW_HIP(hipMalloc(&p->rat_lev, sizeof(uint16_t) * levs.size()));
+    NW_HIP(hipMalloc(&p->rat_lev_off, sizeof(int64_t) * p->num_cells));
+    NW_HIP(hipMalloc(&p->rat_depth, sizeof(int32_t) * p->num_cells));
+    NW_HIP(hipMemcpy(p->pre_pos, ppos.data(), sizeof(uint16_t) * ppos.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_base, pbase.data(), sizeof(int64_t) * p->num_cells, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->pre_size, psize.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->rat_link, link.data(), sizeof(uint32_t) * link.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->rat_b, bcoef.data(), sizeof(float2) * bcoef.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->rat_invd, invd.data(), sizeof(double2) * invd.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->rat_lev, levs.data(), sizeof(uint16_t) * levs.size(), hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->rat_lev_off, lev_off.data(), sizeof(int64_t) * p->num_cells,
+                     hipMemcpyHostToDevice));
+    NW_HIP(hipMemcpy(p->rat_depth, depth.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
+    p->pre_cells = p->num_cells;
+    p->pre_places = N;
+    p->cheb_maxj = 1;
+    p->cheb_small = 1;
+    for (int64_t c = 0; c < p->num_cells; ++c) {
+        const int n = psize[c] & 0xFFFF, J = psize[c] >> 16;
+        p->cheb_maxj = std::max(p->cheb_maxj, J);
+        if (n > kChebSmallBlock) p->cheb_small = 0;
+    }
+    // launch order: most work first (levels x rows per thread, then rows)
+    std::vector<int32_t> order(p->num_cells);
+    std::iota(order.begin(), order.end(), 0);
+    auto work_of = [&](int32_t c) {
+        const int64_t n = cell_ptr[c + 1] - cell_ptr[c];
+        return std::make_pair((int64_t)depth[c] * ((n + kChebBlock - 1) / kChebBlock) + n / 64, n);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return work_of(a) > work_of(b); });
+    NW_HIP(hipMalloc(&p->order, sizeof(int32_t) * p->num_cells));
+    NW_HIP(hipMemcpy(p->order, order.data(), sizeof(int32_t) * p->num_cells, hipMemcpyHostToDevice));
+    return NW_OK;
+}
+
 static void free_prop(nw_propagator* p) {
     if (!p) return;
     if (p->nb_idx) (void)hipFree(p->nb_idx);
@@ -760,6 +927,12 @@ static void free_prop(nw_propagator* p) {
     if (p->pre_coef) (void)hipFree(p->pre_coef);
     if (p->pre_base) (void)hipFree(p->pre_base);
     if (p->pre_size) (void)hipFree(p->pre_size);
+    if (p->rat_link) (void)hipFree(p->rat_link);
+    if (p->rat_b) (void)hipFree(p->rat_b);
+    if (p->rat_invd) (void)hipFree(p->rat_invd);
+    if (p->rat_lev) (void)hipFree(p->rat_lev);
+    if (p->rat_lev_off) (void)hipFree(p->rat_lev_off);
+    if (p->rat_depth) (void)hipFree(p->rat_depth);
     if (p->cell_row0) (void)hipFree(p->cell_row0);
     if (p->cell_nz0) (void)hipFree(p->cell_nz0);
     if (p->row_off) (void)hipFree(p->row_off);
@@ -775,10 +948,13 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
                  const double* R, const double* C, double dt_ms, double prune, int flags,
                  nw_propagator* p) {
     const bool sym = (flags & NW_PROPAGATOR_SYMMETRIC) != 0;
-    const bool cheb = (flags & NW_PROPAGATOR_CHEBYSHEV) != 0;
-    NW_REQUIRE(!(sym && cheb), "the symmetric and Chebyshev stores are exclusive");
+    const bool rat = (flags & NW_PROPAGATOR_RATIONAL) != 0;
+    const bool cheb = (flags & NW_PROPAGATOR_CHEBYSHEV) != 0 || rat;
+    NW_REQUIRE((int)sym + (int)((flags & NW_PROPAGATOR_CHEBYSHEV) != 0) + (int)rat <= 1,
+               "the symmetric, Chebyshev and rational stores are exclusive");
     p->symmetric = sym ? 1 : 0;
     p->chebyshev = cheb ? 1 : 0;
+    p->rational = rat ? 1 : 0;
     auto t0 = std::chrono::steady_clock::now();
     NW_REQUIRE(num_cells > 0, "num_cells must be positive");
     NW_REQUIRE(cell_ptr[0] == 0, "cell_ptr[0] must be 0");
@@ -789,7 +965,7 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
         int64_t n = cell_ptr[c + 1] - cell_ptr[c];
         NW_REQUIRE(n >= 1 && n <= limit, "cell %lld has %lld segments (1..%lld supported%s)",
                    (long long)c, (long long)n, (long long)limit,
-                   cheb ? " by the Chebyshev store" : "");
+                   rat ? " by the rational store" : cheb ? " by the Chebyshev store" : "");
         nmax = std::max<int>(nmax, (int)n);
     }
     NW_HIP(hipSetDevice(device));
@@ -803,7 +979,8 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
         NW_HIP(hipMalloc(&p->cell_row0, sizeof(int64_t) * (num_cells + 1)));
         NW_HIP(hipMemcpy(p->cell_row0, cell_ptr, sizeof(int64_t) * (num_cells + 1),
                          hipMemcpyHostToDevice));
-        int rc = build_chebyshev(p, cell_ptr, parent, R, C, dt_ms);
+        int rc = rat ? build_rational(p, cell_ptr, parent, R, C, dt_ms)
+                     : build_chebyshev(p, cell_ptr, parent, R, C, dt_ms);
         p->build_seconds =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return rc;
@@ -1064,7 +1241,7 @@ int nw_propagator_build_ex(int device, int64_t num_cells, const int64_t* cell_pt
                            nw_propagator** out) {
     NW_REQUIRE(out && cell_ptr_host && parent_local_host && R_host && C_host,
                "nw_propagator_build: null argument");
-    NW_REQUIRE((flags & ~(NW_PROPAGATOR_SYMMETRIC | NW_PROPAGATOR_CHEBYSHEV)) == 0,
+    NW_REQUIRE((flags & ~(NW_PROPAGATOR_SYMMETRIC | NW_PROPAGATOR_CHEBYSHEV | NW_PROPAGATOR_RATIONAL)) == 0,
                "unknown propagator flags %d", flags);
     *out = nullptr;
     auto* p = new nw_propagator();
@@ -1092,6 +1269,9 @@ int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
     s->mean_terms = p->mean_terms;
     s->workgroup_threads = p->chebyshev ? (p->cheb_small ? kChebSmallBlock : kChebBlock) : 0;
     s->row_groups = p->chebyshev ? (p->cheb_maxj == 1 ? 1 : kChebRows) : 0;
+    s->rational = p->rational;
+    s->max_levels = p->rat_max_levels;
+    s->mean_levels = p->rat_mean_levels;
     s->device_bytes = p->vals_capacity * (int64_t)(sizeof(float) + sizeof(uint16_t)) +
                       (p->num_rows + p->num_cells) * (int64_t)sizeof(uint32_t) +
                       p->num_rows * (int64_t)sizeof(double) * (p->symmetric ? 2 : 1) +
@@ -1102,13 +1282,18 @@ int nw_propagator_stats_get(const nw_propagator* p, nw_propagator_stats* s) {
                           p->pre_places * (int64_t)(sizeof(uint32_t) + sizeof(float4)) +
                           (int64_t)p->h_cheb_c.size() * (int64_t)sizeof(double) +
                           p->num_cells * (int64_t)(3 * sizeof(int64_t) + 3 * sizeof(int32_t));
+    if (p->rational)  // level places, links, edge pairs, pivots, FP64 voltages, level starts
+        s->device_bytes = p->num_rows * (int64_t)(sizeof(uint16_t) + sizeof(uint32_t) + sizeof(float2) +
+                                                  kRatPoles * sizeof(double2) + sizeof(double)) +
+                          (p->num_rows + 2 * p->num_cells) * (int64_t)sizeof(uint16_t) +
+                          p->num_cells * (int64_t)(4 * sizeof(int64_t) + 4 * sizeof(int32_t));
     return NW_OK;
 }
 
 int nw_propagator_cell_chebyshev(const nw_propagator* p, int64_t cell, int32_t* K_out,
                                  double* rho_out, double* coef_host) {
     NW_REQUIRE(p && K_out && rho_out, "nw_propagator_cell_chebyshev: null argument");
-    NW_REQUIRE(p->chebyshev, "the propagator is not in the Chebyshev store");
+    NW_REQUIRE(p->chebyshev && !p->rational, "the propagator is not in the Chebyshev store");
     NW_REQUIRE(cell >= 0 && cell < p->num_cells, "cell index out of range");
     *K_out = p->h_cheb_K[cell];
     *rho_out = p->h_rho[cell];
@@ -1125,6 +1310,13 @@ int nw_chebyshev_exp_coefficients(double rho, double tol, double* coef_host, int
     if (coef_host)
         std::memcpy(coef_host, c.data(), sizeof(double) * std::min<size_t>(c.size(), capacity));
     return (int)c.size();
+}
+
+int nw_rational_coefficients(double* out, int32_t capacity) {
+    NW_REQUIRE(out && capacity >= 1 + 4 * kRatPoles, "nw_rational_coefficients: room for %d values",
+               1 + 4 * kRatPoles);
+    std::memcpy(out, kRatHost, sizeof(kRatHost));
+    return 1 + 4 * kRatPoles;
 }
 
 int nw_propagator_destroy(nw_propagator* p) {

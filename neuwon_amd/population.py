@@ -21,7 +21,7 @@ def _torch():
     return torch
 
 
-STORES = ("general", "symmetric", "chebyshev")
+STORES = ("general", "symmetric", "chebyshev", "rational")
 
 
 def store_name(store=None, symmetric=False):
@@ -44,7 +44,10 @@ class Propagator:
     stores S = W P W^-1 (w = sqrt(C)) on and above the diagonal only; a pair (i, j) is then kept
     when max(P_ij, P_ji) >= prune.  store="chebyshev" stores no matrix: products are Chebyshev
     series of the cell's tree operator, unpruned (prune is ignored), for cells up to 5120
-    segments.  The fused HH step and the generic half step read every store, the adjoint the
+    segments.  store="rational" stores no matrix either: exp(B) x as a five-pole rational
+    approximation, each pole an exact solve over the cell's tree in level order (unpruned; cells
+    up to 5120 segments, at most 63 children per segment); it runs in the Chebyshev store's
+    launches, so ``chebyshev`` is true for both matrix-free stores.  The fused HH step and the generic half step read every store, the adjoint the
     first two, the species step the general one.
     """
 
@@ -58,9 +61,11 @@ class Propagator:
         self.half_dt_ms, self.prune, self.device = float(half_dt_ms), float(prune), int(device)
         self.store = store_name(store, symmetric)
         self.symmetric = self.store == "symmetric"
-        self.chebyshev = self.store == "chebyshev"
+        self.chebyshev = self.store in ("chebyshev", "rational")
+        self.rational = self.store == "rational"
         flags = {"general": 0, "symmetric": _lib.NW_PROPAGATOR_SYMMETRIC,
-                 "chebyshev": _lib.NW_PROPAGATOR_CHEBYSHEV}[self.store]
+                 "chebyshev": _lib.NW_PROPAGATOR_CHEBYSHEV,
+                 "rational": _lib.NW_PROPAGATOR_RATIONAL}[self.store]
         h = ctypes.c_void_p()
         with _lib.lock():
             _lib.check(lib.nw_propagator_build_ex(self.device, len(self.cell_ptr) - 1,

@@ -51,7 +51,8 @@ def _rel(a, b):
 @pytest.mark.parametrize("forward_store,reverse_store",
                          [("general", "general"), ("symmetric", "symmetric"),
                           ("chebyshev", "symmetric"), ("chebyshev", "general"),
-                          ("chebyshev", "chebyshev")])
+                          ("chebyshev", "chebyshev"), ("rational", "rational"),
+                          ("rational", "chebyshev")])
 def test_adjoint_matches_oracle(torch_gpu, forward_store, reverse_store):
     """Forward passes (loss and recomputation with the record) and reverse pass on each store
     pairing: per-segment and rate gradients and the loss against the FP64 oracle adjoint fed the
