@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per compartment-step of the step kernel on this workload, per propagator format, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs scaled by what the same counters read for a
 # plain 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); None where not measured
-MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3, "chebyshev": 151.0,
+MEASURED_TRAFFIC_PER_CMPT_STEP = {"general": 6896.1, "symmetric": 3753.3, "chebyshev": 154.4,
                                   "rational": None}
 # Kept entries per row of the propagator pruned at 1e-9 (pair rule), this seeded 10M workload:
 # the canonical algorithmic count of the Chebyshev store, which keeps no matrix, uses it

@@ -561,6 +561,8 @@ __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const
 constexpr int kChebBuf = kChebBlock * kChebRows;
 constexpr int kChebCoefCap = 1024;
 constexpr int cheb_lds(int maxj) { return 2 * maxj * kChebBlock + kChebCoefCap; }
+// the rational product: x, then q / z as complex, then up to 4095 level starts (doubles)
+constexpr int rat_lds(int maxj) { return 3 * maxj * kChebBlock + 1024; }
 constexpr uint32_t kChebHole = 0xFFFFFFFFu;
 
 // v of lane l - 1 (wave_shr:1; lane 0 receives 0) and of lane l + 1 (wave_shl:1; lane 63: 0).
@@ -696,32 +698,31 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
 template <int MAXJ, int NT>
 __device__ __forceinline__ int rat_cell(const ChebStore& cs, int64_t row0, int cell, int n, double* lds,
                                         double (&out)[kChebRows]) {
-    double2* q = reinterpret_cast<double2*>(lds);
-    uint16_t* lev = reinterpret_cast<uint16_t*>(lds + 2 * MAXJ * kChebBlock);
+    // LDS: x (FP64, the caller's places) | q and z (FP64 complex) | level starts
+    double2* q = reinterpret_cast<double2*>(lds + MAXJ * kChebBlock);
+    uint16_t* lev = reinterpret_cast<uint16_t*>(lds + 3 * MAXJ * kChebBlock);
     const int D = cs.rat_depth[cell];
     const uint16_t* glev = cs.rat_lev + cs.rat_lev_off[cell];
     for (int l = threadIdx.x; l <= D; l += NT) lev[l] = glev[l];
     const uint32_t* link = cs.rat_link + row0;
     const float2* bb = cs.rat_b + row0;
-    double xt[MAXJ];
     uint32_t lk[MAXJ];
     int lvl[MAXJ];
     float bdn[MAXJ], bc0[MAXJ], bc1[MAXJ];
 #pragma unroll
     for (int j = 0; j < MAXJ; ++j) {
         const int k = threadIdx.x + j * kChebBlock;
-        xt[j] = 0.0;
         lk[j] = 0u;
         bdn[j] = bc0[j] = bc1[j] = 0.0f;
+        out[j] = 0.0;
         if (k < n) {
-            xt[j] = lds[k];
+            out[j] = kRatDev[0] * lds[k];
             lk[j] = link[k];
             bdn[j] = bb[k].y;
             const int fc = (int)((lk[j] >> 13) & 0x1FFFu), cnt = (int)(lk[j] >> 26);
             if (cnt > 0) bc0[j] = bb[fc].x;
             if (cnt > 1) bc1[j] = bb[fc + 1].x;
         }
-        out[j] = kRatDev[0] * xt[j];
     }
     __syncthreads();
     // the level of every owned row (-1: no row), by bisection of the level starts
@@ -751,7 +752,7 @@ __device__ __forceinline__ int rat_cell(const ChebStore& cs, int64_t row0, int c
             for (int j = 0; j < MAXJ; ++j) {
                 const int k = threadIdx.x + j * kChebBlock;
                 if (lvl[j] == l) {
-                    double re = xt[j], im = 0.0;
+                    double re = lds[k], im = 0.0;
                     const int fc = (int)((lk[j] >> 13) & 0x1FFFu), cnt = (int)(lk[j] >> 26);
                     if (cnt > 0) {
                         const double2 c0 = q[fc];
@@ -795,13 +796,19 @@ __device__ __forceinline__ int rat_cell(const ChebStore& cs, int64_t row0, int c
             __syncthreads();
         }
     }
-    return n;
+    return n < MAXJ * kChebBlock ? n : MAXJ * kChebBlock;
 }
 
 // KIND selects the products a kernel is built with: kProdAny (the store's flag decides at run
 // time), kProdRational or kProdChebyshev (launches for one store only: no registers or code of
 // the other product).
 constexpr int kProdAny = 0, kProdRational = 1, kProdChebyshev = 2;
+// LDS (doubles) of a launch built for KIND and MAXJ
+constexpr int prod_lds(int maxj, int kind) {
+    return kind == kProdChebyshev ? cheb_lds(maxj)
+         : kind == kProdRational ? rat_lds(maxj)
+         : (cheb_lds(maxj) > rat_lds(maxj) ? cheb_lds(maxj) : rat_lds(maxj));
+}
 template <int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int cell, int n, int64_t base,
                                          double* lds, double (&out)[kChebRows]) {
@@ -826,6 +833,7 @@ __device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int 
     const int J = cs.pre_size[cell] >> 16;
     if constexpr (MAXJ == 1) {
         cheb_pre_rows<1, double, BUF, NT>(far, cf, c, K, size, lds, out);
+        return kChebBlock;  // J = 1 in a one-group launch: out[1..] are never read
     } else {
         if (J <= 1) cheb_pre_rows<1, double, BUF, NT>(far, cf, c, K, size, lds, out);
         else if (J == 2) cheb_pre_rows<2, double, BUF, NT>(far, cf, c, K, size, lds, out);

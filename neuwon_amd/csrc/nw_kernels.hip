@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
-    __shared__ double b0[cheb_lds(MAXJ)];
+    __shared__ double b0[prod_lds(MAXJ, KIND)];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -696,10 +696,10 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
 // as the forward product.  The voltage adjoint passes between launches as FP64 (lam64); a call
 // reads it from lam_v at its first launch and leaves it there after its last.
 // ----------------------------------------------------------------------------------------------
-template <int HALF, int MAXJ, int NT = kChebBlock>
+template <int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last) {
-    __shared__ double b0[cheb_lds(MAXJ)];
+    __shared__ double b0[prod_lds(MAXJ, KIND)];
     __shared__ double red[7][NT / kWave];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
@@ -726,7 +726,7 @@ k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell<MAXJ, NT>(cs, row0, cell, nrows, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ, NT, KIND>(cs, row0, cell, nrows, pre_base, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
@@ -1233,24 +1233,28 @@ int nw_hh_adjoint(const nw_propagator* p, const nw_hh_params* prm, int32_t n_ste
         const ChebStore cs = cheb_store(p);
         const unsigned grid = (unsigned)p->num_cells;
         hipStream_t s = (hipStream_t)stream;
-        const bool one = p->cheb_maxj == 1;
-        constexpr int SB = kChebSmallBlock;
-        for (int k = n_steps - 1; k >= 0; --k) {
-            if (p->cheb_small) {
-                k_hh_adj_cheb<1, 1, SB><<<grid, SB, 0, s>>>(a, cs, p->cheb_v64, k,
-                                                            k == n_steps - 1, 0);
-                k_hh_adj_cheb<0, 1, SB><<<grid, SB, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);
-            } else if (one) {
-                k_hh_adj_cheb<1, 1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k,
-                                                                k == n_steps - 1, 0);
-                k_hh_adj_cheb<0, 1><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);
-            } else {
-                k_hh_adj_cheb<1, kChebRows><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k,
-                                                                        k == n_steps - 1, 0);
-                k_hh_adj_cheb<0, kChebRows><<<grid, kChebBlock, 0, s>>>(a, cs, p->cheb_v64, k, 0,
-                                                                        k == 0);
+        // one launch per half step in reverse order, built for the store's product and the
+        // largest cell
+        auto launch = [&](auto maxj, auto nt, auto kind) {
+            constexpr int MJ = decltype(maxj)::value, T = decltype(nt)::value;
+            constexpr int KD = decltype(kind)::value;
+            for (int k = n_steps - 1; k >= 0; --k) {
+                k_hh_adj_cheb<1, MJ, T, KD><<<grid, T, 0, s>>>(a, cs, p->cheb_v64, k,
+                                                               k == n_steps - 1, 0);
+                k_hh_adj_cheb<0, MJ, T, KD><<<grid, T, 0, s>>>(a, cs, p->cheb_v64, k, 0, k == 0);
             }
-        }
+        };
+        auto by_shape = [&](auto kind) {
+            using I = std::integral_constant<int, 1>;
+            using I5 = std::integral_constant<int, kChebRows>;
+            using TS = std::integral_constant<int, kChebSmallBlock>;
+            using TB = std::integral_constant<int, kChebBlock>;
+            if (p->cheb_small) launch(I{}, TS{}, kind);
+            else if (p->cheb_maxj == 1) launch(I{}, TB{}, kind);
+            else launch(I5{}, TB{}, kind);
+        };
+        if (p->rational) by_shape(std::integral_constant<int, kProdRational>{});
+        else by_shape(std::integral_constant<int, kProdChebyshev>{});
         NW_HIP(hipGetLastError());
         return NW_OK;
     }

@@ -1153,7 +1153,7 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
 template <int MAXJ, int NT = kChebBlock>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_half_step_cheb(HalfArgs a, ChebStore cs) {
-    __shared__ double b0[cheb_lds(MAXJ)];
+    __shared__ double b0[prod_lds(MAXJ, kProdAny)];
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int n = (int)(a.cell_row0[cell + 1] - row0);
