@@ -3,9 +3,11 @@
 Workload (default): a population of independent random-tree cells, 200-5000 compartments each
 (uniform), drawn until 10M compartments; leak + HH Na/K as two-state kinetic schemes on every
 segment, per-cell gNa/gK scaled logU(0.5, 2); Poisson 5 Hz current pulses (0.5 nA x 1 ms) at every
-root; FP32 state; dt = 0.1 ms.  The propagator exp(A dt) of every cell is applied as a Chebyshev
-series of its tree operator (--format chebyshev, no stored matrix, unpruned), or stored pruned at
-1e-9 (--format symmetric / general, FP64 construction).
+root; FP32 state; dt = 0.1 ms.  The propagator exp(A dt) of every cell is applied without a stored matrix,
+unpruned, as a Chebyshev series of its tree operator (--format chebyshev) or as five exact tree
+solves of a rational approximation (--format rational) -- by default (--format auto) whichever
+the autotuner times faster on this workload before the timed region -- or stored pruned at 1e-9
+(--format symmetric / general, FP64 construction).
 
 A step is one staggered time step of every compartment.  W warm-up steps run first, then exactly K
 timed steps, bracketed by torch.cuda.synchronize().  Prints one JSON line.
@@ -67,12 +69,13 @@ def parse():
     ap.add_argument("--launch-steps", type=int, default=0,
                     help="steps per kernel launch (0: all timed steps in one launch)")
     ap.add_argument("--block-threads", type=int, default=0)
-    ap.add_argument("--format", choices=("symmetric", "general", "chebyshev", "rational"),
-                    default="chebyshev",
+    ap.add_argument("--format", choices=("symmetric", "general", "chebyshev", "rational", "auto"),
+                    default="auto",
                     help="propagator store: symmetric half (S = W P W^-1), general CSR, "
-                         "chebyshev (no stored matrix: series of the tree operator, unpruned) or "
+                         "chebyshev (no stored matrix: series of the tree operator, unpruned), "
                          "rational (no stored matrix: five-pole rational approximation, exact "
-                         "tree solves, unpruned)")
+                         "tree solves, unpruned) or auto (the faster matrix-free store on this "
+                         "workload, timed by the autotuner before the timed region)")
     ap.add_argument("--cpu-sample-cells", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -300,6 +303,9 @@ def main():
                                         "duration_ms": 1.0, "seed": args.seed},
                                   block_threads=args.block_threads, store=args.format)
     st = pop.propagator.stats
+    fmt = pop.propagator.store
+    if pop.tuning.get("store"):
+        log(f"autotuner: {pop.tuning['store']}")
     nnz_bar = NNZ_BAR_PRUNED_1E9 if pop.propagator.chebyshev else pop.propagator.mean_nnz_per_row
     log(f"population ready in {time.time() - t0:.1f}s: {pop.num_segments} compartments, "
         f"nnz/row {nnz_bar:.1f}, propagator build {st['build_seconds']:.1f}s, "
@@ -333,8 +339,8 @@ def main():
     bytes_per_launch = b_step * N * (K / launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     stored_bytes_step = (st["stored"] * 6.0 * 2) / N  # FP32 value + u16 column, two half steps
-    f_step = format_bytes(args.format, nnz_bar)
-    traffic = MEASURED_TRAFFIC_PER_CMPT_STEP[args.format]
+    f_step = format_bytes(fmt, nnz_bar)
+    traffic = MEASURED_TRAFFIC_PER_CMPT_STEP[fmt]
     units_per_s = N * (K / launches) / avg_launch_s
     out = {
         "metric": "compartment-steps/s @10M cmpts: FP32 batched vs FP64 plain; adjoint grads/s",
@@ -356,9 +362,10 @@ def main():
                                 "series of each cell's tree operator (no stored matrix, unpruned)"
                                 if pop.propagator.chebyshev else
                                 "batched HH population, FP32 state, FP64-built propagator pruned 1e-9"),
-                   "propagator_format": args.format, "compartments": N, "cells": pop.num_cells, "dt_ms": args.dt,
+                   "propagator_format": fmt, "compartments": N, "cells": pop.num_cells, "dt_ms": args.dt,
                    "nnz_per_row": round(nnz_bar, 2), "steps_per_launch": L,
-                   "propagator_build_s": round(st["build_seconds"], 2)},
+                   "propagator_build_s": round(st["build_seconds"], 2),
+                   "autotune": pop.tuning.get("store")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic * N * (K / launches),
@@ -398,7 +405,7 @@ def main():
     if not args.no_adjoint:
         del pop
         torch.cuda.empty_cache()
-        out["adjoint"] = adjoint_rate(store=args.format if pop_store_free else None)
+        out["adjoint"] = adjoint_rate(store=fmt if pop_store_free else None)
     print(json.dumps(out), flush=True)
 
 

@@ -175,7 +175,11 @@ class HHPopulation:
     stim: dict(mode='constant'|'poisson'|None, amp_na, rate_hz, duration_ms, seed) applied to the
     root (segment 0) of every cell.
     symmetric: store the propagator in the symmetric half format (half the bytes per step).
-    store: "general", "symmetric" or "chebyshev" (no stored matrix; see Propagator).
+    store: "general", "symmetric", "chebyshev" or "rational" (no stored matrix; see Propagator),
+    or "auto": the faster of the two matrix-free stores on these cells, timed by the autotuner.
+    block_threads: workgroup size of the stored formats (128, 256, 512, 1024; 0: the default), or
+    "auto": timed by the autotuner on this population, whose state is then reset.  The
+    autotuner's measurements are kept in ``tuning``.
     """
 
     def __init__(self, cells, dt=0.1, celsius=6.3, v0=-70.0, cytoplasmic_resistance=100.0,
@@ -190,7 +194,17 @@ class HHPopulation:
         self.dt = float(dt)
         self.celsius = float(celsius)
         self.gate_mode = gate_mode
-        self.block_threads = int(block_threads)
+        self.tuning = {}
+        if store == "auto":
+            from . import autotune
+            store, self.tuning["store"] = autotune.choose_store(
+                cells, dt=dt, celsius=celsius, v0=v0, cytoplasmic_resistance=cytoplasmic_resistance,
+                membrane_capacitance=membrane_capacitance, gbar=gbar, reversal=reversal,
+                prune=prune, gate_mode=gate_mode, stim=stim, ap_threshold=ap_threshold,
+                device=device, insert_area=insert_area)
+        tune_blocks = block_threads == "auto"
+        self.block_threads = 0 if tune_blocks else int(block_threads)
+        self.v0 = float(v0)
         g = synth.concat(cells)
         self.num_cells = len(cells)
         self.cell_ptr = g["cell_ptr"]
@@ -212,12 +226,9 @@ class HHPopulation:
                       "k": gbar["k"] * channels.CM2_TO_UM2 * area * sk,
                       "l": gbar["l"] * channels.CM2_TO_UM2 * area}
         f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=self.dev)
-        ss = channels.hh_steady_state(v0, celsius)
         N = self.num_segments
-        self.v = f32(np.full(N, v0))
-        self.m = f32(np.full(N, ss["m"]))
-        self.h = f32(np.full(N, ss["h"]))
-        self.n = f32(np.full(N, ss["n"]))
+        self.v = torch.empty(N, dtype=torch.float32, device=self.dev)
+        self.m, self.h, self.n = (torch.empty_like(self.v) for _ in range(3))
         self.g_na, self.g_k, self.g_l = f32(self.g_bar["na"]), f32(self.g_bar["k"]), f32(self.g_bar["l"])
         self.inv_c = f32(1.0 / C)
         self.grid = channels.LinearInput(-100.0, 100.0, 2000)
@@ -228,7 +239,7 @@ class HHPopulation:
         self.ap_local = torch.as_tensor(zeros_i32, device=self.dev)
         self.ap_state = torch.zeros(self.num_cells, dtype=torch.uint8, device=self.dev)
         self.ap_count = torch.zeros(self.num_cells, dtype=torch.int32, device=self.dev)
-        self.steps = 0
+        self.reset_state()
         p = _lib.NwHHParams()
         p.dt_ms, p.vmin, p.vmax, p.num_nodes = self.dt, self.grid.minimum, self.grid.maximum, \
             self.grid.num_buckets + 1
@@ -239,6 +250,26 @@ class HHPopulation:
         p.block_threads = self.block_threads
         self.params = p
         self.set_stim(stim)
+        if tune_blocks:
+            if self.propagator.chebyshev:
+                self.tuning["block_threads"] = {
+                    "choice": 0, "note": "matrix-free stores: the launch shape follows the cell sizes"}
+            else:
+                from . import autotune
+                self.block_threads, self.tuning["block_threads"] = \
+                    autotune.choose_block_threads(self)
+                self.reset_state()
+
+    def reset_state(self):
+        """Voltages at v0, gates at their steady state, AP detectors and the step count at 0."""
+        ss = channels.hh_steady_state(self.v0, self.celsius)
+        self.v.fill_(self.v0)
+        self.m.fill_(float(ss["m"]))
+        self.h.fill_(float(ss["h"]))
+        self.n.fill_(float(ss["n"]))
+        self.ap_state.zero_()
+        self.ap_count.zero_()
+        self.steps = 0
 
     def set_stim(self, stim):
         """Stimulus of the root of every cell from the next step on (None switches it off)."""

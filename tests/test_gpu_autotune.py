@@ -1,0 +1,57 @@
+"""The autotuner on the GPU: HHPopulation(store="auto") times both matrix-free stores and then
+behaves exactly like a population built with the chosen store; block_threads="auto" on a stored
+format times every workgroup size, resets the state and then matches a fresh population with the
+chosen size bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-autotune] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _run(pop, steps):
+    pop.advance(steps)
+    pop.torch.cuda.synchronize()
+    return pop.v.cpu().numpy().copy(), pop.ap_count.cpu().numpy().copy()
+
+
+def test_store_auto(torch_gpu):
+    from neuwon_amd import population, synth
+    cells = synth.population(total=30_000, seed=7, size_range=(200, 5000))
+    stim = {"mode": "poisson", "amp_na": 0.5, "rate_hz": 40.0, "duration_ms": 1.0, "seed": 3}
+    pop = population.HHPopulation(cells, stim=stim, store="auto")
+    rep = pop.tuning["store"]
+    _say(f"{pop.num_cells} cells: {rep}")
+    assert set(rep["ms_per_step"]) == {"chebyshev", "rational"}
+    assert pop.propagator.store == rep["choice"] == min(rep["ms_per_step"], key=rep["ms_per_step"].get)
+    assert pop.steps == 0
+    ref = population.HHPopulation(cells, stim=stim, store=rep["choice"])
+    v, ap = _run(pop, 60)
+    v_ref, ap_ref = _run(ref, 60)
+    assert np.array_equal(v, v_ref) and np.array_equal(ap, ap_ref)
+    again = population.HHPopulation(cells, stim=stim, store="auto")
+    assert again.tuning["store"]["cached"] and again.propagator.store == rep["choice"]
+
+
+def test_block_threads_auto_on_the_general_store(torch_gpu):
+    from neuwon_amd import population, synth
+    cells = synth.population(total=8_000, seed=8, size_range=(200, 2000))
+    stim = {"mode": "constant", "amp_na": 0.5}
+    pop = population.HHPopulation(cells, stim=stim, store="general", block_threads="auto")
+    rep = pop.tuning["block_threads"]
+    _say(f"{pop.num_cells} cells, general store: {rep}")
+    assert set(rep["ms_per_step"]) == {128, 256, 512, 1024}
+    assert pop.block_threads == rep["choice"] == pop.params.block_threads
+    assert pop.steps == 0 and float(pop.v.max()) == float(pop.v.min()) == -70.0
+    ref = population.HHPopulation(cells, stim=stim, store="general", block_threads=rep["choice"])
+    v, ap = _run(pop, 40)
+    v_ref, ap_ref = _run(ref, 40)
+    assert np.array_equal(v, v_ref) and np.array_equal(ap, ap_ref)
+    cheb = population.HHPopulation(cells, stim=stim, store="chebyshev", block_threads="auto")
+    assert cheb.tuning["block_threads"]["choice"] == 0
