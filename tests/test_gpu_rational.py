@@ -207,3 +207,41 @@ def test_rational_refusals(torch_gpu):
     rc = _lib.load().nw_ca_step(pop.propagator.handle, ctypes.byref(p), 1, _lib.ptr(pop.v),
                                 _lib.ptr(pop.m), _lib.ptr(pop.h), None)
     assert rc != 0 and b"general format" in _lib.load().nw_last_error()
+
+
+def test_rational_deep_cable_and_level_limit(torch_gpu):
+    """An unbranched cable of 1,500 segments (1,500 levels, one row per level: every level step
+    is a single row) with 0.5 nA at the root, 150 steps (a spike travels down the cable), every
+    segment against the FP64 oracle beside the Chebyshev store: within 1e-5 relative or three
+    times the Chebyshev store's own error (the FP32-storage floor at the spike's front), and
+    within 2e-2 mV; a cable deeper than the 4,095 levels the store holds is refused."""
+    from neuwon_amd import population, synth
+    torch = torch_gpu
+    rng = np.random.default_rng(5)
+
+    def cable(n):
+        parent = np.arange(-1, n - 1, dtype=np.int32)
+        return synth.CellSpec(parent, rng.uniform(10.0, 30.0, n),
+                              np.exp(rng.uniform(np.log(0.5), np.log(4.0), n)))
+    cells = [cable(1500)]
+    steps = 150
+    v = {}
+    for store in ("rational", "chebyshev"):
+        pop = population.HHPopulation(cells, store=store, stim={"mode": "constant", "amp_na": 0.5})
+        if store == "rational":
+            assert pop.propagator.stats["max_levels"] == 1500
+        pop.advance(steps)
+        torch.cuda.synchronize()
+        v[store] = pop.v.cpu().numpy().astype(np.float64)
+    co = _oracle_population(pop, ("constant", 0.5e-9))
+    co.advance(steps, threads=len(os.sched_getaffinity(0)))
+    rel = {k: np.abs(x - co.v) / np.abs(co.v) for k, x in v.items()}
+    i = int(np.argmax(rel["rational"]))
+    _say(f"cable of 1500 segments: max rel rational {rel['rational'].max():.2e} (segment {i}, "
+         f"{co.v[i]:.2f} mV), Chebyshev {rel['chebyshev'].max():.2e}; median rational "
+         f"{np.median(rel['rational']):.1e}; max |dv| {np.abs(v['rational'] - co.v).max():.2e} mV")
+    assert rel["rational"].max() < max(1e-5, 3.0 * rel["chebyshev"].max())
+    assert np.abs(v["rational"] - co.v).max() < 2e-2
+    assert np.median(rel["rational"]) < 1e-6
+    with pytest.raises(RuntimeError, match="levels"):
+        population.HHPopulation([cable(4200)], store="rational")
