@@ -83,6 +83,9 @@ def parse():
                     help="skip the gradients/s measurement of the reverse pass")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--accumulate", action="store_true",
+                    help="only time the segmented accumulation (nw_accumulate) on --compartments "
+                         "segments with three instances each (hh's na, k, l), one JSON line")
     return ap.parse_args()
 
 
@@ -282,6 +285,61 @@ def copy_probe(args):
     print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 3}), flush=True)
 
 
+def accumulate_probe(args):
+    """The segmented accumulation alone (the model path's nw_accumulate): every segment carries
+    three conductance sources (hh's na, k and l, instances sorted by segment, the conductances
+    gathered through an index as the model's interleaved mechanism output is), FP32 sources,
+    int64 offsets and indices, sums in FP64.  Algorithmic bytes: per source the conductance, the
+    reversal potential and the index (4 + 4 + 8), per segment its offset (8) and the three outputs
+    (12)."""
+    import ctypes
+    import torch
+    from neuwon_amd import _lib
+    N, per = int(args.compartments), 3
+    M = N * per
+    dev = "cuda"
+    gen = torch.Generator(device=dev).manual_seed(args.seed)
+    ptr = torch.arange(0, M + 1, per, dtype=torch.int64, device=dev)
+    g = torch.rand(M, generator=gen, device=dev) * 1e-9
+    idx = torch.arange(M, dtype=torch.int64, device=dev)
+    e = torch.tensor([50.0, -77.0, -54.3], device=dev).repeat(N)
+    out = [torch.empty(N, device=dev) for _ in range(3)]
+    lib = _lib.load()
+    P = _lib.ptr
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        _lib.check(lib.nw_accumulate(N, P(ptr), P(g), P(idx), P(e), None, P(out[0]), P(out[1]),
+                                     P(out[2]), ctypes.c_void_p(stream.cuda_stream)))
+    for _ in range(max(args.warmup, 1)):
+        launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    # check a sample against a host sum
+    k = min(N, 4096)
+    gs = g[:k * per].double().view(k, per).cpu().numpy()
+    es = e[:k * per].double().view(k, per).cpu().numpy()
+    want_g = gs.sum(1)
+    want_d = (gs * es).sum(1) / want_g
+    err = max(float(np.max(np.abs(out[0][:k].double().cpu().numpy() - want_g) / want_g)),
+              float(np.max(np.abs(out[1][:k].double().cpu().numpy() - want_d) / np.abs(want_d))))
+    alg = M * 16.0 + N * 20.0
+    gbs = alg / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "segmented accumulation (nw_accumulate) bandwidth", "value": gbs,
+                      "unit": "GB/s", "segments": N, "sources_per_segment": per,
+                      "avg_launch_ms": ms, "launches": args.steps,
+                      "algorithmic_bytes_per_launch": alg,
+                      "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS},
+                      "max_rel_error_vs_host_fp64": err}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -291,6 +349,9 @@ def main():
         sys.exit(2)
     if args.copy_probe_mib:
         copy_probe(args)
+        return
+    if args.accumulate:
+        accumulate_probe(args)
         return
     import torch
     from neuwon_amd import population, synth

@@ -198,12 +198,18 @@ def test_lti_13_state(torch_gpu):
     assert np.abs(got.sum(axis=0) - 1).max() < 1e-6
 
 
-def test_accumulate(torch_gpu):
+@pytest.mark.parametrize("layout", ["few", "crowded", "indexed"])
+def test_accumulate(torch_gpu, layout):
+    """0-3 sources per segment (every block staged in LDS), a block of segments with 3,000 sources
+    on one segment (beyond the 2,048 a block stages: summed from memory), and the conductances
+    gathered through a permuted index."""
     from neuwon_amd import _lib
     torch = torch_gpu
     rng = np.random.default_rng(9)
     nseg = 5000
     per = rng.integers(0, 4, nseg)
+    if layout == "crowded":
+        per[300] = 3000
     ptr = np.zeros(nseg + 1, dtype=np.int64)
     np.cumsum(per, out=ptr[1:])
     M = int(ptr[-1])
@@ -212,12 +218,21 @@ def test_accumulate(torch_gpu):
     E = rng.choice([60.0, -88.0, -54.3], M)
     Inon = rng.uniform(-1e-10, 1e-10, nseg)
     t = lambda a, dt=np.float32: torch.as_tensor(np.asarray(a, dtype=dt), device="cuda")
-    d_ptr, d_g, d_e, d_i = t(ptr, np.int64), t(g), t(E), t(Inon)
+    d_ptr, d_e, d_i = t(ptr, np.int64), t(E), t(Inon)
+    if layout == "indexed":      # source k's conductance sits at g_buf[perm[k]]
+        perm = rng.permutation(M)
+        g_buf = np.empty(M)
+        g_buf[perm] = g
+        d_g, d_idx = t(g_buf), t(perm, np.int64)
+    else:
+        d_g, d_idx = t(g), None
     sg, dv, si = (torch.zeros(nseg, device="cuda") for _ in range(3))
-    _lib.check(_lib.load().nw_accumulate(nseg, _lib.ptr(d_ptr), _lib.ptr(d_g), None, _lib.ptr(d_e),
-                                         _lib.ptr(d_i), _lib.ptr(sg), _lib.ptr(dv), _lib.ptr(si),
-                                         None))
+    _lib.check(_lib.load().nw_accumulate(nseg, _lib.ptr(d_ptr), _lib.ptr(d_g), _lib.ptr(d_idx),
+                                         _lib.ptr(d_e), _lib.ptr(d_i), _lib.ptr(sg), _lib.ptr(dv),
+                                         _lib.ptr(si), None))
     torch.cuda.synchronize()
+    g = np.asarray(g, np.float32).astype(np.float64)
+    E = np.asarray(E, np.float32).astype(np.float64)
     ref_g, ref_num = O.accumulate_scatter(nseg, seg_of, g, E)
     ref_d = O.driving_voltage(ref_g, ref_num)
     assert np.allclose(sg.cpu().numpy(), ref_g, rtol=1e-6, atol=1e-16)
