@@ -25,6 +25,7 @@
 #include <complex>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "nw_internal.h"
 
@@ -1150,10 +1151,10 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
 }
 
 // Chebyshev store: x in LDS as FP64, v = sum_k c_k T_k(A~) x (cheb_rows).
-template <int MAXJ, int NT = kChebBlock>
+template <int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_half_step_cheb(HalfArgs a, ChebStore cs) {
-    __shared__ double b0[prod_lds(MAXJ, kProdAny)];
+    __shared__ double b0[prod_lds(MAXJ, KIND)];
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int n = (int)(a.cell_row0[cell + 1] - row0);
@@ -1192,7 +1193,7 @@ k_half_step_cheb(HalfArgs a, ChebStore cs) {
     }
     __syncthreads();
     double out[kChebRows];
-    const int places = cheb_cell<MAXJ, NT>(cs, row0, cell, n, pre_base, b0, out);
+    const int places = cheb_cell<MAXJ, NT, KIND>(cs, row0, cell, n, pre_base, b0, out);
 #pragma unroll
     for (int j = 0; j < kChebRows; ++j) {
         const int r = threadIdx.x + j * kChebBlock;
@@ -1404,15 +1405,22 @@ int nw_electric_half_step(const nw_propagator* p, float half_dt_ms, float* v, co
     if (p->chebyshev) {
         NW_REQUIRE(p->max_cell_rows <= kChebBuf, "cell of %d segments exceeds the Chebyshev store",
                    p->max_cell_rows);
-        if (p->cheb_small)
-            k_half_step_cheb<1, kChebSmallBlock><<<(unsigned)p->num_cells, kChebSmallBlock, 0,
-                                                   (hipStream_t)stream>>>(a, cheb_store(p));
-        else if (p->cheb_maxj == 1)
-            k_half_step_cheb<1><<<(unsigned)p->num_cells, kChebBlock, 0, (hipStream_t)stream>>>(
-                a, cheb_store(p));
-        else
-            k_half_step_cheb<kChebRows><<<(unsigned)p->num_cells, kChebBlock, 0,
-                                          (hipStream_t)stream>>>(a, cheb_store(p));
+        // built for the store's product, the workgroup size and the largest cell
+        auto launch = [&](auto kind) {
+            constexpr int KD = decltype(kind)::value;
+            const unsigned grid = (unsigned)p->num_cells;
+            hipStream_t s = (hipStream_t)stream;
+            if (p->cheb_small)
+                k_half_step_cheb<1, kChebSmallBlock, KD><<<grid, kChebSmallBlock, 0, s>>>(
+                    a, cheb_store(p));
+            else if (p->cheb_maxj == 1)
+                k_half_step_cheb<1, kChebBlock, KD><<<grid, kChebBlock, 0, s>>>(a, cheb_store(p));
+            else
+                k_half_step_cheb<kChebRows, kChebBlock, KD><<<grid, kChebBlock, 0, s>>>(
+                    a, cheb_store(p));
+        };
+        if (p->rational) launch(std::integral_constant<int, kProdRational>{});
+        else launch(std::integral_constant<int, kProdChebyshev>{});
         NW_HIP(hipGetLastError());
         return NW_OK;
     }
