@@ -26,7 +26,10 @@ class HHAdjoint:
     recomputation with the record) and of the reverse pass: "chebyshev" (the default for both
     when every cell has at most 5120 segments), "symmetric" or "general" (see
     population.Propagator).  symmetric: the stored format chosen where a store is not named and
-    the cells do not fit the Chebyshev store (symmetric: half the bytes)."""
+    the cells do not fit the Chebyshev store (symmetric: half the bytes).
+    checkpoint_every: steps between the forward pass's checkpoints, each segment recomputed with
+    its record in the reverse pass; None keeps the record of every step from the forward pass
+    in device memory instead ((5 T + 4) FP32 values per segment) and recomputes nothing."""
 
     def __init__(self, cells, dt=0.05, celsius=6.3, stim_amp_na=0.5, prune=1e-9,
                  checkpoint_every=100, v0=-70.0, device=0, symmetric=True, forward_store=None,
@@ -49,7 +52,9 @@ class HHAdjoint:
         self.forward_store = forward_store
         self.torch = self.pop.torch
         self.dev = self.pop.dev
-        self.K = int(checkpoint_every)
+        # checkpoint_every=None keeps the whole record in device memory (5 T + 4 floats per
+        # segment: 20.5 GB for 512k segments x 2000 steps): no recomputation in the reverse pass
+        self.K = None if checkpoint_every is None else int(checkpoint_every)
         self.v0 = float(v0)
         self.N = self.pop.num_segments
         self.num_cells = self.pop.num_cells
@@ -87,6 +92,14 @@ class HHAdjoint:
         self.target = target.contiguous()
         trace = torch.zeros((self.num_cells, T), dtype=torch.float32, device=self.dev)
         self.ckpts = []
+        if self.K is None:   # one pass that writes the whole record
+            need = (5 * T + 4) * self.N
+            if self._record is None or self._record.numel() < need:
+                self._record = None
+                self._record = torch.empty(need, dtype=torch.float32, device=self.dev)
+            pop.advance(T, trace=trace, record=self._record)
+            self.trace = trace
+            return float(((trace.double() - self.target.double()) ** 2).sum().item())
         for s0 in range(0, T, self.K):
             n = min(self.K, T - s0)
             self.ckpts.append((s0, n, pop.v.clone(), pop.m.clone(), pop.h.clone(), pop.n.clone()))
@@ -107,18 +120,23 @@ class HHAdjoint:
         grads = [z64() for _ in range(3)]
         gth = torch.zeros((self.num_cells, 6), dtype=torch.float64, device=self.dev)
         loss = torch.zeros(self.num_cells, dtype=torch.float64, device=self.dev)
-        need = (5 * self.K + 4) * N
-        if self._record is None or self._record.numel() < need:
-            self._record = torch.empty(need, dtype=torch.float32, device=self.dev)
         P = _lib.ptr
         prm = _lib.NwHHParams.from_buffer_copy(pop.params)
-        for s0, n, v, m, h, nn in reversed(self.ckpts):
-            pop.v.copy_(v)
-            pop.m.copy_(m)
-            pop.h.copy_(h)
-            pop.n.copy_(nn)
-            pop.steps = s0
-            pop.advance(n, record=self._record)
+        if self.K is None:   # the forward pass kept the record of every step
+            segments = [(0, self.T, None, None, None, None)]
+        else:
+            need = (5 * self.K + 4) * N
+            if self._record is None or self._record.numel() < need:
+                self._record = torch.empty(need, dtype=torch.float32, device=self.dev)
+            segments = reversed(self.ckpts)
+        for s0, n, v, m, h, nn in segments:
+            if v is not None:   # recompute the segment from its checkpoint, writing the record
+                pop.v.copy_(v)
+                pop.m.copy_(m)
+                pop.h.copy_(h)
+                pop.n.copy_(nn)
+                pop.steps = s0
+                pop.advance(n, record=self._record)
             prm.step0 = s0
             prm.conductances_valid = 1 if s0 > 0 else 0
             s = torch.cuda.current_stream(self.dev)

@@ -212,3 +212,41 @@ def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
     _say(f"finite differences took {time.time() - t0:.0f}s on {threads} threads")
     for name, adj, fd in checks:
         assert adj == pytest.approx(fd, rel=1e-4), name
+
+
+def test_adjoint_whole_record_matches_checkpointed(torch_gpu):
+    """checkpoint_every=None: the forward pass keeps the record of every step and the reverse
+    pass recomputes nothing.  Gradients against the checkpointed run (they differ only where the
+    checkpointed run restarts a segment from FP32 voltages) and against the FP64 oracle adjoint."""
+    from neuwon_amd import adjoint, synth
+    torch = torch_gpu
+    dt, T, amp = 0.05, 300, 0.1e-9
+    cells = synth.uniform_population(num_cells=4, n=90, seed=6)
+    f32 = lambda a: torch.as_tensor(a.astype(np.float32), device="cuda")
+    res = {}
+    for K in (100, None):
+        prob = adjoint.HHAdjoint(cells, dt=dt, stim_amp_na=amp * 1e9, checkpoint_every=K)
+        pop = prob.pop
+        gb = {k: pop.g_bar[k] for k in ("na", "k", "l")}
+        rng = np.random.default_rng(7)
+        targets = np.zeros((pop.num_cells, T))
+        oracles = []
+        for c in range(pop.num_cells):
+            r0, r1, op = _oracle_problem(prob, c, amp, dt)
+            hid = _hidden({k: gb[k][r0:r1] for k in gb}, rng)
+            targets[c], _ = op.forward(hid, {}, T)
+            oracles.append((r0, r1, op))
+        tgt = torch.as_tensor(targets.astype(np.float32), device="cuda")
+        res[K] = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), tgt)
+    (l_ck, g_ck), (l_rec, g_rec) = res[100], res[None]
+    errs = [_rel(g_rec[k].cpu().numpy(), g_ck[k].cpu().numpy()) for k in ("na", "k", "l")]
+    th = np.max(np.abs(g_rec["theta"] - g_ck["theta"]) / np.abs(g_ck["theta"]))
+    _say(f"whole record vs checkpointed: loss {l_rec:.6e} / {l_ck:.6e}, per-segment rel "
+         f"{['%.1e' % e for e in errs]}, theta {th:.1e}")
+    assert l_rec == pytest.approx(l_ck, rel=1e-5)
+    assert max(errs) < 1e-4 and th < 1e-4
+    for c, (r0, r1, op) in enumerate(oracles):
+        _, ref = op.gradient({k: gb[k][r0:r1] for k in gb}, {}, targets[c])
+        e = max(_rel(g_rec[k].cpu().numpy()[r0:r1], ref[k]) for k in ("na", "k", "l"))
+        assert e < 1e-3
+
