@@ -1052,12 +1052,18 @@ class RxD_Model:
         # symmetric_propagator: store sqrt(C) P sqrt(C)^-1, one triangle (half the bytes per half
         # step; pairs pruned together; sums in atomic order, so not bitwise reproducible)
         self.symmetric_propagator = bool(symmetric_propagator)
-        # propagator_store: "general" | "symmetric" | "chebyshev" (overrides symmetric_propagator).
-        # Unset: the Chebyshev store (exp(A dt) without a stored matrix) when nothing is pruned,
-        # the reference's default threshold, and every cell fits it; otherwise the general store
+        # propagator_store: "general" | "symmetric" | "chebyshev" | "rational" (overrides
+        # symmetric_propagator), or "auto": both matrix-free stores are built and timed on this
+        # model's half step, the faster is kept (``tuning``).  Unset: the Chebyshev store
+        # (exp(A dt) without a stored matrix) when nothing is pruned, the reference's default
+        # threshold, and every cell fits it; otherwise the general store
         from .population import store_name
-        self.propagator_store = (None if propagator_store is None and not self.symmetric_propagator
-                                 else store_name(propagator_store, self.symmetric_propagator))
+        self.tuning = {}
+        if propagator_store == "auto":
+            self.propagator_store = "auto"
+        else:
+            self.propagator_store = (None if propagator_store is None and not self.symmetric_propagator
+                                     else store_name(propagator_store, self.symmetric_propagator))
         # step_graph: when every callback of a step runs on the device (device mechanisms only,
         # play back / recording / traces of device attributes), advance() replays a captured
         # graph of the step's kernels instead of launching them one by one from Python
@@ -1457,10 +1463,57 @@ class RxD_Model:
             if store is None:
                 fits = int(np.diff(self._cell_ptr).max(initial=0)) <= CHEBYSHEV_MAX_CELL
                 store = "chebyshev" if self.prune <= EPSILON and fits else "general"
-            prop = Propagator(self._cell_ptr, self._parent_local, self._R, self._C, time_step,
-                              self.prune, self.device, store=store)
+            if store == "auto":
+                prop = self._tuned_propagator(time_step)
+            else:
+                prop = Propagator(self._cell_ptr, self._parent_local, self._R, self._C, time_step,
+                                  self.prune, self.device, store=store)
             self._propagators = {key: prop}
         return prop
+
+    def _tuned_propagator(self, time_step, warmup=2, steps=6):
+        """The faster matrix-free store for this model's half step: each candidate runs the half
+        step on a copy of the voltages (the model's state is not touched), timed with HIP events;
+        a candidate that does not fit the cells is skipped."""
+        import time as _time
+        from .population import Propagator
+        torch = _torch()
+        t0 = _time.perf_counter()
+        v = self._col("Segment", "voltage")
+        scratch = v.clone()
+        args = [self._col("Segment", n) for n in ("sum_conductance", "driving_voltage", "sum_current")]
+        ms, skipped, built = {}, {}, {}
+        P = _lib.ptr
+        for store in ("chebyshev", "rational"):
+            try:
+                prop = Propagator(self._cell_ptr, self._parent_local, self._R, self._C, time_step,
+                                  self.prune, self.device, store=store)
+            except (RuntimeError, ValueError) as e:
+                skipped[store] = str(e)
+                continue
+
+            def run(k):
+                for _ in range(k):
+                    _lib.check(_lib.load().nw_electric_half_step(
+                        prop.handle, time_step, P(scratch), P(args[0]), P(args[1]), P(args[2]),
+                        P(self._inv_c), None, self._stream()))
+            scratch.copy_(v)
+            run(warmup)
+            stream = torch.cuda.current_stream(self._dev)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            run(steps)
+            e1.record(stream)
+            torch.cuda.synchronize(self._dev)
+            ms[store] = e0.elapsed_time(e1) / steps
+            built[store] = prop
+        if not ms:
+            raise RuntimeError(f"no matrix-free store fits these cells: {skipped}")
+        choice = min(ms, key=ms.get)
+        self.tuning["propagator_store"] = {"choice": choice, "ms_per_half_step": ms,
+                                           "skipped": skipped,
+                                           "tuning_s": _time.perf_counter() - t0}
+        return built[choice]
 
     def _electric_step(self, time_step):
         self._ensure_built()

@@ -55,3 +55,33 @@ def test_block_threads_auto_on_the_general_store(torch_gpu):
     assert np.array_equal(v, v_ref) and np.array_equal(ap, ap_ref)
     cheb = population.HHPopulation(cells, stim=stim, store="chebyshev", block_threads="auto")
     assert cheb.tuning["block_threads"]["choice"] == 0
+
+
+def test_model_propagator_store_auto(torch_gpu):
+    """RxD_Model(propagator_store="auto"): both matrix-free stores are timed on the model's half
+    step (on a copy of the voltages) and the model then runs exactly as one built with the chosen
+    store."""
+    from neuwon_amd.model import RxD_Model, TimeSeries
+
+    def run(store):
+        m = RxD_Model(time_step=0.1, temperature=6.3, mechanisms=["hh"], propagator_store=store)
+        root = m.Neuron([0, 0, 0], 12).root
+        segs = [root]
+        for k in range(3):
+            a = 2 * np.pi * k / 3
+            segs += root.add_section([200 * np.cos(a), 200 * np.sin(a), 0], 1.0,
+                                     maximum_segment_length=10)
+        for s in segs:
+            s.insert("hh")
+        probe = TimeSeries().record(root, "voltage")
+        root.inject_current(1.0, 2.0)
+        for _ in range(150):
+            m.advance()
+        return m, np.array(probe.get_data()), m.database.get_data("Segment.voltage").copy()
+    m, trace, v = run("auto")
+    rep = m.tuning["propagator_store"]
+    _say(f"model: {rep}")
+    assert set(rep["ms_per_half_step"]) == {"chebyshev", "rational"}
+    _, trace_ref, v_ref = run(rep["choice"])
+    assert trace.max() > 20
+    assert np.array_equal(trace, trace_ref) and np.array_equal(v, v_ref)
