@@ -750,20 +750,30 @@ k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last
             double gr_na = 0, gr_k = 0, gr_l = 0;
             lam_out = half_adjoint(lx, v1, mN, hN, nN, a.g_na[gi], a.g_k[gi], a.g_l[gi], e_na, e_k,
                                    e_l, hs, invc, lm, lh, ln, gr_na, gr_k, gr_l);
-            const double g0[3] = {rec[NR + gi], rec[2 * NR + gi], rec[3 * NR + gi]};
-            double lg[3] = {lm, lh, ln};
+            // one gate at a time (fewer live registers), the gate's values rotating through
+            // scalars: an array indexed by the loop counter would live in scratch memory
+            double ga = rec[NR + gi], gb = rec[2 * NR + gi], gc = rec[3 * NR + gi];
+            double la = lm, lb = lh, lc = ln;
+            double t0 = th[0], t1 = th[1], t2 = th[2], t3 = th[3], t4 = th[4], t5 = th[5];
 #pragma unroll 1
-            for (int q = 0; q < 3; ++q) {  // one gate at a time: fewer live registers
+            for (int q = 0; q < 3; ++q) {
                 GateLookup L;
                 lookup6(a.table6 + (int64_t)q * p.num_nodes * 6, p.num_nodes, v1, vmin, frq, L);
-                lam_out += (L.slope[0] + L.slope[1] * g0[q]) * lg[q];
-                th[2 * q] += (L.val[2] + L.val[3] * g0[q]) * lg[q];
-                th[2 * q + 1] += (L.val[4] + L.val[5] * g0[q]) * lg[q];
-                lg[q] *= L.val[1];
+                lam_out += (L.slope[0] + L.slope[1] * ga) * la;
+                t0 += (L.val[2] + L.val[3] * ga) * la;
+                t1 += (L.val[4] + L.val[5] * ga) * la;
+                la *= L.val[1];
+                // rotate: the next gate's values move to the front; after three turns every
+                // value is back in its place
+                const double g_ = ga, l_ = la, u0 = t0, u1 = t1;
+                ga = gb; gb = gc; gc = g_;
+                la = lb; lb = lc; lc = l_;
+                t0 = t2; t1 = t3; t2 = t4; t3 = t5; t4 = u0; t5 = u1;
             }
-            a.lam_m[gi] = (float)lg[0];
-            a.lam_h[gi] = (float)lg[1];
-            a.lam_n[gi] = (float)lg[2];
+            th[0] = t0; th[1] = t1; th[2] = t2; th[3] = t3; th[4] = t4; th[5] = t5;
+            a.lam_m[gi] = (float)la;
+            a.lam_h[gi] = (float)lb;
+            a.lam_n[gi] = (float)lc;
             a.grad_na[gi] += gr_na;
             a.grad_k[gi] += gr_k;
             a.grad_l[gi] += gr_l;
