@@ -738,7 +738,14 @@ k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last
     const double frq = (double)(p.num_nodes - 1) / ((double)p.vmax - (double)p.vmin);
     const double e_na = p.e_na, e_k = p.e_k, e_l = p.e_l;
     double th[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < nrows; i += NT) {
+    // the thread's rows unrolled (one in the one-group launches): a loop over them let the
+    // compiler hoist the constants of exp() out of it into registers that it then spilled
+    // (a 512-thread launch serves cells of at most 512 rows: one row per thread)
+    constexpr int ROWS = NT == kChebSmallBlock ? 1 : MAXJ * kChebBlock / NT;
+#pragma unroll
+    for (int jr = 0; jr < ROWS; ++jr) {
+        const int i = threadIdx.x + jr * NT;
+        if (i >= nrows) break;
         const int64_t gi = row0 + i;
         const double invc = a.inv_c[gi];
         const double lx = b0[pre ? pos[i] : i] / invc;  // (P^T lambda)_i
