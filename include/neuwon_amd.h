@@ -188,6 +188,31 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
                    const float* gmax, float* cond_out, void* stream);
 
 /* ------------------------------------------------------------------------------------------
+ * LTI kinetic tables on the device and the advance grouped by voltage bucket (the table upload
+ * and kernel of lti_sim/codegen.py:29-106 for schemes whose one input is the voltage; table
+ * layout and lookup as nw_lti_advance).
+ * nw_lti_table_create keeps the caller's FP32 node_table (device, [num_nodes][S][S], it must
+ * outlive the handle) and builds two device copies of it: the values widened to FP64 and an FP32
+ * copy with every node padded to a multiple of 4 values.
+ * nw_lti_table_advance advances M instances one step (n_steps = 1): per instance
+ *   y0 = T_b x, y1 = T_(b+1) x, x' = y0 + f (y1 - y0)   (FP64, each row summed over the columns in
+ *   ascending order), then CONSERVE and the conductance output as nw_lti_advance.
+ * A wavefront whose instances fall into at most group_max distinct buckets (-1: the library's
+ * default, 2) reads the two node matrices of each bucket once, by scalar loads; other
+ * wavefronts read every instance's nodes from the padded copy.  Both forms give the same result
+ * bit for bit.  n_steps > 1 holds the inputs over the launch and runs nw_lti_advance.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nw_lti_table nw_lti_table;
+
+int nw_lti_table_create(int device, int32_t S, int32_t num_nodes, float vmin, float vmax,
+                        const float* node_table, nw_lti_table** out);
+int nw_lti_table_destroy(nw_lti_table* t);
+int nw_lti_table_advance(const nw_lti_table* t, int64_t M, const float* voltage,
+                         const int32_t* seg_index, float* states, int64_t state_stride,
+                         float conserve_sum, int32_t n_steps, const float* weights,
+                         const float* gmax, float* cond_out, int32_t group_max, void* stream);
+
+/* ------------------------------------------------------------------------------------------
  * LTI kinetic advance with one or two inputs of either mapping (replaces the kernel generated by
  * lti_sim/codegen.py:39-106 for schemes whose inputs are the voltage and/or a concentration, e.g.
  * AMPA.mod's glutamate, NMDA.mod's transmitter and voltage; inputs mapped as
@@ -267,8 +292,24 @@ typedef struct nw_hh_params {
     int32_t block_threads;    /* 128/256/512/1024; 0 = 512 (symmetric format) or 256 (general)    */
     int32_t gate_mode;        /* 0: two-state LTI tables; 1: exact exponential update of hh.mod    */
     float   q10;              /* gate_mode 1: rate factor 3^((celsius-6.3)/10), hh.mod:67         */
-    int32_t pad_;
+    int32_t launch_mode;      /* NW_LAUNCH_* (matrix-free stores; the stored formats ignore it)    */
 } nw_hh_params;
+
+/* launch_mode of nw_hh_step on the matrix-free (Chebyshev, rational) stores, which run one launch
+ * per half step:
+ *   NW_LAUNCH_DIRECT         the 2 n_steps launches are issued one by one on `stream`;
+ *   NW_LAUNCH_GRAPH          they are replayed as one HIP graph launch on `stream`: a graph is
+ *                            captured (on a stream of the handle's own) the first time an argument
+ *                            set is seen -- every argument but prm->step0, which the kernels read
+ *                            from device memory written just before each replay -- and kept with
+ *                            the handle (the last 4 argument sets), so repeated calls replay it;
+ *                            on a stream that is itself being captured the launches are direct;
+ *   NW_LAUNCH_GRAPH_PREPARE  the graph of this argument set is captured and kept, nothing runs
+ *                            (e.g. before a timed region).
+ * A graph holds the array addresses it was captured with: calls with other arrays capture again. */
+#define NW_LAUNCH_DIRECT 0
+#define NW_LAUNCH_GRAPH 1
+#define NW_LAUNCH_GRAPH_PREPARE 2
 
 int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
                const float* gate_table, float* v, float* m, float* h, float* n, const float* g_na,

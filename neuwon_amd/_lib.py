@@ -33,7 +33,10 @@ class NwHHParams(ctypes.Structure):
                 ("stim_amp_a", c_f32), ("stim_seed", c_u32), ("poisson_threshold", c_u32),
                 ("pulse_ticks", c_i32), ("ap_threshold", c_f32), ("conductances_valid", c_i32),
                 ("step0", c_i64), ("block_threads", c_i32), ("gate_mode", c_i32), ("q10", c_f32),
-                ("pad_", c_i32)]
+                ("launch_mode", c_i32)]
+
+
+NW_LAUNCH_DIRECT, NW_LAUNCH_GRAPH, NW_LAUNCH_GRAPH_PREPARE = 0, 1, 2
 
 
 class NwCaParams(ctypes.Structure):
@@ -87,6 +90,10 @@ SIGNATURES = {
                                c_f32, c_i32, c_vp, c_vp, c_vp, c_vp]),
     "nw_lti_advance_nd": (c_i32, [c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_i64, c_f32, c_i32,
                                   c_vp, c_vp, c_vp, c_vp]),
+    "nw_lti_table_create": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_vp, ctypes.POINTER(c_vp)]),
+    "nw_lti_table_destroy": (c_i32, [c_vp]),
+    "nw_lti_table_advance": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_f32, c_i32, c_vp,
+                                     c_vp, c_vp, c_i32, c_vp]),
     "nw_hh_step": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "nw_hh_adjoint": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp,

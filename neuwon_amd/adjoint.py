@@ -85,11 +85,17 @@ class HHAdjoint:
         """Run the trajectory, keep checkpoints, return the loss (FP64 float)."""
         torch = self.torch
         pop = self.pop
+        target = torch.as_tensor(target)
+        if target.dim() != 2 or target.shape[0] != self.num_cells or target.shape[1] < 1:
+            raise ValueError(f"target of shape {tuple(target.shape)}: one row of at least one "
+                             f"step per cell ({self.num_cells} rows) expected")
         self._set_parameters(g_na, g_k, g_l, theta)
         self._reset_state()
         T = int(target.shape[1])
         self.T = T
-        self.target = target.contiguous()
+        # the reverse kernel reads FP32 rows of T values on the device: any other dtype, device
+        # or layout is converted here
+        self.target = target.to(device=self.dev, dtype=torch.float32).contiguous()
         trace = torch.zeros((self.num_cells, T), dtype=torch.float32, device=self.dev)
         self.ckpts = []
         if self.K is None:   # one pass that writes the whole record
@@ -100,10 +106,17 @@ class HHAdjoint:
             pop.advance(T, trace=trace, record=self._record)
             self.trace = trace
             return float(((trace.double() - self.target.double()) ** 2).sum().item())
+        # one trace buffer for every segment of every gradient, so that every full segment
+        # replays the same graph of the step's launches
+        if getattr(self, "_seg_trace", None) is None:
+            self._seg_trace = torch.zeros((self.num_cells, self.K), dtype=torch.float32,
+                                          device=self.dev)
+        seg_trace = self._seg_trace
         for s0 in range(0, T, self.K):
             n = min(self.K, T - s0)
             self.ckpts.append((s0, n, pop.v.clone(), pop.m.clone(), pop.h.clone(), pop.n.clone()))
-            tr = torch.zeros((self.num_cells, n), dtype=torch.float32, device=self.dev)
+            tr = seg_trace if n == self.K else \
+                torch.zeros((self.num_cells, n), dtype=torch.float32, device=self.dev)
             pop.advance(n, trace=tr)
             trace[:, s0:s0 + n] = tr
         self.trace = trace

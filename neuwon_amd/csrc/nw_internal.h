@@ -423,6 +423,18 @@ struct nw_propagator {
     int32_t rat_max_levels = 0;
     double rat_mean_levels = 0.0;
     double mean_terms = 0.0;
+    // Graphs of nw_hh_step's half-step launches on the matrix-free stores (NW_LAUNCH_GRAPH): one
+    // per argument set, least recently used dropped beyond kStepGraphs; the kernels of a graph
+    // read the first step's index from step_dev, written before every replay.
+    int64_t* step_dev = nullptr;   // [1]
+    struct StepGraph {
+        std::vector<unsigned char> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    mutable std::vector<StepGraph> graphs;
+    mutable uint64_t graph_clock = 0;
+    mutable hipStream_t capture_stream = nullptr;
     // host mirrors
     std::vector<int64_t> h_cell_row0, h_cell_nz0;
     std::vector<int32_t> h_cheb_K;
@@ -430,6 +442,8 @@ struct nw_propagator {
     std::vector<double> h_cheb_c, h_rho;
     int64_t vals_capacity = 0;
 };
+
+constexpr int kStepGraphs = 4;
 
 // Device pointers of a Chebyshev store, passed to the kernels that read it.
 struct ChebStore {

@@ -13,6 +13,7 @@
 // The voltage of the cell stays in LDS across steps; gates and per-segment constants stream from
 // HBM (they are a few percent of the propagator's bytes).
 #include <algorithm>
+#include <cstring>
 
 #include <type_traits>
 
@@ -50,6 +51,8 @@ struct HHArgs {
     int64_t num_rows;
     const double* wgt;   // symmetric format: w of every segment
     ChebStore cheb;      // Chebyshev store
+    const int64_t* step_dev;  // graph replays: index of the first step in device memory (else
+                              // prm.step0)
 };
 
 // Stimulus current (A) of the cell's stimulated segment on input tick `tick` (1-based).
@@ -301,7 +304,7 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     const uint16_t* pos = a.cheb.pre_pos + row0;
     const double hdt = 0.5 * (double)p.dt_ms;
     const double hdt_s = hdt * 1e-3;
-    const int64_t gstep = p.step0 + step;
+    const int64_t gstep = (a.step_dev ? *a.step_dev : p.step0) + step;
     const int64_t tick = 2 * gstep + 1 + HALF;
     const int stim_row = a.stim_local ? a.stim_local[cell] : -1;
     const bool g_valid = HALF == 1 || p.conductances_valid || step > 0;
@@ -1164,6 +1167,70 @@ __global__ void k_ap_detect(int64_t num, const int32_t* __restrict__ seg,
     state[i] = over ? 1 : 0;
 }
 
+__global__ void k_set_i64(int64_t* __restrict__ dst, int64_t value) {
+    if (threadIdx.x == 0) *dst = value;
+}
+
+// The caller-provided arguments of a call (every one but prm.step0) as bytes: the key of a
+// graph.  Arguments derived from the handle are the same for every call on it.
+static std::vector<unsigned char> hh_step_key(const nw_hh_params& prm, int n_steps, const HHArgs& a) {
+    static_assert(sizeof(nw_hh_params) == 80, "nw_hh_params has no padding");
+    nw_hh_params q = prm;
+    q.step0 = 0;
+    const void* ptrs[] = {a.table, a.v, a.m, a.h, a.n, a.g_na, a.g_k, a.g_l, a.inv_c, a.stim_local,
+                          a.ap_local, a.ap_state, a.ap_count, a.trace, a.record};
+    std::vector<unsigned char> key(sizeof q + sizeof ptrs + 2 * sizeof(int64_t));
+    unsigned char* k = key.data();
+    std::memcpy(k, &q, sizeof q);
+    std::memcpy(k + sizeof q, ptrs, sizeof ptrs);
+    const int64_t tail[2] = {n_steps, a.trace_stride};
+    std::memcpy(k + sizeof q + sizeof ptrs, tail, sizeof tail);
+    return key;
+}
+
+// NW_LAUNCH_GRAPH / NW_LAUNCH_GRAPH_PREPARE: find or capture the graph of this argument set,
+// then (GRAPH) write the first step's index to the handle's device word and replay the graph.
+template <typename Issue>
+static int hh_step_graph(const nw_propagator* p, const nw_hh_params& prm, int n_steps, HHArgs a,
+                         Issue& issue, hipStream_t s) {
+    NW_REQUIRE(p->step_dev, "the store has no device step counter");
+    a.step_dev = p->step_dev;
+    std::vector<unsigned char> key = hh_step_key(prm, n_steps, a);
+    nw_propagator::StepGraph* hit = nullptr;
+    for (auto& g : p->graphs)
+        if (g.key == key) hit = &g;
+    if (!hit) {
+        if (!p->capture_stream)
+            NW_HIP(hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        NW_HIP(hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal));
+        issue(a, p->capture_stream);
+        const hipError_t launch_err = hipGetLastError();
+        const hipError_t end_err = hipStreamEndCapture(p->capture_stream, &graph);
+        NW_HIP(launch_err);
+        NW_HIP(end_err);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t inst_err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        NW_HIP(inst_err);
+        if ((int)p->graphs.size() >= kStepGraphs) {  // drop the least recently used
+            size_t old = 0;
+            for (size_t i = 1; i < p->graphs.size(); ++i)
+                if (p->graphs[i].used < p->graphs[old].used) old = i;
+            (void)hipGraphExecDestroy(p->graphs[old].exec);
+            p->graphs.erase(p->graphs.begin() + (std::ptrdiff_t)old);
+        }
+        p->graphs.push_back(nw_propagator::StepGraph{std::move(key), exec, 0});
+        hit = &p->graphs.back();
+    }
+    hit->used = ++p->graph_clock;
+    if (prm.launch_mode == NW_LAUNCH_GRAPH_PREPARE) return NW_OK;
+    k_set_i64<<<1, 64, 0, s>>>(p->step_dev, prm.step0);
+    NW_HIP(hipGetLastError());
+    NW_HIP(hipGraphLaunch(hit->exec, s));
+    return NW_OK;
+}
+
 }  // namespace nw
 
 using namespace nw;
@@ -1203,32 +1270,50 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
                    p->max_cell_rows);
         const unsigned grid = (unsigned)p->num_cells;
         double* v64 = p->cheb_v64;
+        NW_REQUIRE(prm->launch_mode >= NW_LAUNCH_DIRECT && prm->launch_mode <= NW_LAUNCH_GRAPH_PREPARE,
+                   "launch_mode must be NW_LAUNCH_DIRECT, NW_LAUNCH_GRAPH or NW_LAUNCH_GRAPH_PREPARE");
         // one launch per half step, built for the store's product and the largest cell
-        auto launch = [&](auto gate, auto maxj, auto nt, auto kind) {
-            constexpr int G = decltype(gate)::value, MJ = decltype(maxj)::value;
-            constexpr int T = decltype(nt)::value, KD = decltype(kind)::value;
-            for (int st = 0; st < n_steps; ++st) {
-                k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, s>>>(a, v64, st, st == 0);
-                k_hh_half_cheb<G, 1, MJ, T, KD><<<grid, T, 0, s>>>(a, v64, st, 0);
+        auto issue = [&](const HHArgs& args, hipStream_t st_) {
+            auto launch = [&](auto gate, auto maxj, auto nt, auto kind) {
+                constexpr int G = decltype(gate)::value, MJ = decltype(maxj)::value;
+                constexpr int T = decltype(nt)::value, KD = decltype(kind)::value;
+                for (int st = 0; st < n_steps; ++st) {
+                    k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
+                    k_hh_half_cheb<G, 1, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, 0);
+                }
+            };
+            auto by_shape = [&](auto gate, auto kind) {
+                using I = std::integral_constant<int, 1>;
+                using I5 = std::integral_constant<int, kChebRows>;
+                using TS = std::integral_constant<int, kChebSmallBlock>;
+                using TB = std::integral_constant<int, kChebBlock>;
+                if (p->cheb_small) launch(gate, I{}, TS{}, kind);
+                else if (p->cheb_maxj == 1) launch(gate, I{}, TB{}, kind);
+                else launch(gate, I5{}, TB{}, kind);
+            };
+            auto by_kind = [&](auto gate) {
+                if (p->rational) by_shape(gate, std::integral_constant<int, kProdRational>{});
+                else by_shape(gate, std::integral_constant<int, kProdChebyshev>{});
+            };
+            if (prm->gate_mode == 1) by_kind(std::integral_constant<int, 1>{});
+            else by_kind(std::integral_constant<int, 0>{});
+        };
+        bool graph = prm->launch_mode != NW_LAUNCH_DIRECT;
+        if (graph) {
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            NW_HIP(hipStreamIsCapturing(s, &cap));
+            if (cap != hipStreamCaptureStatusNone) {
+                NW_REQUIRE(prm->launch_mode != NW_LAUNCH_GRAPH_PREPARE,
+                           "NW_LAUNCH_GRAPH_PREPARE on a stream that is being captured");
+                graph = false;
             }
-        };
-        auto by_shape = [&](auto gate, auto kind) {
-            using I = std::integral_constant<int, 1>;
-            using I5 = std::integral_constant<int, kChebRows>;
-            using TS = std::integral_constant<int, kChebSmallBlock>;
-            using TB = std::integral_constant<int, kChebBlock>;
-            if (p->cheb_small) launch(gate, I{}, TS{}, kind);
-            else if (p->cheb_maxj == 1) launch(gate, I{}, TB{}, kind);
-            else launch(gate, I5{}, TB{}, kind);
-        };
-        auto by_kind = [&](auto gate) {
-            if (p->rational) by_shape(gate, std::integral_constant<int, kProdRational>{});
-            else by_shape(gate, std::integral_constant<int, kProdChebyshev>{});
-        };
-        if (prm->gate_mode == 1) by_kind(std::integral_constant<int, 1>{});
-        else by_kind(std::integral_constant<int, 0>{});
-        NW_HIP(hipGetLastError());
-        return NW_OK;
+        }
+        if (!graph) {
+            issue(a, s);
+            NW_HIP(hipGetLastError());
+            return NW_OK;
+        }
+        return hh_step_graph(p, *prm, n_steps, a, issue, s);
     }
     // default: 512 threads for the symmetric format (two workgroups per CU for 5000-segment cells
     // under the LDS budget), 256 for the general one (flat from 256 to 1024 at 10M compartments)

@@ -698,6 +698,7 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     NW_HIP(hipMalloc(&p->cheb_off, sizeof(int64_t) * p->num_cells));
     NW_HIP(hipMalloc(&p->cheb_c, sizeof(double) * p->h_cheb_c.size()));
     NW_HIP(hipMalloc(&p->cheb_v64, sizeof(double) * N));
+    NW_HIP(hipMalloc(&p->step_dev, sizeof(int64_t)));
     NW_HIP(hipMemcpy(p->nb_idx, nb.data(), sizeof(int32_t) * 3 * N, hipMemcpyHostToDevice));
     NW_HIP(hipMemcpy(p->nb_coef, nc.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice));
     NW_HIP(hipMemcpy(p->ov_idx, ovi.data(), sizeof(int32_t) * ovi.size(), hipMemcpyHostToDevice));
@@ -871,6 +872,7 @@ static int build_rational(nw_propagator* p, const int64_t* cell_ptr, const int32
                      hipMemcpyHostToDevice));
     NW_HIP(hipMemcpy(p->cheb_c, p->h_cheb_c.data(), sizeof(double), hipMemcpyHostToDevice));
     NW_HIP(hipMalloc(&p->cheb_v64, sizeof(double) * std::max<int64_t>(N, 1)));
+    NW_HIP(hipMalloc(&p->step_dev, sizeof(int64_t)));
     NW_HIP(hipMalloc(&p->pre_pos, sizeof(uint16_t) * ppos.size()));
     NW_HIP(hipMalloc(&p->pre_base, sizeof(int64_t) * p->num_cells));
     NW_HIP(hipMalloc(&p->pre_size, sizeof(int32_t) * p->num_cells));
@@ -915,6 +917,11 @@ static int build_rational(nw_propagator* p, const int64_t* cell_ptr, const int32
 
 static void free_prop(nw_propagator* p) {
     if (!p) return;
+    for (auto& g : p->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    p->graphs.clear();
+    if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
+    if (p->step_dev) (void)hipFree(p->step_dev);
     if (p->nb_idx) (void)hipFree(p->nb_idx);
     if (p->nb_coef) (void)hipFree(p->nb_coef);
     if (p->ov_idx) (void)hipFree(p->ov_idx);

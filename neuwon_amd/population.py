@@ -290,20 +290,43 @@ class HHPopulation:
         else:
             p.stim_mode = STIM_NONE
 
-    def advance(self, n_steps=1, trace=None, stream=None, record=None):
-        """Advance every cell by n_steps time steps (one launch).  trace: optional float32 device
-        tensor [num_cells, >= n_steps] receiving the AP-segment voltage after each step.
-        record: optional float32 device tensor of (5 n_steps + 4) * num_segments values receiving
-        the state of every step for the reverse pass."""
+    def advance(self, n_steps=1, trace=None, stream=None, record=None, graph=True):
+        """Advance every cell by n_steps time steps.  trace: optional float32 device tensor
+        [num_cells, >= n_steps] receiving the AP-segment voltage after each step.  record:
+        optional float32 device tensor of (5 n_steps + 4) * num_segments values receiving the
+        state of every step for the reverse pass.
+        The stored formats run all steps in one launch.  The matrix-free stores run one launch per
+        half step; with graph=True (and n_steps >= 2) the 2 n_steps launches are replayed as one
+        HIP graph, captured the first time these arguments (step count, trace, record) are seen
+        and kept by the propagator (``prepare`` captures it ahead of time)."""
+        self._launch(n_steps, trace, stream, record,
+                     _lib.NW_LAUNCH_GRAPH if graph and n_steps >= 2 else _lib.NW_LAUNCH_DIRECT)
+
+    def prepare(self, n_steps, trace=None, record=None, stream=None):
+        """Capture the graph that advance(n_steps, trace, record) replays on a matrix-free store
+        (nothing runs, the state is untouched); returns the number of graphs captured or found
+        (0 on the stored formats, which need none)."""
+        if not self.propagator.chebyshev or n_steps < 2:
+            return 0
+        steps = self.steps
+        self._launch(n_steps, trace, stream, record, _lib.NW_LAUNCH_GRAPH_PREPARE)
+        self.steps = steps
+        return 1
+
+    def _launch(self, n_steps, trace, stream, record, mode):
         torch = self.torch
+        n_steps = int(n_steps)
+        if n_steps < 0:
+            raise ValueError(f"n_steps = {n_steps}: must be non-negative")
+        self._check_output(trace, "trace", (self.num_cells, n_steps))
+        self._check_output(record, "record", ((5 * n_steps + 4) * self.num_segments,))
         p = self.params
         p.step0 = self.steps
         p.conductances_valid = 1 if self.steps > 0 else 0
+        p.launch_mode = mode
         s = stream if stream is not None else torch.cuda.current_stream(self.dev)
         tr_ptr, tr_stride = None, 0
         if trace is not None:
-            assert trace.dtype == torch.float32 and trace.is_contiguous()
-            assert trace.shape[0] == self.num_cells and trace.shape[1] >= n_steps
             tr_ptr, tr_stride = _lib.ptr(trace), trace.shape[1]
         P = _lib.ptr
         with _lib.lock():
@@ -313,6 +336,25 @@ class HHPopulation:
                 P(self.inv_c), P(self.stim_local), P(self.ap_local), P(self.ap_state),
                 P(self.ap_count), tr_ptr, tr_stride, P(record), ctypes.c_void_p(s.cuda_stream)))
         self.steps += n_steps
+
+    def _check_output(self, t, name, shape):
+        """An output array the kernels write: a contiguous float32 tensor on this population's
+        device with at least ``shape`` (trace: [num_cells, >= n_steps]; record: a flat array of
+        >= (5 n_steps + 4) num_segments values).  Anything else would be written out of bounds or
+        through a host address, so it is refused before the launch."""
+        if t is None:
+            return
+        torch = self.torch
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{name} must be a torch tensor (got {type(t).__name__})")
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.device != self.dev:
+            raise ValueError(f"{name} must be a contiguous float32 tensor on {self.dev} (got "
+                             f"{t.dtype}, device {t.device}, contiguous {t.is_contiguous()})")
+        if len(shape) == 1:
+            if t.numel() < shape[0]:
+                raise ValueError(f"{name} holds {t.numel()} values, {shape[0]} needed")
+        elif t.dim() != 2 or t.shape[0] != shape[0] or t.shape[1] < shape[1]:
+            raise ValueError(f"{name} of shape {tuple(t.shape)}: [{shape[0]}, >= {shape[1]}] needed")
 
     def time(self):
         return self.steps * self.dt

@@ -250,3 +250,55 @@ def test_adjoint_whole_record_matches_checkpointed(torch_gpu):
         e = max(_rel(g_rec[k].cpu().numpy()[r0:r1], ref[k]) for k in ("na", "k", "l"))
         assert e < 1e-3
 
+
+
+def test_target_of_any_dtype_and_device(torch_gpu):
+    """A float64 target (e.g. torch.as_tensor of a numpy array), or one on the host, gives the
+    gradients of the float32 device target: the reverse kernel reads FP32 rows."""
+    from neuwon_amd import adjoint, synth
+    torch = torch_gpu
+    cells = synth.uniform_population(num_cells=3, n=60, seed=8)
+    prob = adjoint.HHAdjoint(cells, dt=0.05, stim_amp_na=0.1, checkpoint_every=50)
+    pop = prob.pop
+    T = 120
+    rng = np.random.default_rng(3)
+    tgt = rng.uniform(-70.0, -60.0, (pop.num_cells, T)).astype(np.float32)
+    args = (pop.g_na.clone(), pop.g_k.clone(), pop.g_l.clone(), np.ones(6))
+    loss32, g32 = prob.loss_and_grad(*args, torch.as_tensor(tgt, device="cuda"))
+    for other in (torch.as_tensor(tgt.astype(np.float64), device="cuda"),
+                  torch.as_tensor(tgt.astype(np.float64)), tgt.astype(np.float64)):
+        loss, g = prob.loss_and_grad(*args, other)
+        assert loss == pytest.approx(loss32, rel=1e-12)
+        assert prob.last_loss == pytest.approx(loss32, rel=1e-6)
+        for k in ("na", "k", "l"):
+            assert torch.equal(g[k], g32[k])
+        assert np.array_equal(g["theta"], g32["theta"])
+    with pytest.raises(ValueError):
+        prob.loss_and_grad(*args, torch.zeros((pop.num_cells + 1, T), device="cuda"))
+
+
+def test_population_refuses_bad_trace_and_record(torch_gpu):
+    """advance() refuses trace and record tensors the kernels would write out of bounds or
+    through a host address."""
+    from neuwon_amd import population, synth
+    torch = torch_gpu
+    cells = synth.uniform_population(num_cells=2, n=40, seed=1)
+    pop = population.HHPopulation(cells, store="chebyshev")
+    n = 5
+    bad_traces = [torch.zeros((2, n - 1), device="cuda"),
+                  torch.zeros((3, n), device="cuda"),
+                  torch.zeros((2, n), device="cuda", dtype=torch.float64),
+                  torch.zeros((2, n)),
+                  torch.zeros((n, 2), device="cuda").t()]
+    for t in bad_traces:
+        with pytest.raises(ValueError):
+            pop.advance(n, trace=t)
+    need = (5 * n + 4) * pop.num_segments
+    for r in (torch.zeros(need - 1, device="cuda"), torch.zeros(need),
+              torch.zeros(need, device="cuda", dtype=torch.float16)):
+        with pytest.raises(ValueError):
+            pop.advance(n, record=r)
+    assert pop.steps == 0
+    pop.advance(n, trace=torch.zeros((2, n), device="cuda"),
+                record=torch.zeros(need, device="cuda"))
+    assert pop.steps == n
