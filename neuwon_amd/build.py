@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libneuwon_amd.so")
-SOURCES = ["nw_propagator.hip", "nw_kernels.hip", "nw_clock.hip"]
+SOURCES = ["nw_propagator.hip", "nw_kernels.hip", "nw_clock.hip", "nw_kinetic.hip"]
 ARCH = os.environ.get("NEUWON_AMD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1384,6 +1384,14 @@ class RxD_Model:
             # inputs other than the voltage alone: the general table kernel, inputs described once
             # per build (the arrays do not move until the next build)
             mech._nd_inputs = None
+            mech._lti_table = None
+            if spec.input_names == ["v"]:
+                # the voltage alone: the bucket-grouped kernel (instances are in segment order, so
+                # neighbouring instances share voltage buckets)
+                from .kinetic import LTITable
+                g = spec.scheme.grid
+                mech._lti_table = LTITable(mech._node_table, g.minimum, g.maximum,
+                                           device=dev.index or 0)
             if spec.input_names != ["v"]:
                 arr = (_lib.NwLtiInput * len(spec.inputs))()
                 for k, (n, g) in enumerate(zip(spec.input_names, spec.inputs)):
@@ -1447,12 +1455,10 @@ class RxD_Model:
                 P(mech._node_table), M, P(mech._states), M, spec.scheme.conserve_sum or 0.0, 1,
                 P(mech._weights), P(mech._gmax_dev), P(out), self._stream()))
             return
-        grid = spec.scheme.grid
-        _lib.check(_lib.load().nw_lti_advance(
-            spec.scheme.num_states, grid.num_buckets + 1, grid.minimum, grid.maximum,
-            P(mech._node_table), M, P(self._col("Segment", "voltage")), P(mech._seg_dev),
+        _lib.check(_lib.load().nw_lti_table_advance(
+            mech._lti_table.handle, M, P(self._col("Segment", "voltage")), P(mech._seg_dev),
             P(mech._states), M, spec.scheme.conserve_sum or 0.0, 1, P(mech._weights),
-            P(mech._gmax_dev), P(out), self._stream()))
+            P(mech._gmax_dev), P(out), -1, self._stream()))
 
     def _propagator(self, time_step):
         from .population import Propagator

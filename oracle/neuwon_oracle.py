@@ -586,12 +586,38 @@ def _c_lib():
                     ("stim_mode", ctypes.c_int32), ("stim_amp", ctypes.c_double),
                     ("seed", ctypes.c_uint32), ("threshold", ctypes.c_uint32),
                     ("pulse_ticks", ctypes.c_int32), ("gate_mode", ctypes.c_int32),
-                    ("celsius", ctypes.c_double)]
+                    ("celsius", ctypes.c_double), ("fp32_state", ctypes.c_int32)]
     lib.oracle_hh_run.restype = ctypes.c_int
+    lib.oracle_lti_run.restype = ctypes.c_int
+    lib.oracle_lti_run.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int32, ctypes.c_double, ctypes.c_int32]
     lib.oracle_stim_hash.restype = ctypes.c_uint32
     lib.oracle_stim_hash.argtypes = [ctypes.c_uint32] * 3
     lib.Params = Params
     return lib
+
+
+def c_lti_run(states, v0, poly_table, wave=None, minimum=-100.0, maximum=100.0,
+              conserve_sum=None, threads=0):
+    """The C restatement of ``lti_advance_vec`` (lti_sim/codegen.py:39-106), OpenMP over instances:
+    states [S][M] FP64 advanced in place by len(wave) steps (or one step with wave None), step k at
+    the voltages v0 + wave[k]."""
+    import ctypes
+    lib = _c_lib()
+    states_ = np.ascontiguousarray(states, dtype=np.float64)
+    S, M = states_.shape
+    nb, _, _, terms = poly_table.shape
+    poly = np.ascontiguousarray(poly_table, dtype=np.float64)
+    v0_ = np.ascontiguousarray(v0, dtype=np.float64)
+    w = None if wave is None else np.ascontiguousarray(wave, dtype=np.float64)
+    n = 1 if w is None else len(w)
+    P = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    rc = lib.oracle_lti_run(S, nb, terms, minimum, maximum, P(poly), M, P(states_), P(v0_), P(w), n,
+                            float(conserve_sum or 0.0), int(threads))
+    assert rc == 0
+    return states_
 
 
 class CHHPopulation:
@@ -599,10 +625,15 @@ class CHHPopulation:
 
     cell_ptr: int64 [cells+1]; P: scipy CSR over all segments (block diagonal, global columns);
     C, g_bar (dict na/k/l): per segment; stim: None | ('constant', amp_A) |
-    ('poisson', amp_A, seed, threshold, pulse_ticks), applied to segment 0 of every cell."""
+    ('poisson', amp_A, seed, threshold, pulse_ticks), applied to segment 0 of every cell.
+
+    fp32_storage: an emulation of FP32 storage, to measure the error floor it sets (it is not
+    the reference's algorithm, which is FP64 throughout): the per-segment maximal conductances,
+    1/C and the gate tables' node values are rounded to FP32 once, every gate after each update;
+    round_v also rounds every voltage at the end of every step."""
 
     def __init__(self, cell_ptr, P, C, g_bar, v0=-70.0, dt=0.1, celsius=6.3, gate_mode="table",
-                 stim=None, E=None, tables=None, gates0=None):
+                 stim=None, E=None, tables=None, gates0=None, fp32_storage=False, round_v=False):
         import ctypes
         self.lib = _c_lib()
         self.cell_ptr = np.ascontiguousarray(cell_ptr, dtype=np.int64)
@@ -610,8 +641,11 @@ class CHHPopulation:
         self.rowptr = np.ascontiguousarray(P.indptr, dtype=np.int64)
         self.col = np.ascontiguousarray(P.indices, dtype=np.int32)
         self.val = np.ascontiguousarray(P.data, dtype=np.float64)
-        self.C = np.ascontiguousarray(C, dtype=np.float64)
-        self.g = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in g_bar.items()}
+        f32 = (lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)) if fp32_storage \
+            else (lambda a: np.asarray(a, dtype=np.float64))
+        C = np.asarray(C, dtype=np.float64)
+        self.C = np.ascontiguousarray(1.0 / f32(1.0 / C) if fp32_storage else C)
+        self.g = {k: np.ascontiguousarray(f32(v)) for k, v in g_bar.items()}
         N = len(self.C)
         self.v = np.full(N, float(v0))
         gates0 = gates0 or {g: float(gate_inf(v0, g, celsius)) for g in "mhn"}
@@ -631,12 +665,13 @@ class CHHPopulation:
             p.stim_mode, p.stim_amp = 1, stim[1]
         else:
             p.stim_mode, p.stim_amp, p.seed, p.threshold, p.pulse_ticks = 2, stim[1], stim[2], stim[3], stim[4]
+        p.fp32_state = (1 if fp32_storage else 0) | (2 if round_v else 0)
         self.p = p
         if tables is None:
             tables = {}
             for g in "mhn":
                 deriv = two_state_derivative(hh_gate_rate(g, celsius))
-                tables[g] = node_table_to_polynomial(lti_node_table(deriv, 2, dt))
+                tables[g] = node_table_to_polynomial(f32(lti_node_table(deriv, 2, dt)))
         self.poly = np.ascontiguousarray(np.stack([tables[g] for g in "mhn"]), dtype=np.float64)
 
     def advance(self, n_steps=1, threads=0):

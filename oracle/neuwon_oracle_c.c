@@ -30,6 +30,10 @@ typedef struct {
     int32_t pulse_ticks;
     int32_t gate_mode; /* 0 tables, 1 exact */
     double celsius;
+    /* FP32-storage emulation (not part of the reference's algorithm: it measures the error floor
+     * that storing the state in FP32 sets): bit 0 rounds every gate to FP32 after its update,
+     * bit 1 every voltage to FP32 at the end of every step */
+    int32_t fp32_state;
 } oracle_params;
 
 static uint32_t stim_hash(uint32_t seed, uint32_t cell, uint32_t tick) {
@@ -164,6 +168,7 @@ int oracle_hh_run(const oracle_params* p, int64_t num_cells, const int64_t* cell
                             st[gi] = lti_two_state(poly + (int64_t)g * p->num_buckets * 8, p, v[gi],
                                                    st[gi]);
                         }
+                        if (p->fp32_state & 1) st[gi] = (double)(float)st[gi];
                     }
                     const double mm = m[gi], hh = h[gi], nn = n[gi];
                     const double g_na = gna[gi] * mm * mm * mm * hh;
@@ -173,9 +178,59 @@ int oracle_hh_run(const oracle_params* p, int64_t num_cells, const int64_t* cell
                     d_num[gi] = g_na * p->e_na + g_k * p->e_k + g_l * p->e_l;
                 }
                 half_step(p, r0, cn, rowptr, col, val, C, v, sum_g, d_num, (uint32_t)c, tick + 1, x);
+                if (p->fp32_state & 2)
+                    for (int64_t i = 0; i < cn; ++i) v[r0 + i] = (double)(float)v[r0 + i];
             }
         }
         free(x);
+    }
+    return 0;
+}
+
+/* The generated LTI kernel (lti_sim/codegen.py:39-106) for M instances of an S-state scheme with
+ * one linear input: poly = [bucket][col][row][term] (num_buckets buckets over vmin..vmax), states
+ * [S][M] in place.  Step k reads the voltage v0[i] + wave[k] (wave may be NULL: held).  CONSERVE
+ * renormalisation when conserve > 0.  Instances are independent: OpenMP over instances. */
+int oracle_lti_run(int32_t S, int32_t num_buckets, int32_t terms, double vmin, double vmax,
+                   const double* poly, int64_t M, double* states, const double* v0,
+                   const double* wave, int32_t n_steps, double conserve, int32_t threads) {
+    if (S < 1 || S > 32 || terms < 1 || terms > 8) return -1;
+    const double frq = num_buckets / (vmax - vmin);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t i = 0; i < M; ++i) {
+        double s[32], scratch[32], basis[8];
+        for (int k = 0; k < S; ++k) s[k] = states[(int64_t)k * M + i];
+        for (int32_t step = 0; step < n_steps; ++step) {
+            const double v = v0[i] + (wave ? wave[step] : 0.0);
+            const double u = (v - vmin) * frq;
+            int64_t b = (int64_t)u; /* truncation toward zero */
+            double f;
+            if (b > num_buckets - 1) { b = num_buckets - 1; f = 1.0; }
+            else if (b < 0) { b = 0; f = 0.0; }
+            else f = u - (double)b;
+            basis[0] = 1.0;
+            for (int t = 1; t < terms; ++t) basis[t] = basis[t - 1] * f;
+            for (int r = 0; r < S; ++r) scratch[r] = 0.0;
+            const double* tb = poly + b * (int64_t)S * S * terms;
+            for (int col = 0; col < S; ++col)
+                for (int row = 0; row < S; ++row) {
+                    const double* c = tb + ((int64_t)col * S + row) * terms;
+                    double p = 0.0;
+                    for (int t = 0; t < terms; ++t) p += c[t] * basis[t];
+                    scratch[row] += p * s[col];
+                }
+            if (conserve > 0.0) {
+                double tot = 0.0;
+                for (int r = 0; r < S; ++r) tot += scratch[r];
+                const double corr = conserve / tot;
+                for (int r = 0; r < S; ++r) scratch[r] *= corr;
+            }
+            for (int r = 0; r < S; ++r) s[r] = scratch[r];
+        }
+        for (int k = 0; k < S; ++k) states[(int64_t)k * M + i] = s[k];
     }
     return 0;
 }

@@ -287,6 +287,21 @@ def test_lti_polynomial_table_is_linear_interpolation():
     vn = np.array([-100 + 37 * 1.0])
     exact = O.lti_matrix(deriv, [vn[0]], 3, 0.1)
     assert np.abs(O.lti_advance_vec(st[:, :1], vn, poly) [:, 0] - exact @ st[:, 0]).max() < 1e-14
+    # the C restatement (the kinetic workload's CPU baseline): the same arithmetic, one step
+    # held and several steps at moving voltages, inputs beyond the table clamped
+    v2 = np.concatenate([v, [-150.0, 150.0, -100.0, 100.0]])
+    st2 = rng.dirichlet(np.ones(3), len(v2)).T
+    assert np.abs(O.c_lti_run(st2, v2, poly, conserve_sum=1.0)
+                  - O.lti_advance_vec(st2, v2, poly, conserve_sum=1.0)).max() < 1e-15
+    wave = np.array([0.0, 3.7, -12.5, 40.0])
+    ref = st2.copy()
+    for w in wave:
+        ref = O.lti_advance_vec(ref, v2 + w, poly, conserve_sum=1.0)
+    got = O.c_lti_run(st2, v2, poly, wave=wave, conserve_sum=1.0, threads=3)
+    assert np.abs(got - ref).max() < 1e-14
+    # a planted slip (the previous bucket) is seen
+    bad = O.lti_advance_vec(st2, v2 - 1.0, poly, conserve_sum=1.0)
+    assert np.abs(O.c_lti_run(st2, v2, poly, conserve_sum=1.0) - bad).max() > 1e-6
 
 
 def test_golden_hh_trace():
