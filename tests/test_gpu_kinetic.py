@@ -65,18 +65,50 @@ def test_markov13_1M(torch_gpu):
     got = held.cpu().numpy().astype(np.float64)
     assert np.abs(got.sum(axis=0) - 1.0).max() < 1e-5
     # parity on a sample against the reference kernel's FP64 arithmetic (polynomial table form)
-    poly = O.node_table_to_polynomial(node)
+    # on the same FP32 table the GPU reads
+    node32 = node.astype(np.float32).astype(np.float64)
+    poly = O.node_table_to_polynomial(node32)
     sample = rng.choice(M, 2000, replace=False)
     ref = st0[:, sample].astype(np.float64)
     vv = v[sample].astype(np.float64)
     for _ in range(steps):
         ref = O.lti_advance_vec(ref, vv, poly, conserve_sum=1.0)
-    err = np.abs(got[:, sample] - ref).max()
-    _say(f"max abs state error vs FP64 oracle after {steps} steps: {err:.2e}")
-    assert err < 1.2e-5
-    # the one-step-per-launch path is the same arithmetic
+    # what the held kernel rounds: each instance's interpolated propagator is formed once in FP32
+    # (registers), the states stay FP64 over the launch; its distance from the FP64 oracle is the
+    # floor of this form
+    u = (vv + 100.0) * 10.0
+    b = np.clip(np.trunc(u).astype(np.int64), 0, 1999)
+    f = np.where(np.trunc(u) > 1999, 1.0, u - b)
+    Pm = (node32[b] + f[:, None, None] * (node32[b + 1] - node32[b])).astype(np.float32)
+    emu = st0[:, sample].astype(np.float64).T[:, :, None]
+    Pm = Pm.astype(np.float64)
+    for _ in range(steps):
+        emu = Pm @ emu
+        emu /= emu.sum(axis=1, keepdims=True)
+    emu = emu[:, :, 0].T
+    big = ref >= 1e-6
+    d = np.abs(got[:, sample] - ref)
+    rel = (d / np.maximum(ref, 1e-30))[big].max()
+    floor = (np.abs(emu - ref) / np.maximum(ref, 1e-30))[big].max()
+    vs_emu = (np.abs(got[:, sample] - emu) / np.maximum(emu, 1e-30))[big].max()
+    _say(f"after {steps} held steps: max abs state error vs FP64 oracle {d.max():.2e}; per-state "
+         f"relative {rel:.2e} where the occupancy >= 1e-6 (the held form's FP32-matrix floor "
+         f"{floor:.2e}); against an emulation of the held form {vs_emu:.2e}; smallest occupancy "
+         f"{ref.min():.1e}")
+    assert rel < max(1e-5, 3.0 * floor)
+    assert d.max() < max(1e-6, 3.0 * np.abs(emu - ref).max())
+    assert vs_emu < 1e-6
+    # the one-step-per-launch path: FP64 interpolation each step, states rounded to FP32 between
+    # launches (its floor: the oracle with the same rounding)
     got1 = stepwise.cpu().numpy().astype(np.float64)
     ref1 = st0[:, sample].astype(np.float64)
+    emu1 = ref1.copy()
     for _ in range(100):
         ref1 = O.lti_advance_vec(ref1, vv, poly, conserve_sum=1.0)
-    assert np.abs(got1[:, sample] - ref1).max() < 2e-5
+        emu1 = O.lti_advance_vec(emu1, vv, poly, conserve_sum=1.0).astype(np.float32).astype(np.float64)
+    big1 = ref1 >= 1e-6
+    rel1 = (np.abs(got1[:, sample] - ref1) / np.maximum(ref1, 1e-30))[big1].max()
+    floor1 = (np.abs(emu1 - ref1) / np.maximum(ref1, 1e-30))[big1].max()
+    _say(f"100 steps one per launch: per-state relative {rel1:.2e} (FP32-state floor {floor1:.2e})")
+    assert rel1 < max(1e-5, 3.0 * floor1)
+    assert np.abs(got1[:, sample] - ref1).max() < max(1e-6, 3.0 * np.abs(emu1 - ref1).max())

@@ -76,14 +76,17 @@ def test_stored_fp32_propagator(torch_gpu):
         assert np.abs(rs32 - rs64).max() < 1e-7
 
 
-def voltage_error(v, ref, dt, window_ms=2.0, threshold=20.0):
-    """Errors of a voltage trace against the FP64 reference, split into subthreshold samples (more
-    than window_ms from every spike of the reference) and samples within window_ms of a spike."""
+def voltage_error(v, ref, dt, window_ms=2.0, threshold=20.0, excursion=-20.0):
+    """Errors of a voltage trace against the FP64 reference, split into subthreshold samples
+    (relative error) and samples near an excursion (absolute error, mV): within window_ms of any
+    sample where the reference is at or above `excursion` mV -- every action potential and every
+    depolarisation that approaches one, where the voltage crosses 0 mV and a relative error is
+    not defined.  Spikes: upward crossings of `threshold` (the AP detector's 20 mV)."""
     v, ref = np.asarray(v, np.float64), np.asarray(ref, np.float64)
     up = np.nonzero((ref[1:] >= threshold) & (ref[:-1] < threshold))[0] + 1
     t = np.arange(len(ref)) * dt
     near = np.zeros(len(ref), dtype=bool)
-    for k in up:
+    for k in np.nonzero(ref >= excursion)[0]:
         near |= np.abs(t - t[k]) <= window_ms
     d = np.abs(v - ref)
     rel = d / np.abs(ref)
