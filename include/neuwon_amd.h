@@ -204,6 +204,17 @@ int nw_lti_advance(int32_t S, int32_t num_nodes, float vmin, float vmax, const f
  * ------------------------------------------------------------------------------------------ */
 typedef struct nw_lti_table nw_lti_table;
 
+/* Table build of lti_sim/lti_model.py:51-65 on the device: P[m] = expm(A[m] * time_step) for
+ * `count` S x S FP64 rate matrices (device, row-major, S <= 16), every column of P[m] divided by
+ * its sum when normalise_columns (the reference normalises, lti_model.py:63-64).  One wavefront
+ * per matrix on the FP64 matrix cores: X = A dt + lambda I (lambda = max |A_ii dt|, non-negative
+ * for a rate matrix), scaled by 2^-s to a 1-norm <= 1/2, an 18-term Taylor series, s squarings of
+ * e^(-lambda 2^-s) expm(X 2^-s); with normalise_columns and columns of A that sum to zero (to
+ * their rounding), every squaring's columns are rescaled to sum one (the propagators of a
+ * conserving scheme are column-stochastic at every stage). */
+int nw_lti_expm_batched(int64_t count, int32_t S, const double* A, double time_step,
+                        int32_t normalise_columns, double* P, void* stream);
+
 int nw_lti_table_create(int device, int32_t S, int32_t num_nodes, float vmin, float vmax,
                         const float* node_table, nw_lti_table** out);
 int nw_lti_table_destroy(nw_lti_table* t);

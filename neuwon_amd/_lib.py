@@ -92,6 +92,7 @@ SIGNATURES = {
                                   c_vp, c_vp, c_vp, c_vp]),
     "nw_lti_table_create": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_vp, ctypes.POINTER(c_vp)]),
     "nw_lti_table_destroy": (c_i32, [c_vp]),
+    "nw_lti_expm_batched": (c_i32, [c_i64, c_i32, c_vp, c_f64, c_i32, c_vp, c_vp]),
     "nw_lti_table_advance": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_f32, c_i32, c_vp,
                                      c_vp, c_vp, c_i32, c_vp]),
     "nw_hh_step": (c_i32, [c_vp, ctypes.POINTER(NwHHParams), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,

@@ -215,11 +215,16 @@ class LTIScheme:
         P /= P.sum(axis=0, keepdims=True)
         return P
 
-    def node_table(self):
-        """FP64 [node_0](x [node_1])[row][col] propagators on the grid nodes (input 0 slowest)."""
+    def node_rates(self):
+        """Rate matrices [node_0](x [node_1])[S][S] at the grid nodes (input 0 slowest)."""
         grids = np.meshgrid(*[g.nodes() for g in self.inputs], indexing="ij")
-        A = self._rates(*grids)
-        P = scipy.linalg.expm(A * self.time_step)
+        return self._rates(*grids)
+
+    def node_table(self):
+        """FP64 [node_0](x [node_1])[row][col] propagators on the grid nodes (input 0 slowest),
+        on the host (scipy.linalg.expm); ``kinetic.device_node_table`` builds the same table on
+        the device."""
+        P = scipy.linalg.expm(self.node_rates() * self.time_step)
         P /= P.sum(axis=-2, keepdims=True)
         return P
 

@@ -8,6 +8,8 @@ wavefront and bucket instead of once per instance.
 """
 import ctypes
 
+import numpy as np
+
 from . import _lib
 
 DEFAULT_GROUP_MAX = -1   # the library's default number of distinct buckets per grouped wavefront
@@ -57,3 +59,31 @@ class LTITable:
             self.close()
         except Exception:
             pass
+
+
+def expm_batched(A, time_step, normalise_columns=True, stream=None):
+    """expm(A time_step) of every matrix of a float64 device tensor [..., S, S] (S <= 16) on the
+    device (nw_lti_expm_batched), every column divided by its sum when normalise_columns (the
+    table build of lti_sim/lti_model.py:51-65)."""
+    import torch
+    S = int(A.shape[-1])
+    if A.dim() < 2 or A.shape[-2] != S or A.dtype != torch.float64 or not A.is_cuda:
+        raise ValueError(f"expm_batched: float64 device tensor [..., S, S] expected (got "
+                         f"{tuple(A.shape)} {A.dtype} on {A.device})")
+    A = A.contiguous()
+    P = torch.empty_like(A)
+    count = A.numel() // (S * S)
+    s = stream if stream is not None else torch.cuda.current_stream(A.device)
+    _lib.check(_lib.load().nw_lti_expm_batched(count, S, _lib.ptr(A), float(time_step),
+                                               1 if normalise_columns else 0, _lib.ptr(P),
+                                               ctypes.c_void_p(s.cuda_stream)))
+    return P
+
+
+def device_node_table(scheme, device=0):
+    """The scheme's node table [nodes...][S][S] (float64, on the device): its rate matrices at the
+    grid nodes (host, from the scheme's kinetic equations) exponentiated on the device."""
+    import torch
+    A = torch.as_tensor(np.ascontiguousarray(scheme.node_rates(), dtype=np.float64),
+                        device=torch.device("cuda", device))
+    return expm_batched(A, scheme.time_step, normalise_columns=True)

@@ -301,3 +301,35 @@ def test_own_receptor_files():
     mech = nmodl.kinetic_mechanism(os.path.join(OWN, "glu5.mod"), 0.025, 30.0, {})
     assert mech.point_process and mech.input_names == ["gluo"] and mech.reversal == 0.0
     assert mech.scheme.weights[idx["O"]] == pytest.approx(2e-9, rel=1e-14)            # Siemens
+
+
+def _kinetic_fixture(name):
+    import json
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "kinetic_tables", f"{name}.npz"),
+                allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    S = meta["num_states"]
+    shape = tuple(meta["node_shape"])
+    A = np.zeros((int(np.prod(shape)), S, S))
+    A[:, z["pattern"][:, 0], z["pattern"][:, 1]] = z["values"]
+    return A.reshape(shape + (S, S)), meta
+
+
+@pytest.mark.parametrize("name", ["Nav11", "ampa13"])
+def test_kinetic_table_fixtures(name):
+    """The rate matrices the GPU tests tabulate: rate matrices (non-negative transitions, columns
+    summing to zero), the grid of the front end, and -- where the reference is on this machine --
+    equal to a fresh parse of its file."""
+    A, meta = _kinetic_fixture(name)
+    S = meta["num_states"]
+    off = ~np.eye(S, dtype=bool)
+    assert (A[..., off] >= 0.0).all()
+    assert np.abs(A.sum(axis=-2)).max() <= 1e-9 * np.abs(A).max()
+    g = meta["grids"][0]
+    assert A.shape[:-2] == (g["num_buckets"] + 1,)
+    assert meta["conserve_sum"] == 1.0
+    path = os.path.join(REF, meta["source"])
+    if os.path.exists(path):
+        sc = nmodl.load_kinetic(path, time_step=meta["time_step"], celsius=meta["celsius"])
+        assert sc.state_names == meta["states"] and sc.input_names == meta["inputs"]
+        np.testing.assert_array_equal(sc.node_rates(), A)
