@@ -2,14 +2,15 @@
 // lti_sim/codegen.py:39-106 for schemes with one linear input, the membrane voltage).
 //
 // The reference kernel interpolates each instance's propagator between the two table nodes of its
-// bucket and multiplies it into the instance's state.  Every instance of a wave that shares a
-// bucket needs the same two node matrices, so the kernel reads them once per wave and bucket:
+// bucket and multiplies it into the instance's state.  Every instance of a wavefront that shares
+// a bucket needs the same two node matrices, so the kernel reads them once per wavefront and
+// bucket:
 //   x' = y0 + f (y1 - y0),  y0 = T_b x,  y1 = T_(b+1) x      (codegen.py:81-94, linear terms)
-// with T_b read by scalar loads (wave-uniform addresses: SGPR operands of the FP64 FMAs, no
-// per-lane table traffic).  A wave whose lanes fall into more than `group_max` distinct buckets
-// gathers each lane's node matrices itself (16-byte loads from the padded FP32 copy).  Both paths
-// do the same FP64 operations in the same order on the same values, so an instance's result does
-// not depend on the path its wave took.
+// as FP64 FMAs with the node entries in SGPRs (scalar loads: no per-lane table traffic), each
+// entry shared by the several instances a lane holds.  A wavefront whose instances fall into more
+// than `group_max` distinct buckets gathers each instance's node matrices itself (16-byte loads
+// from the padded FP32 copy).  Both forms do the same FP64 operations in the same order, so their
+// results are bitwise equal.
 #include <algorithm>
 #include <cstring>
 
@@ -44,6 +45,7 @@ __global__ void k_lti_table_layouts(int64_t num_nodes, int S, int pad, const flo
 }
 
 using ConstF64 = const __attribute__((address_space(4))) double*;
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 struct LtiGroupArgs {
     int64_t M;
@@ -61,129 +63,295 @@ struct LtiGroupArgs {
     const float* weights;
     const float* gmax;
     float* cond_out;
-    int64_t tiles_per_group;
 };
 
-// Waves whose lanes fall into at most this many distinct buckets take the grouped path.
+// Wavefronts whose instances fall into at most this many distinct buckets take the grouped path.
 constexpr int kLtiGroupMax = 2;
 
-// out = y0 + f (y1 - y0), y0 = T0 x, y1 = T1 x with T0, T1 the node matrices at `t0` (T1 at
-// t0 + S*S): entries in row-major order, every row's two sums over the columns in ascending
-// order (the sums of the reference kernel, codegen.py:81-94), in FP64, the row finished as soon
-// as its sums are.
-template <int S, typename P>
-__device__ __forceinline__ void interp_product(P t0, const double (&x)[S], double f,
-                                               double (&out)[S]) {
+// Instances per lane in k_lti_grouped: every node entry a grouped wavefront reads by a scalar load
+// feeds this many FMAs per lane (the load's latency spread over more work), within the register
+// budget of the states held per instance (two FP64 vectors of S).
+template <int S>
+constexpr int lti_instances_per_lane() { return 1; }
+
+// out[p] = y0 + f[p] (y1 - y0), y0 = T0 x[p], y1 = T1 x[p] for the P instances of the lane, T0
+// and T1 the node matrices at `t0` (T1 at t0 + S*S): entries in row-major order, every row's two
+// sums over the columns in ascending order (the sums of the reference kernel, codegen.py:81-94),
+// in FP64, each entry used for all P instances as it arrives.
+template <int S, int P, typename Ptr>
+__device__ __forceinline__ void interp_products(Ptr t0, const double (&x)[P][S], const double (&f)[P],
+                                                double (&out)[P][S]) {
 #pragma unroll
     for (int r = 0; r < S; ++r) {
-        double y0 = 0.0, y1 = 0.0;
+        double y0[P], y1[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) y0[p] = y1[p] = 0.0;
 #pragma unroll
         for (int c = 0; c < S; ++c) {
-            y0 = fma((double)t0[r * S + c], x[c], y0);
-            y1 = fma((double)t0[S * S + r * S + c], x[c], y1);
+            const double a0 = (double)t0[r * S + c], a1 = (double)t0[S * S + r * S + c];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                y0[p] = fma(a0, x[p][c], y0[p]);
+                y1[p] = fma(a1, x[p][c], y1[p]);
+            }
         }
-        out[r] = fma(f, y1 - y0, y0);
+#pragma unroll
+        for (int p = 0; p < P; ++p) out[p][r] = fma(f[p], y1[p] - y0[p], y0[p]);
     }
 }
 
-// One instance per thread, tiles of 256 consecutive instances; workgroup w walks tiles
-// w * tiles_per_group ... (w + 1) * tiles_per_group - 1 in order, so the node matrices its
-// wavefronts read by scalar loads stay in the scalar cache from one tile to the next (instances
-// in segment order: neighbouring tiles sit on neighbouring voltages); the next tile's voltages
-// and states are loaded while the current one is computed.
+// The gather form: the lane's own node matrices for one instance, 16-byte loads of the padded
+// FP32 nodes consumed in the row-major order of interp_products (the same operations in the same
+// order, so both forms give bitwise equal results).
 template <int S>
-__global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
-    const int nb = g.num_nodes - 1;
-    const int64_t tiles = (g.M + 255) / 256;
-    int64_t tile = (int64_t)blockIdx.x * g.tiles_per_group;
-    const int64_t tile_end = tile + g.tiles_per_group < tiles ? tile + g.tiles_per_group : tiles;
-    // the loads of tile `t` into v_n, x_n (FP32 as stored)
-    float v_n = 0.0f, x_n[S];
-    auto load = [&](int64_t t) {
-        const int64_t i = t * 256 + threadIdx.x;
-        if (t < tile_end && i < g.M) {
-            v_n = g.voltage[g.seg_index ? (int64_t)g.seg_index[i] : i];
+__device__ __forceinline__ void gather_product(const LtiGroupArgs& g, int b, double f,
+                                               const double (&x)[S], double (&out)[S]) {
+    const float4* t4 = reinterpret_cast<const float4*>(g.f32_pad + (int64_t)b * g.pad);
+    const int q1 = g.pad / 4;
+    double y0 = 0.0, y1 = 0.0;
 #pragma unroll
-            for (int k = 0; k < S; ++k) x_n[k] = g.states[k * g.stride + i];
+    for (int q = 0; q < (S * S + 3) / 4; ++q) {
+        const float4 a0 = t4[q], a1 = t4[q1 + q];
+        const float e0[4] = {a0.x, a0.y, a0.z, a0.w}, e1[4] = {a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int e = 4 * q + jj;
+            if (e < S * S) {
+                y0 = fma((double)e0[jj], x[e % S], y0);
+                y1 = fma((double)e1[jj], x[e % S], y1);
+                if (e % S == S - 1) {
+                    out[e / S] = fma(f, y1 - y0, y0);
+                    y0 = y1 = 0.0;
+                }
+            }
+        }
+    }
+}
+
+// Every 64-byte line of the two node matrices at t0 requested at once before a pass reads them
+// (one scalar-cache miss latency for the pass instead of one per dependent scalar load): lines
+// from t0 rounded down to 64 bytes, in groups of 8 (the FP64 table carries 8 KB of padding for
+// the last groups).  The loads complete inside the asm statement.
+#define NW_TL(off) "s_load_dword %0, %1, " off "\n"
+#define NW_T8(a, b) NW_TL(a "00") NW_TL(a "40") NW_TL(a "80") NW_TL(a "c0") \
+                    NW_TL(b "00") NW_TL(b "40") NW_TL(b "80") NW_TL(b "c0")
+#define NW_TG1 NW_T8("0x0", "0x1")
+#define NW_TG2 NW_TG1 NW_T8("0x2", "0x3")
+#define NW_TG3 NW_TG2 NW_T8("0x4", "0x5")
+#define NW_TG4 NW_TG3 NW_T8("0x6", "0x7")
+#define NW_TG5 NW_TG4 NW_T8("0x8", "0x9")
+#define NW_TG6 NW_TG5 NW_T8("0xa", "0xb")
+#define NW_TG7 NW_TG6 NW_T8("0xc", "0xd")
+#define NW_TG8 NW_TG7 NW_T8("0xe", "0xf")
+#define NW_TG9 NW_TG8 NW_T8("0x10", "0x11")
+#define NW_TOUCH(lines) asm volatile(lines "s_waitcnt lgkmcnt(0)" : "=&s"(d) : "s"(a) : "memory")
+
+template <int S>
+__device__ __forceinline__ void touch_node_pair(ConstF64 t0) {
+    constexpr int kLines = (2 * S * S * 8 + 63) / 64 + 1;
+    constexpr int kGroups = (kLines + 7) / 8;
+    static_assert(kGroups <= 9, "node pair beyond the touch groups");
+    const ConstF64 a = (ConstF64)((uintptr_t)t0 & ~(uintptr_t)63);
+    int d;
+    if constexpr (kGroups == 1) NW_TOUCH(NW_TG1);
+    else if constexpr (kGroups == 2) NW_TOUCH(NW_TG2);
+    else if constexpr (kGroups == 3) NW_TOUCH(NW_TG3);
+    else if constexpr (kGroups == 4) NW_TOUCH(NW_TG4);
+    else if constexpr (kGroups == 5) NW_TOUCH(NW_TG5);
+    else if constexpr (kGroups == 6) NW_TOUCH(NW_TG6);
+    else if constexpr (kGroups == 7) NW_TOUCH(NW_TG7);
+    else if constexpr (kGroups == 8) NW_TOUCH(NW_TG8);
+    else NW_TOUCH(NW_TG9);
+    asm volatile("" ::"s"(d));
+}
+#undef NW_TOUCH
+#undef NW_TG9
+#undef NW_TG8
+#undef NW_TG7
+#undef NW_TG6
+#undef NW_TG5
+#undef NW_TG4
+#undef NW_TG3
+#undef NW_TG2
+#undef NW_TG1
+#undef NW_T8
+#undef NW_TL
+// Schemes from this many states touch their node pair before a pass (smaller pairs span few
+// lines, which the pass's own loads fetch about as fast).
+constexpr int kTouchMinStates = 6;
+
+// A wavefront-uniform value made opaque in SGPRs (the row stride: recomputed row bases then stay
+// out of the SGPRs the node entries need), and a wavefront-uniform pointer likewise: an access
+// p[o] with a 32-bit lane offset then takes the SGPR-base form (one VGPR per address instead of a
+// 64-bit VGPR pair per array row).
+template <typename T>
+__device__ __forceinline__ T opaque(T v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* sbase(T* p) {
+    auto q = (__attribute__((address_space(1))) T*)p;
+    asm volatile("" : "+s"(q));
+    return q;
+}
+
+// CONSERVE (codegen.py:95-102), the states and the conductance of instance base + o from out
+// (every address a wavefront-uniform base plus the lane's 32-bit offset).
+template <int S>
+__device__ __forceinline__ void finish_lane(const LtiGroupArgs& g, int64_t base, uint32_t o,
+                                            double (&out)[S]) {
+    if (g.conserve_sum > 0.0f) {
+        double tot = 0.0;
+#pragma unroll
+        for (int r = 0; r < S; ++r) tot += out[r];
+        const double corr = (double)g.conserve_sum / tot;
+#pragma unroll
+        for (int r = 0; r < S; ++r) out[r] *= corr;
+    }
+    const int64_t stride = opaque(g.stride);
+#pragma unroll
+    for (int k = 0; k < S; ++k) sbase(g.states + k * stride + base)[o] = (float)out[k];
+    if (g.cond_out) {
+        double w = 0.0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) w = fma((double)g.weights[k], (double)(float)out[k], w);
+        sbase(g.cond_out + base)[o] = (float)((double)sbase(g.gmax + base)[o] * w);
+    }
+}
+
+// The bucket of the first lane left in rem (instance sets in order), or -1 when all are empty.
+// rem holds ballots, so the scan is uniform over the wavefront.
+template <int P>
+__device__ __forceinline__ int next_bucket(const uint64_t (&rem)[P], const int (&b)[P]) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+        if (rem[p]) return __builtin_amdgcn_readlane(b[p], __builtin_ctzll(rem[p]));
+    return -1;
+}
+
+// Distinct buckets among the wavefront's active instances, counted up to cap + 1.
+template <int P>
+__device__ __forceinline__ int distinct_buckets(const uint64_t (&lanes)[P], const int (&b)[P],
+                                                int cap) {
+    uint64_t scan[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) scan[p] = lanes[p];
+    int distinct = 0;
+    for (; distinct <= cap; ++distinct) {
+        const int u = next_bucket<P>(scan, b);
+        if (u < 0) break;
+#pragma unroll
+        for (int p = 0; p < P; ++p) scan[p] &= ~__ballot(b[p] == u);
+    }
+    return distinct;
+}
+
+// Tiles of `tile` consecutive instances; the G workgroups take balanced runs of consecutive tiles,
+// so a workgroup's wavefronts meet the same few buckets from one tile to the next (instances in
+// segment order: neighbouring tiles sit on neighbouring voltages).  Workgroups are dispatched to
+// the device's 8 XCDs round robin (workgroup i on XCD i mod 8); run (i mod 8) G/8 + i/8 gives
+// each XCD one contiguous stretch of instances, whose node matrices then stay in that XCD's L2.
+__device__ __forceinline__ void tile_range(const LtiGroupArgs& g, int tile, int64_t& begin,
+                                           int64_t& end) {
+    const int64_t tiles = (g.M + tile - 1) / tile, G = gridDim.x, i = blockIdx.x;
+    const int64_t r = G % 8 == 0 ? (i % 8) * (G / 8) + i / 8 : i;
+    begin = r * tiles / G;
+    end = (r + 1) * tiles / G;
+}
+
+// One wavefront per 64 P consecutive instances of a tile (lane l holds instances l, l + 64, ...,
+// so every load and store is 64 consecutive words); loads run one tile ahead, every one
+// unconditional (past the end of the run or of the instances they repeat the last valid
+// address), so they stay in flight across the current tile's arithmetic.  A wavefront of at most
+// group_max distinct buckets makes one pass per bucket: the bucket's two node matrices by scalar
+// loads (FP64 FMAs with SGPR operands, shared by the lane's P instances), the instances of the
+// bucket taking the result; other wavefronts gather per instance.
+template <int S, int P>
+__global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
+    constexpr int kTile = 256 * P;
+    const int nb = g.num_nodes - 1;
+    int64_t tile, tile_end;
+    tile_range(g, kTile, tile, tile_end);
+    if (tile >= tile_end) return;
+    const uint32_t lane0 = (threadIdx.x >> 6) * 64 * P + (threadIdx.x & 63);
+    float v_n[P], x_n[P][S];
+    auto load = [&](int64_t t) {
+        const int64_t stride = opaque(g.stride);
+        const int64_t base = (t < tile_end ? t : tile_end - 1) * kTile;
+        const uint32_t last = (uint32_t)std::min<int64_t>(g.M - 1 - base, kTile - 1);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const uint32_t o = std::min<uint32_t>(lane0 + 64 * p, last);
+            v_n[p] = g.seg_index ? g.voltage[(uint32_t)sbase(g.seg_index + base)[o]]
+                                : sbase(g.voltage + base)[o];
+#pragma unroll
+            for (int k = 0; k < S; ++k) x_n[p][k] = sbase(g.states + k * stride + base)[o];
         }
     };
     load(tile);
     for (; tile < tile_end; ++tile) {
-        const int64_t i = tile * 256 + threadIdx.x;
-        const bool active = i < g.M;
-        int b = -1;
-        double f = 0.0;
-        double x[S];
+        const int64_t base = tile * kTile;
+        const int64_t left = g.M - base;
+        bool active[P];
+        int b[P];
+        double f[P], x[P][S];
+        uint64_t lanes[P];
 #pragma unroll
-        for (int k = 0; k < S; ++k) x[k] = active ? (double)x_n[k] : 0.0;
-        if (active) bucket_of<double>((double)v_n, g.vmin, g.bucket_frq, nb, b, f);
-        load(tile + 1);
-        // distinct buckets among the wave's active lanes, counted up to group_max + 1
-        const uint64_t lanes = __ballot(active);
-        int distinct = 0;
-        for (uint64_t scan = lanes; scan && distinct <= g.group_max; ++distinct) {
-            const int lead = __builtin_ctzll(scan);
-            const int u = __builtin_amdgcn_readlane(b, lead);
-            scan &= ~__ballot(b == u);
+        for (int p = 0; p < P; ++p) {
+            active[p] = (int64_t)(lane0 + 64 * p) < left;
+            b[p] = -1;
+            f[p] = 0.0;
+#pragma unroll
+            for (int k = 0; k < S; ++k) x[p][k] = (double)x_n[p][k];
+            if (active[p]) bucket_of<double>((double)v_n[p], g.vmin, g.bucket_frq, nb, b[p], f[p]);
+            lanes[p] = __ballot(active[p]);
         }
-        double out[S];
-        if (distinct <= g.group_max) {
-            // one pass per distinct bucket: its two node matrices by scalar loads, the lanes of
-            // the bucket active
-            for (uint64_t rem = lanes; rem;) {
-                const int lead = __builtin_ctzll(rem);
-                const int u = __builtin_amdgcn_readlane(b, lead);
-                const bool mine = b == u;
-                rem &= ~__ballot(mine);
-                // the address from a copy of u the compiler cannot equate with the lane's own b
-                // (inside the branch they are equal, and an address in b would be per lane)
+        load(tile + 1);
+        if (distinct_buckets<P>(lanes, b, g.group_max) <= g.group_max) {
+            uint64_t rem[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) rem[p] = lanes[p];
+            for (;;) {
+                const int u = next_bucket<P>(rem, b);
+                if (u < 0) break;
+                bool mine[P], any = false;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    mine[p] = b[p] == u;
+                    rem[p] &= ~__ballot(mine[p]);
+                    any = any || mine[p];
+                }
+                // the address from a copy of u the compiler cannot equate with the lanes' own b
+                // (where they are equal, an address in b would be per lane)
                 int ua = u;
                 asm volatile("" : "+s"(ua));
-                if (mine) {
-                    ConstF64 t0 = (ConstF64)(g.f64 + (int64_t)ua * S * S);
-                    interp_product<S>(t0, x, f, out);
+                if (any) {
+                    // every instance lies in one bucket: it is finished in that bucket's pass
+                    double out[P][S];
+                    const ConstF64 t0 = (ConstF64)(g.f64 + (int64_t)ua * S * S);
+                    if constexpr (S >= kTouchMinStates) touch_node_pair<S>(t0);
+                    interp_products<S, P>(t0, x, f, out);
+                    // the products complete here: sunk into the per-instance branches below, they
+                    // would hold every node entry in SGPRs across the first branch
+#pragma unroll
+                    for (int p = 0; p < P; ++p)
+#pragma unroll
+                        for (int k = 0; k < S; ++k) asm volatile("" : "+v"(out[p][k]));
+#pragma unroll
+                    for (int p = 0; p < P; ++p)
+                        if (mine[p]) finish_lane<S>(g, base, lane0 + 64 * p, out[p]);
                 }
             }
-        } else if (active) {
-            // every lane its own node matrices: 16-byte loads of the padded FP32 nodes, consumed
-            // in the same row-major order as the grouped path
-            const float4* t4 = reinterpret_cast<const float4*>(g.f32_pad + (int64_t)b * g.pad);
-            const int q1 = g.pad / 4;
-            double y0 = 0.0, y1 = 0.0;
+        } else {
 #pragma unroll
-            for (int q = 0; q < (S * S + 3) / 4; ++q) {
-                const float4 a0 = t4[q], a1 = t4[q1 + q];
-                const float e0[4] = {a0.x, a0.y, a0.z, a0.w}, e1[4] = {a1.x, a1.y, a1.z, a1.w};
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int e = 4 * q + j;
-                    if (e < S * S) {
-                        y0 = fma((double)e0[j], x[e % S], y0);
-                        y1 = fma((double)e1[j], x[e % S], y1);
-                        if (e % S == S - 1) {
-                            out[e / S] = fma(f, y1 - y0, y0);
-                            y0 = y1 = 0.0;
-                        }
-                    }
+            for (int p = 0; p < P; ++p)
+                if (active[p]) {
+                    double out[S];
+                    gather_product<S>(g, b[p], f[p], x[p], out);
+                    finish_lane<S>(g, base, lane0 + 64 * p, out);
                 }
-            }
-        }
-        if (!active) continue;
-        if (g.conserve_sum > 0.0f) {  // codegen.py:95-102
-            double tot = 0.0;
-#pragma unroll
-            for (int r = 0; r < S; ++r) tot += out[r];
-            const double corr = (double)g.conserve_sum / tot;
-#pragma unroll
-            for (int r = 0; r < S; ++r) out[r] *= corr;
-        }
-#pragma unroll
-        for (int k = 0; k < S; ++k) g.states[k * g.stride + i] = (float)out[k];
-        if (g.cond_out) {
-            double w = 0.0;
-#pragma unroll
-            for (int k = 0; k < S; ++k) w = fma((double)g.weights[k], (double)(float)out[k], w);
-            g.cond_out[i] = (float)((double)g.gmax[i] * w);
         }
     }
 }
@@ -199,8 +367,6 @@ __global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
 // squarings of e^(-lambda 2^-s) expm(X 2^-s) follow, renormalised column by column when the
 // scheme conserves its total and normalised columns are asked for.
 // ----------------------------------------------------------------------------------------------
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
 // C = A B for 16 x 16 row-major FP64 matrices in LDS, one wavefront; C may alias A or B.
 // Operands of v_mfma_f64_16x16x4f64 per lane l: A[l % 16][k0 + l / 16], B[k0 + l / 16][l % 16];
 // result register r of lane l: row 4 r + l / 16, column l % 16 (measured on the device with unit
@@ -347,9 +513,12 @@ int nw_lti_table_create(int device, int32_t S, int32_t num_nodes, float vmin, fl
     t->vmax = vmax;
     t->pad_stride = (S * S + 3) / 4 * 4;
     t->f32 = node_table;
-    const size_t n_pad = (size_t)num_nodes * t->pad_stride, n64 = (size_t)num_nodes * S * S;
+    // the FP64 nodes are followed by 8 KB of zeros, the lines touch_node_pair requests past the
+    // last node pair
+    const size_t n_pad = (size_t)num_nodes * t->pad_stride, n64 = (size_t)num_nodes * S * S + 1024;
     hipError_t e = hipMalloc(&t->f32_pad, sizeof(float) * n_pad);
     if (e == hipSuccess) e = hipMalloc(&t->f64, sizeof(double) * n64);
+    if (e == hipSuccess) e = hipMemset(t->f64, 0, sizeof(double) * n64);
     if (e == hipSuccess) {
         k_lti_table_layouts<<<256, 256>>>(num_nodes, S, t->pad_stride, node_table, t->f32_pad,
                                           t->f64);
@@ -405,25 +574,24 @@ int nw_lti_table_advance(const nw_lti_table* t, int64_t M, const float* voltage,
     g.weights = weights;
     g.gmax = gmax;
     g.cond_out = cond_out;
-    // tiles of 256 instances, a run of consecutive tiles per workgroup, as many workgroups as
-    // are resident at once on the device (one round: no tail of late workgroups)
-    const int64_t tiles = (M + 255) / 256;
+    // as many workgroups as are resident at once on the device, each a run of consecutive tiles
+    // (one round: no tail of late workgroups)
     hipStream_t s = (hipStream_t)stream;
-    auto launch = [&](auto kernel) -> int {
+    auto launch = [&](auto kernel, int tile) -> int {
         int per_cu = 0;
         NW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
         const int64_t resident = std::max<int64_t>(1, (int64_t)std::max(per_cu, 1) * t->num_cus);
-        const int64_t groups = std::min<int64_t>(tiles, resident);
-        g.tiles_per_group = (tiles + groups - 1) / groups;
-        const unsigned grid = (unsigned)((tiles + g.tiles_per_group - 1) / g.tiles_per_group);
+        const unsigned grid = (unsigned)std::min<int64_t>((M + tile - 1) / tile, resident);
         kernel<<<grid, 256, 0, s>>>(g);
         return NW_OK;
     };
     int rc = NW_OK;
-#define NW_LTI_G(K)                                              \
-    case K:                                                      \
-        rc = launch(k_lti_grouped<K>);                           \
-        break;
+#define NW_LTI_G(K)                                                               \
+    case K: {                                                                     \
+        constexpr int P = lti_instances_per_lane<K>();                            \
+        rc = launch(k_lti_grouped<K, P>, 256 * P);                                \
+        break;                                                                    \
+    }
     switch (t->S) {
         NW_LTI_G(1) NW_LTI_G(2) NW_LTI_G(3) NW_LTI_G(4) NW_LTI_G(5) NW_LTI_G(6) NW_LTI_G(7)
         NW_LTI_G(8) NW_LTI_G(9) NW_LTI_G(10) NW_LTI_G(11) NW_LTI_G(12) NW_LTI_G(13)

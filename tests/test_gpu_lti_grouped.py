@@ -16,9 +16,18 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _scheme():
+def _scheme(states=13):
+    """The 13-state AMPA13-graph scheme or a 6-state chain."""
     from neuwon_amd import channels
-    sc = channels.random_ampa13_scheme(seed=2, time_step=0.1)
+    if states == 13:
+        sc = channels.random_ampa13_scheme(seed=2, time_step=0.1)
+    else:
+        rng = np.random.default_rng(9)
+        edges = [(i, i + 1) for i in range(states - 1)] + [(i + 1, i) for i in range(states - 1)]
+        sc = channels.voltage_markov_scheme(edges, np.exp(rng.uniform(-4.0, 0.0, len(edges))),
+                                            rng.uniform(-0.05, 0.05, len(edges)), time_step=0.1,
+                                            num_states=states,
+                                            weights=np.eye(states)[-1])
     node32 = sc.node_table().astype(np.float32)
     return sc, node32
 
@@ -38,12 +47,13 @@ def _table(torch, node32):
 
 
 @pytest.mark.parametrize("order", ["sorted", "random"])
-def test_grouped_equals_gather_form(torch_gpu, order):
+@pytest.mark.parametrize("states", [13, 16, 8, 6, 5, 2])
+def test_grouped_equals_gather_form(torch_gpu, order, states):
     """Sorted voltages (one or two buckets per wavefront: the grouped path) and random ones (up to
     64 buckets per wavefront): group_max 0 (every wavefront gathers), the default and 64 (every
-    wavefront grouped) give bitwise equal states and conductances."""
+    wavefront grouped) give the same states and conductances, bit for bit."""
     torch = torch_gpu
-    sc, node32 = _scheme()
+    sc, node32 = _scheme(states)
     tab = _table(torch, node32)
     M = 200_003                     # not a multiple of the wavefront or the workgroup
     rng = np.random.default_rng(7)
