@@ -377,6 +377,18 @@ int nw_ca_step(const nw_propagator* p, const nw_ca_params* prm, int32_t n_steps,
 int nw_weighted_sum(int64_t n, const double* w, const float* x, const float* y, const float* z,
                     double z_scale, double* partials, int32_t num_partials, void* stream);
 
+/* nw_weighted_sum without a workspace argument.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The num_partials partials (at most NW_SUM_SCRATCH_PARTIALS) live in a device
+ * array of the library's own, one per device, held until the process ends; a second kernel sums
+ * them in order into total[0] (FP64, device), the value a host loop over nw_weighted_sum's
+ * partials gives.  The call allocates nothing.  Calls that overlap on one device (two streams,
+ * two threads) share that array and race. */
+#define NW_SUM_SCRATCH_PARTIALS 4096
+int nw_weighted_sum_total(int64_t n, const double* w, const float* x, const float* y,
+                          const float* z, double z_scale, int32_t num_partials, double* total,
+                          void* stream);
+
 /* Stimulus generator, the same hash the step kernel evaluates (exposed for checks). */
 uint32_t nw_stim_hash(uint32_t seed, uint32_t cell, uint32_t tick);
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "nw_stim_hash": (c_u32, [c_u32, c_u32, c_u32]),
     "nw_ca_step": (c_i32, [c_vp, ctypes.POINTER(NwCaParams), c_i32, c_vp, c_vp, c_vp, c_vp]),
     "nw_weighted_sum": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_f64, c_vp, c_i32, c_vp]),
+    "nw_weighted_sum_total": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_vp, c_f64, c_i32, c_vp, c_vp]),
     "nw_clock_tick": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nw_trace_all": (c_i32, [c_i64, c_vp, c_vp, c_vp, c_f64, c_f64, c_vp]),
     "nw_hh_mechanism": (c_i32, [c_i64, c_i32, c_f32, c_f32, c_vp, c_i32, c_f32, c_f32, c_vp, c_vp,

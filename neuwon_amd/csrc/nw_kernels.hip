@@ -13,6 +13,7 @@
 // The voltage of the cell stays in LDS across steps; gates and per-segment constants stream from
 // HBM (they are a few percent of the propagator's bytes).
 #include <algorithm>
+#include <mutex>
 #include <cstring>
 
 #include <type_traits>
@@ -1155,6 +1156,16 @@ __global__ void k_weighted_sum(int64_t n, const double* __restrict__ w, const fl
     }
 }
 
+// total[0] = part[0] + part[1] + ... + part[num - 1], summed in that order by one thread (the
+// order of a host loop over the partials, so the two agree bit for bit).
+__global__ void k_sum_in_order(const double* __restrict__ part, int num, double* __restrict__ total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < num; ++k) t += part[k];
+        total[0] = t;
+    }
+}
+
 // AP detector of every neuron (neuron.py:72-76): over = v[seg] >= threshold;
 // detected = over and not previously over; state = over.
 __global__ void k_ap_detect(int64_t num, const int32_t* __restrict__ seg,
@@ -1228,6 +1239,37 @@ static int hh_step_graph(const nw_propagator* p, const nw_hh_params& prm, int n_
     k_set_i64<<<1, 64, 0, s>>>(p->step_dev, prm.step0);
     NW_HIP(hipGetLastError());
     NW_HIP(hipGraphLaunch(hit->exec, s));
+    return NW_OK;
+}
+
+}  // namespace nw
+
+// Partials of nw_weighted_sum_total: a file-scope device array, one copy per device, held until
+// the process ends.  Host code reaches it through the address hipGetSymbolAddress returns for the
+// current device (scratch_of_device), never through the array's name.
+constexpr int kSumScratch = NW_SUM_SCRATCH_PARTIALS;
+static __device__ __align__(256) double scratch[kSumScratch];
+
+namespace nw {
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the first call there.
+static int scratch_of_device(double** out) {
+    int dev = 0;
+    NW_HIP(hipGetDevice(&dev));
+    NW_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the library tracks", dev,
+               kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        NW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NW_REQUIRE(p, "no device address for the partials array on device %d", dev);
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
     return NW_OK;
 }
 
@@ -1529,6 +1571,25 @@ int nw_weighted_sum(int64_t n, const double* w, const float* x, const float* y, 
                "nw_weighted_sum: bad argument");
     k_weighted_sum<<<(unsigned)num_partials, 256, 0, (hipStream_t)stream>>>(n, w, x, y, z, z_scale,
                                                                             partials);
+    NW_HIP(hipGetLastError());
+    return NW_OK;
+}
+
+int nw_weighted_sum_total(int64_t n, const double* w, const float* x, const float* y,
+                          const float* z, double z_scale, int32_t num_partials, double* total,
+                          void* stream) {
+    NW_REQUIRE(n >= 0 && w && x && total, "nw_weighted_sum_total: null argument");
+    NW_REQUIRE(num_partials >= 1 && num_partials <= kSumScratch,
+               "nw_weighted_sum_total: num_partials %d outside 1..%d (the partials array it holds "
+               "per device)", num_partials, kSumScratch);
+    double* part = nullptr;
+    int rc = scratch_of_device(&part);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    // every partial the sum reads is written by this call's first kernel: nothing to clear
+    k_weighted_sum<<<(unsigned)num_partials, 256, 0, s>>>(n, w, x, y, z, z_scale, part);
+    NW_HIP(hipGetLastError());
+    k_sum_in_order<<<1, 64, 0, s>>>(part, num_partials, total);
     NW_HIP(hipGetLastError());
     return NW_OK;
 }
