@@ -402,9 +402,9 @@ def adjoint_rate(seed=4, store=None):
 
 
 KINETIC_BYTES_PER_INSTANCE_STEP = 4.0 + 2 * 13 * 4.0   # v read, 13 FP32 states read and written
-# HBM bytes per instance-step of k_lti_grouped<13> on the coherent protocol, from FETCH_SIZE and
-# WRITE_SIZE runs (scaled as for the step kernels); None until measured
-MEASURED_KINETIC_TRAFFIC = (None, "profiles/k_lti_grouped_1M_13state.md")
+# HBM bytes per instance-step of k_lti_grouped<13, 1> on the coherent protocol, from FETCH_SIZE and
+# WRITE_SIZE runs (scaled as for the step kernels)
+MEASURED_KINETIC_TRAFFIC = (113.4, "profiles/k_lti_grouped_1M_13state.md")
 
 
 def _kinetic_protocol(torch, name, v0, steps, dev):
@@ -493,7 +493,7 @@ def kinetic_bench(args):
                         "frac": achieved / HBM_PEAK_GBS,
                         "traffic": None if traffic is None else traffic * M,
                         "traffic_source": traffic_src,
-                        "kernel": "nw::k_lti_grouped<13>", "avg_launch_ms": avg * 1e3,
+                        "kernel": "nw::k_lti_grouped<13, 1>", "avg_launch_ms": avg * 1e3,
                         "algorithmic_bytes_per_instance_step": KINETIC_BYTES_PER_INSTANCE_STEP},
            "random_voltages": {
                "value": M * K / rnd["wall_s"], "avg_launch_ms": rnd["kernel_ms"] / K,
@@ -550,9 +550,8 @@ def kinetic_bench(args):
     print(json.dumps(out), flush=True)
 
 
-# HBM bytes per compartment-step of k_ca_step on the 5M workload (FETCH_SIZE x 2 + WRITE_SIZE);
-# None until measured
-MEASURED_SPECIES_TRAFFIC = (None, "profiles/k_ca_step_5M_species.md")
+# HBM bytes per compartment-step of k_ca_step on the 5M workload (FETCH_SIZE x 2 + WRITE_SIZE)
+MEASURED_SPECIES_TRAFFIC = (116.9, "profiles/k_ca_step_5M_species.md")
 
 
 def species_bench(args):
