@@ -222,7 +222,7 @@ def test_chebyshev_large_cell(torch_gpu):
 def test_chebyshev_model_with_a_many_branched_soma(torch_gpu):
     """RxD_Model(propagator_store='chebyshev'): a soma with six dendrites (seven neighbours, the
     overflow list) and hh everywhere, 2 nA for 1 ms; the soma trace and every segment at the end
-    against the general store built without pruning (both the same exp(A dt) to ~1e-10)."""
+    against the general store built without pruning (both the same exp(A dt) to ~5e-9)."""
     from neuwon_amd.model import RxD_Model, TimeSeries
 
     def run(store):
