@@ -1,6 +1,9 @@
 """State snapshots on the GPU: advancing from a restored snapshot reproduces the run that wrote
 it.  Bit for bit with the general propagator store; the symmetric store adds its atomically
-ordered sums in a different order each run, so there it agrees to FP32 rounding."""
+ordered sums in a different order each run, so there it agrees to FP32 rounding.  A GPU snapshot
+restored into the FP64 oracle continues the same trajectory as the GPU resume, at the parity
+bar."""
+import os
 import sys
 
 import numpy as np
@@ -134,3 +137,84 @@ def test_model_resume(torch_gpu, tmp_path, symmetric):
     c.Neuron([5000, 0, 0], 5)
     with pytest.raises(ValueError):
         c.load_state(f)
+
+
+def _oracle_stim(pop):
+    p = pop.params
+    if p.stim_mode == 2:
+        return ("poisson", float(p.stim_amp_a), int(p.stim_seed), int(p.poisson_threshold),
+                int(p.pulse_ticks))
+    if p.stim_mode == 1:
+        return ("constant", float(p.stim_amp_a))
+    return None
+
+
+@pytest.mark.parametrize("store", ["general", "chebyshev"])
+def test_hh_snapshot_resumed_in_the_oracle(torch_gpu, tmp_path, store):
+    """A snapshot the GPU wrote after 1,000 Poisson-stimulated steps, restored into the FP64 oracle
+    (neuwon/database/sql.py:78-128 is the persistence it replaces) and into a fresh GPU
+    population: both run 1,000 more steps, every cell's root voltage after every step and every
+    voltage at the end compared at the parity bar, beside the oracle with FP32 storage emulated
+    from the same snapshot, on the FP32 matrix the device stores for the stored format (the
+    floor)."""
+    import scipy.sparse
+
+    import fp32_floor
+    from neuwon_amd import checkpoint, population, synth
+    from oracle import neuwon_oracle as O
+    torch = torch_gpu
+    cells = synth.population(total=20_000, seed=11, size_range=(200, 2000))
+    stim = {"mode": "poisson", "amp_na": 0.6, "rate_hz": 40.0, "duration_ms": 1.0, "seed": 5}
+    a = population.HHPopulation(cells, stim=stim, store=store)
+    a.advance(1000)
+    f = tmp_path / "hh.npz"
+    a.save(f)
+    b = population.HHPopulation(cells, store=store)
+    b.load(f)
+    steps = 1000
+    tr = torch.zeros((b.num_cells, steps), dtype=torch.float32, device=b.dev)
+    b.advance(steps, trace=tr)
+    torch.cuda.synchronize()
+    tr = tr.cpu().numpy().astype(np.float64)
+    v_gpu = b.v.cpu().numpy().astype(np.float64)
+    # the oracle from the snapshot file: the operator the store applies (pruned as stored for the
+    # general store, exp(A dt) unpruned for the matrix-free one), the state, the step count
+    arrays, meta = checkpoint.load(f, "HHPopulation")
+    prune = b.propagator.prune if store == "general" else 0.0
+    blocks = [population.cell_propagator_f64(b.parent_local[r0:r1], b.R[r0:r1], b.C[r0:r1],
+                                             0.5 * b.dt, prune)
+              for r0, r1 in zip(b.cell_ptr[:-1], b.cell_ptr[1:])]
+    P = scipy.sparse.block_diag(blocks, format="csr")
+    # the floor: FP32 storage of the state and, for the stored format, the FP32 matrix the device
+    # holds (exported as stored: FP32 values, the diagonal that keeps the FP64 row sums)
+    P_floor = P if store != "general" else scipy.sparse.block_diag(
+        [b.propagator.export_cell(c).astype(np.float64) for c in range(b.num_cells)], format="csr")
+    roots = b.cell_ptr[:-1]
+    runs = []
+    for emulate in (False, True):
+        co = O.CHHPopulation(b.cell_ptr, P_floor if emulate else P, b.C, b.g_bar, dt=b.dt,
+                             celsius=b.celsius,
+                             gate_mode=b.gate_mode, stim=_oracle_stim(b), fp32_storage=emulate,
+                             round_v=emulate and store == "general")
+        co.v[:] = arrays["v"].astype(np.float64)
+        for g in "mhn":
+            co.gates[g][:] = arrays[g].astype(np.float64)
+        m, h, n = (co.gates[g] for g in "mhn")
+        g_na, g_k, g_l = co.g["na"] * m ** 3 * h, co.g["k"] * n ** 4, co.g["l"]
+        co.sum_g[:] = g_na + g_k + g_l
+        co.d_num[:] = g_na * b.reversal["na"] + g_k * b.reversal["k"] + g_l * b.reversal["l"]
+        co.steps = int(meta["steps"])
+        ref = np.empty((b.num_cells, steps))
+        for k in range(steps):
+            co.advance(1, threads=len(os.sched_getaffinity(0)))
+            ref[:, k] = co.v[roots]
+        runs.append((ref, co.v.copy()))
+    (ref, v_ref), (flo, v_flo) = runs
+    spikes = int(b.ap_count.cpu().numpy().sum() - arrays["ap_count"].sum())
+    _say(f"{store} store, resumed at step {meta['steps']} in the oracle: {b.num_cells} cells, "
+         f"{spikes} spikes in the last {steps} steps")
+    fp32_floor.check_cells(tr, ref, flo, b.dt, say=_say)
+    end, floor_end = np.abs(v_gpu - v_ref).max(), np.abs(v_flo - v_ref).max()
+    _say(f"every segment at the end: {end:.2e} mV (floor {floor_end:.2e})")
+    assert spikes > 0
+    assert end < max(2e-3, 3.0 * floor_end)
