@@ -574,7 +574,9 @@ class KineticSim:
 def _c_lib():
     import ctypes
     import os
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle.so")
+    # NEUWON_ORACLE_LIB: another build of the same source (the sanitizer build of the tests)
+    path = os.environ.get("NEUWON_ORACLE_LIB") or \
+        os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle.so")
     if not os.path.exists(path):
         raise RuntimeError(f"{path} missing: run `make -C oracle`")
     lib = ctypes.CDLL(path)
