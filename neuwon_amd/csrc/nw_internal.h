@@ -558,26 +558,40 @@ __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const
 }
 
 // Chebyshev products over cells laid out in preorder slots.  Thread t owns the J consecutive
-// slots s = J t + j and keeps their last two terms in FP64 registers; the term of slot s is
-// written to LDS place j * 1024 + t.  T_k = 2 A~ T_{k-1} - T_{k-2} is formed as
-//   2 d T_{k-1}(s) + 2 a_l T_{k-1}(s-1) + 2 a_r T_{k-1}(s+1) + 2 a_f T_{k-1}(f) + 2 a_g T_{k-1}(g)
-//   - T_{k-2}(s),   d = 1 - (a_l + a_r + a_f + a_g) in FP64,
+// slots s = J t + j and keeps their last two terms in FP64 registers.  T_k = 2 A~ T_{k-1} - T_{k-2}
+// is formed as
+//   2 d T_{k-1}(s) + 2 a_l T_{k-1}(s-1) + 2 a_r T_{k-1}(s+1) + 2 a_o T_{k-1}(f_s) + 2 a_p T_{k-1}(f_{s-1})
+//   - T_{k-2}(s),   d = 1 - (a_l + a_r + a_o + a_p) in FP64,
 // with s - 1 and s + 1 taken from the thread's own registers or, across threads of a wave, from
-// the neighbouring lane (DPP row shifts of the whole wave), and the far neighbours f, g read from
-// the LDS vector of the previous term.  The FP32 coefficients a = (2/rho) b are widened to FP64
-// and doubled exactly, so constant vectors stay constant to FP64 rounding.
+// the neighbouring lane (DPP row shifts of the whole wave).  Every slot has one LDS read port:
+// per term it reads the place f_s of the previous term vector.  A row takes its far neighbours
+// from its own port and from the port of slot s - 1 (the thread's register, or lane l - 1's last
+// port by DPP), so two far neighbours cost one read per slot; the layout's holes keep every row
+// within these two ports.  The FP32 coefficients a = (2/rho) b are widened to FP64 and doubled
+// exactly, so constant vectors stay constant to FP64 rounding.
+//
+// In the two term vectors, row groups 2p and 2p + 1 of thread t sit side by side (one 16-byte
+// store per pair and term), an odd last row group alone (cheb_term_place).  x arrives at, and
+// out[j] belongs to, the contract place j * 1024 + t of slot J t + j.
 //
 // A launch is built for at most MAXJ row groups per thread (MAXJ = 1 or kChebRows): its LDS holds
 // two term vectors of BUF = MAXJ * 1024 doubles and the series coefficients (cells of more terms
 // are read in natural order).  The one-group launch needs 24 KB and at most 64 VGPRs, so two
-// workgroups share a CU.  Each cell's layout leaves its last place (J * 1024 - 1) empty: unused
-// far slots read the zero kept there in both vectors.
+// workgroups share a CU.  Each cell's layout leaves its last slot (J * 1024 - 1) empty: unused
+// ports read the zero kept at its term place in both vectors.
 constexpr int kChebBuf = kChebBlock * kChebRows;
 constexpr int kChebCoefCap = 1024;
 constexpr int cheb_lds(int maxj) { return 2 * maxj * kChebBlock + kChebCoefCap; }
 // the rational product: x, then q / z as complex, then up to 4095 level starts (doubles)
 constexpr int rat_lds(int maxj) { return 3 * maxj * kChebBlock + 1024; }
-constexpr uint32_t kChebHole = 0xFFFFFFFFu;
+// pre_far flag: the slot holds no row (a hole, or padding after the cell's last slot)
+constexpr uint32_t kChebNoRow = 0x80000000u;
+
+// Place of slot J t + j in a term vector.
+__host__ __device__ constexpr int cheb_term_place(int t, int j, int J) {
+    return (j >> 1) < (J >> 1) ? (j >> 1) * 2 * kChebBlock + 2 * t + (j & 1)
+                               : (J >> 1) * 2 * kChebBlock + t;
+}
 
 // v of lane l - 1 (wave_shr:1; lane 0 receives 0) and of lane l + 1 (wave_shl:1; lane 63: 0).
 __device__ __forceinline__ double lane_before(double x) {
@@ -597,46 +611,61 @@ __device__ __forceinline__ double lane_after(double x) {
 // registers of 5 slots do not leave room).
 template <int J, typename CT>
 struct ChebRegs {
-    uint32_t f[J], g[J];   // LDS byte offsets of the far neighbours
-    CT cl[J], cr[J], cf[J], cg[J];
+    uint32_t f[J];         // LDS byte offset (in a term vector) the slot's port reads
+    CT cl[J], cr[J], co[J], cp[J];
     double d2[J];          // 2 d
 };
 
+// Store the thread's J values into a term vector at their term places.
+template <int J>
+__device__ __forceinline__ void cheb_store_terms(double* v, const double (&u)[J]) {
+#pragma unroll
+    for (int p = 0; p < J / 2; ++p)
+        *reinterpret_cast<double2*>(v + p * 2 * kChebBlock + 2 * threadIdx.x) = make_double2(u[2 * p], u[2 * p + 1]);
+    if (J & 1) v[(J / 2) * 2 * kChebBlock + threadIdx.x] = u[J - 1];
+}
+
+// The value x holds in lane `lane` (wave-uniform), from two scalar reads.
+__device__ __forceinline__ double lane_value(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // One term: t holds T_{k-1} of the thread's slots, u holds T_{k-2} and receives T_k (the two
-// arrays swap roles from term to term, so no register is copied); far neighbours from buffer RD,
-// T_k to the other buffer, out += c_k T_k, then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
+// arrays swap roles from term to term, so no register is copied); the ports read buffer RD,
+// T_k goes to the other buffer, out += ck T_k, then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
 // (wave-uniform): the wave holds a slot in use; a wave without one only keeps the barriers.
+// The last row group's port is read first: row 0 needs it (through lane l - 1).
 template <int J, typename CT, int RD, bool FIRST, int BUF>
 __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R, const double (&t)[J],
-                                          double (&u)[J], bool active, int k, double (&out)[kChebRows]) {
+                                          double (&u)[J], bool active, double ck, double (&out)[kChebRows]) {
     if (active) {
         const char* rd = reinterpret_cast<const char*>(lds + RD * BUF);
         double* wr = lds + (1 - RD) * BUF;
-        const double ck = lds[2 * BUF + k];
-        double vf[J], vg[J];
+        double vf[J];
+        vf[J - 1] = *reinterpret_cast<const double*>(rd + R.f[J - 1]);
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-            vf[j] = *reinterpret_cast<const double*>(rd + R.f[j]);
-            vg[j] = *reinterpret_cast<const double*>(rd + R.g[j]);
-        }
+        for (int j = 0; j < J - 1; ++j) vf[j] = *reinterpret_cast<const double*>(rd + R.f[j]);
         const double before = lane_before(t[J - 1]);
         const double after = lane_after(t[0]);
+        const double port_before = lane_before(vf[J - 1]);
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const double tl = j > 0 ? t[j - 1] : before;
             const double tr = j < J - 1 ? t[j + 1] : after;
+            const double vp = j > 0 ? vf[j - 1] : port_before;
             double acc = FIRST ? R.d2[j] * t[j] : fma(R.d2[j], t[j], -u[j]);
             acc = fma((double)R.cl[j], tl, acc);
             acc = fma((double)R.cr[j], tr, acc);
-            acc = fma((double)R.cf[j], vf[j], acc);
-            acc = fma((double)R.cg[j], vg[j], acc);
+            acc = fma((double)R.co[j], vf[j], acc);
+            acc = fma((double)R.cp[j], vp, acc);
             u[j] = FIRST ? 0.5 * acc : acc;
         }
+        cheb_store_terms<J>(wr, u);
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-            wr[threadIdx.x + j * kChebBlock] = u[j];
-            out[j] = fma(ck, u[j], out[j]);
-        }
+        for (int j = 0; j < J; ++j) out[j] = fma(ck, u[j], out[j]);
     }
     __syncthreads();
 }
@@ -649,12 +678,8 @@ template <int J, typename CT, int BUF, int NT>
 __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, const float4* __restrict__ cf,
                                               const double* __restrict__ c, int K, int size, double* lds,
                                               double (&out)[kChebRows]) {
-    constexpr int zero = J * kChebBlock - 1;
+    constexpr int zero = cheb_term_place(kChebBlock - 1, J - 1, J);
     for (int k = threadIdx.x; k <= K; k += NT) lds[2 * BUF + k] = c[k];
-    if (threadIdx.x == 0) {
-        lds[zero] = 0.0;
-        lds[BUF + zero] = 0.0;
-    }
     ChebRegs<J, CT> R;
     double x[J], y[J];
     const int wave_slot0 = __builtin_amdgcn_readfirstlane(threadIdx.x & ~(nw::kWave - 1)) * J;
@@ -663,36 +688,48 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
     for (int j = 0; j < J; ++j) {
         const int q = threadIdx.x + j * kChebBlock;
         const uint32_t w = far[q];
-        if (w != kChebHole) {
+        R.f[j] = 8u * (w & 0xFFFFu);
+        if (!(w & kChebNoRow)) {
             const float4 a = cf[q];
-            R.f[j] = 8u * (w & 0xFFFFu);
-            R.g[j] = 8u * (w >> 16);
             R.cl[j] = (CT)(2.0f * a.x);
             R.cr[j] = (CT)(2.0f * a.y);
-            R.cf[j] = (CT)(2.0f * a.z);
-            R.cg[j] = (CT)(2.0f * a.w);
+            R.co[j] = (CT)(2.0f * a.z);
+            R.cp[j] = (CT)(2.0f * a.w);
             R.d2[j] = 2.0 * (1.0 - (((double)a.x + (double)a.y) + ((double)a.z + (double)a.w)));
             x[j] = lds[q];
-        } else {  // hole or padding: zero, read by nobody through LDS
-            R.f[j] = R.g[j] = 8u * zero;
-            R.cl[j] = R.cr[j] = R.cf[j] = R.cg[j] = (CT)0;
+        } else {  // hole or padding: zero, read by nobody but through the zero place
+            R.cl[j] = R.cr[j] = R.co[j] = R.cp[j] = (CT)0;
             R.d2[j] = 0.0;
             x[j] = 0.0;
         }
         y[j] = 0.0;
     }
+    __syncthreads();  // x read from the contract places: buffer 0 takes the term layout
+    cheb_store_terms<J>(lds, x);
+    if (threadIdx.x == 0) {  // (NT < 1024: the zero slot's thread does not exist)
+        lds[zero] = 0.0;
+        lds[BUF + zero] = 0.0;
+    }
     __syncthreads();
-    const double c0 = lds[2 * BUF];
+    // the series coefficients in blocks of 64: lane i holds c_{64 b + i}, each term reads its own
+    // from the lane (k & 63) (two scalar reads instead of an LDS read per term)
+    const int lane = threadIdx.x & (nw::kWave - 1);
+    double cb = lds[2 * BUF + lane];
+    const double c0 = lane_value(cb, 0);
 #pragma unroll
     for (int j = 0; j < J; ++j) out[j] = c0 * x[j];
     // x = T_0, then y = T_1, x = T_2, y = T_3, ...
-    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, 1, out);
+    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, lane_value(cb, 1), out);
     int k = 2;
     for (; k + 1 <= K; k += 2) {
-        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
-        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, k + 1, out);
+        if ((k & (nw::kWave - 1)) == 0) cb = lds[2 * BUF + k + lane];  // (k + 63 < kChebCoefCap)
+        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, lane_value(cb, k & (nw::kWave - 1)), out);
+        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, lane_value(cb, (k + 1) & (nw::kWave - 1)), out);
     }
-    if (k <= K) cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
+    if (k <= K) {
+        if ((k & (nw::kWave - 1)) == 0) cb = lds[2 * BUF + k + lane];
+        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, lane_value(cb, k & (nw::kWave - 1)), out);
+    }
 }
 
 // The product for one cell, preorder slots (base >= 0) or natural order, from x in LDS (at the
@@ -853,7 +890,7 @@ __device__ __forceinline__ int cheb_cell(const ChebStore& cs, int64_t row0, int 
         else if (J == 2) cheb_pre_rows<2, double, BUF, NT>(far, cf, c, K, size, lds, out);
         else if (J == 3) cheb_pre_rows<3, double, BUF, NT>(far, cf, c, K, size, lds, out);
         else if (J == 4) cheb_pre_rows<4, double, BUF, NT>(far, cf, c, K, size, lds, out);
-        else cheb_pre_rows<5, float, BUF, NT>(far, cf, c, K, size, lds, out);
+        else cheb_pre_rows<5, double, BUF, NT>(far, cf, c, K, size, lds, out);
     }
     return J * kChebBlock;
     }

@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
-    __shared__ double b0[prod_lds(MAXJ, KIND)];
+    __shared__ __align__(16) double b0[prod_lds(MAXJ, KIND)];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -703,7 +703,7 @@ __global__ void __launch_bounds__(BLOCK) k_hh_adjoint(AdjArgs a) {
 template <int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_adj_cheb(AdjArgs a, ChebStore cs, double* lam64, int k, int first, int last) {
-    __shared__ double b0[prod_lds(MAXJ, KIND)];
+    __shared__ __align__(16) double b0[prod_lds(MAXJ, KIND)];
     __shared__ double red[7][NT / kWave];
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];

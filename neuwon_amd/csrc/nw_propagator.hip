@@ -258,11 +258,17 @@ struct Workspace {
     uint32_t* d_off = nullptr;
     rocblas_handle blas = nullptr;
     hipStream_t stream = nullptr;
+    // the series path's per-cell arrays (grown on demand)
+    int32_t *s_nbi = nullptr, *s_ovi = nullptr;
+    double *s_nbc = nullptr, *s_ovc = nullptr, *s_dg = nullptr, *s_c = nullptr, *s_C = nullptr;
+    int s_rows = 0, s_ov = 0, s_terms = 0;
 
-    int init(int n) {
+    // dense: the Taylor path's six n x n buffers (cells above kSeriesMaxRows rows); otherwise
+    // two (P and its symmetrised copy)
+    int init(int n, bool dense = true) {
         nmax = n;
         size_t nn = (size_t)n * n * sizeof(double);
-        for (auto& x : X) NW_HIP(hipMalloc(&x, nn));
+        for (int i = 0; i < (dense ? 6 : 2); ++i) NW_HIP(hipMalloc(&X[i], nn));
         NW_HIP(hipMalloc(&d_ri, sizeof(int32_t) * 2 * n));
         NW_HIP(hipMalloc(&d_ci, sizeof(int32_t) * 2 * n));
         NW_HIP(hipMalloc(&d_v, sizeof(double) * 2 * n));
@@ -276,7 +282,42 @@ struct Workspace {
         NW_BLAS(rocblas_set_pointer_mode(blas, rocblas_pointer_mode_host));
         return NW_OK;
     }
+    int series_arrays(int n, int ov, int terms) {
+        if (n > s_rows) {
+            if (s_nbi) (void)hipFree(s_nbi);
+            if (s_nbc) (void)hipFree(s_nbc);
+            if (s_dg) (void)hipFree(s_dg);
+            if (s_C) (void)hipFree(s_C);
+            s_nbi = nullptr;
+            s_nbc = s_dg = s_C = nullptr;
+            NW_HIP(hipMalloc(&s_nbi, sizeof(int32_t) * 3 * n));
+            NW_HIP(hipMalloc(&s_nbc, sizeof(double) * 3 * n));
+            NW_HIP(hipMalloc(&s_dg, sizeof(double) * n));
+            NW_HIP(hipMalloc(&s_C, sizeof(double) * n));
+            s_rows = n;
+        }
+        if (ov > s_ov) {
+            if (s_ovi) (void)hipFree(s_ovi);
+            if (s_ovc) (void)hipFree(s_ovc);
+            s_ovi = nullptr;
+            s_ovc = nullptr;
+            NW_HIP(hipMalloc(&s_ovi, sizeof(int32_t) * ov));
+            NW_HIP(hipMalloc(&s_ovc, sizeof(double) * ov));
+            s_ov = ov;
+        }
+        if (terms > s_terms) {
+            if (s_c) (void)hipFree(s_c);
+            s_c = nullptr;
+            NW_HIP(hipMalloc(&s_c, sizeof(double) * terms));
+            s_terms = terms;
+        }
+        return NW_OK;
+    }
     ~Workspace() {
+        for (int32_t* x : {s_nbi, s_ovi})
+            if (x) (void)hipFree(x);
+        for (double* x : {s_nbc, s_ovc, s_dg, s_c, s_C})
+            if (x) (void)hipFree(x);
         for (auto& x : X)
             if (x) (void)hipFree(x);
         if (d_ri) (void)hipFree(d_ri);
@@ -497,17 +538,259 @@ static double tree_spectral_radius(int n, const int32_t* parent, const double* d
     return hi * (1.0 + 1e-8);
 }
 
+// ----------------------------------------------------------------------------------------------
+// P = exp(A dt) of one cell by uniformization (cells of at most kSeriesMaxRows rows).  With
+// B = A dt (off-diagonal b_ij = dt / (R_edge C_i) >= 0, zero row sums) and sigma at least every
+// |B_ii|, M = I + B / sigma is entrywise nonnegative with unit row sums, and
+//   exp(B) = e^-sigma exp(sigma M) = sum_k w_k M^k,   w_k = e^-sigma sigma^k / k!
+// (Poisson weights; Jensen's method).  Every term is nonnegative, so each entry of P, however
+// small, is accurate to about K ulps relative (no cancellation, as in the nonnegative Taylor
+// polynomial of the dense path), and the kept weights are normalised to sum to one, so P 1 = 1.
+// Workgroup j forms column j, M^k e_j for k = 0 .. K, and writes row j of P from it: A = C^-1 G
+// with G symmetric gives P^T = C P C^-1, so P_jr = C_r P_rj / C_j.  The work is about
+// 2 (deg + 1) K n^2 flop per cell (K about sigma + 8 sqrt(sigma)) instead of the 30 n^3 of the
+// dense Taylor path with its squarings.
+// ----------------------------------------------------------------------------------------------
+constexpr int kSeriesMaxRows = kChebRows * kChebBlock;
+constexpr double kSeriesTol = 1e-17;      // Poisson mass dropped on each side
+constexpr double kSeriesMargin = 1.02;    // sigma = margin * max_i |B_ii|: M_ii >= 0.02
+
+// Poisson weights of mean sigma from the mode outwards (no underflow of e^-sigma), both tails
+// dropped below tol of the total, normalised; returns the last index K (w[0 .. K]).
+static int poisson_weights(double sigma, double tol, std::vector<double>& w) {
+    w.clear();
+    if (!(sigma > 0.0)) {
+        w.push_back(1.0);
+        return 0;
+    }
+    const int mode = (int)std::floor(sigma);
+    std::vector<double> up(1, 1.0), down;
+    for (int k = mode + 1;; ++k) {  // w_k / w_mode
+        const double x = up.back() * sigma / k;
+        if (x < 0.01 * tol && k > mode + 4) break;
+        up.push_back(x);
+    }
+    for (int k = mode; k >= 1; --k) {
+        const double x = (down.empty() ? 1.0 : down.back()) * k / sigma;
+        if (x < 0.01 * tol) break;
+        down.push_back(x);  // w_{k-1} / w_mode
+    }
+    const int lo = mode - (int)down.size();
+    const int K = mode + (int)up.size() - 1;
+    w.assign(K + 1, 0.0);
+    for (size_t i = 0; i < down.size(); ++i) w[mode - 1 - i] = down[i];
+    for (size_t i = 0; i < up.size(); ++i) w[mode + i] = up[i];
+    double sum = 0.0;
+    for (int k = K; k >= lo; --k) sum += w[k];
+    // drop the tails below tol of the total, then renormalise
+    double acc = 0.0;
+    int last = K;
+    while (last > mode && (acc += w[last]) < tol * sum) w[last--] = 0.0;
+    acc = 0.0;
+    for (int k = lo; k < mode && (acc += w[k]) < tol * sum; ++k) w[k] = 0.0;
+    w.resize(last + 1);
+    sum = 0.0;
+    for (int k = last; k >= 0; --k) sum += w[k];
+    for (auto& x : w) x /= sum;
+    return last;
+}
+
+// NT threads, ROWS rows per thread (row r = threadIdx.x + NT q); nbi / nbc: three neighbour
+// slots per row (local index, M_ij; an unused slot names the row itself with coefficient 0;
+// slot 2 holds -(start + 1) of an overflow list ended by -1 when the row has more than three
+// neighbours), dg: M_ii.  Rows row0 .. row0 + gridDim.x - 1 of P.
+template <int NT, int ROWS>
+__global__ void __launch_bounds__(NT) k_expm_rows(int n, const int32_t* __restrict__ nbi,
+                                                  const double* __restrict__ nbc,
+                                                  const int32_t* __restrict__ ovi,
+                                                  const double* __restrict__ ovc,
+                                                  const double* __restrict__ dg,
+                                                  const double* __restrict__ wk, int K,
+                                                  const double* __restrict__ C, int row0,
+                                                  double* __restrict__ P) {
+    __shared__ double buf[2][ROWS * NT];
+    const int col = row0 + (int)blockIdx.x;
+    uint32_t i01[ROWS];
+    int32_t i2[ROWS];
+    double a[ROWS][3], d[ROWS], t[ROWS], out[ROWS];
+    const double w0 = wk[0];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+        const int r = threadIdx.x + q * NT;
+        const bool row = r < n;
+        i01[q] = row ? ((uint32_t)nbi[3 * r] | ((uint32_t)nbi[3 * r + 1] << 16)) : 0u;
+        i2[q] = row ? nbi[3 * r + 2] : 0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) a[q][e] = row ? nbc[3 * r + e] : 0.0;
+        d[q] = row ? dg[r] : 0.0;
+        t[q] = r == col ? 1.0 : 0.0;
+        out[q] = w0 * t[q];
+        buf[0][r] = t[q];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int k = 1; k <= K; ++k) {
+        const double ck = wk[k];
+        const double* v = buf[cur];
+        double* w = buf[1 - cur];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+            const int r = threadIdx.x + q * NT;
+            if (r < n) {
+                double s = d[q] * t[q];
+                s = fma(a[q][0], v[i01[q] & 0xFFFFu], s);
+                s = fma(a[q][1], v[i01[q] >> 16], s);
+                if (i2[q] >= 0) {
+                    s = fma(a[q][2], v[i2[q]], s);
+                } else {
+                    for (int e = -i2[q] - 1; ovi[e] >= 0; ++e) s = fma(ovc[e], v[ovi[e]], s);
+                }
+                t[q] = s;
+                w[r] = s;
+                out[q] = fma(ck, s, out[q]);
+            }
+        }
+        __syncthreads();
+        cur = 1 - cur;
+    }
+    const double inv_cc = 1.0 / C[col];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+        const int r = threadIdx.x + q * NT;
+        if (r < n) P[(size_t)col * n + r] = out[q] * (C[r] * inv_cc);
+    }
+}
+
+template <int NT>
+static void launch_expm_rows(int rows, int n, const int32_t* nbi, const double* nbc,
+                             const int32_t* ovi, const double* ovc, const double* dg,
+                             const double* wk, int K, const double* C, double* P, hipStream_t st) {
+    const int per = (n + NT - 1) / NT;
+    switch (per) {
+        case 1: k_expm_rows<NT, 1><<<rows, NT, 0, st>>>(n, nbi, nbc, ovi, ovc, dg, wk, K, C, 0, P); break;
+        case 2: k_expm_rows<NT, 2><<<rows, NT, 0, st>>>(n, nbi, nbc, ovi, ovc, dg, wk, K, C, 0, P); break;
+        case 3: k_expm_rows<NT, 3><<<rows, NT, 0, st>>>(n, nbi, nbc, ovi, ovc, dg, wk, K, C, 0, P); break;
+        case 4: k_expm_rows<NT, 4><<<rows, NT, 0, st>>>(n, nbi, nbc, ovi, ovc, dg, wk, K, C, 0, P); break;
+        default: k_expm_rows<NT, 5><<<rows, NT, 0, st>>>(n, nbi, nbc, ovi, ovc, dg, wk, K, C, 0, P); break;
+    }
+}
+
+// The uniformization operator of one cell on the host: M's neighbour slots (nb, nc, overflow
+// lists ovi / ovc) and diagonal dg, and the Poisson weights w[0 .. K]; returns K.
+struct SeriesOperator {
+    std::vector<int32_t> nb, ovi;
+    std::vector<double> nc, ovc, dg, w;
+    double sigma = 0.0;
+};
+static int series_operator(int n, const int32_t* parent, const double* R, const double* C,
+                           double dt_ms, SeriesOperator& op) {
+    const double dt = dt_ms * 1e-3;  // seconds, electric.py:86
+    std::vector<std::vector<std::pair<int, double>>> adj(n);
+    std::vector<double> rowsum(n, 0.0);
+    for (int c = 0; c < n; ++c) {  // b_ij = dt / (R_edge C_i), electric.py:92-108
+        const int q = parent[c];
+        if (q < 0) continue;
+        NW_REQUIRE(q < n, "parent index %d out of range for cell of %d segments", q, n);
+        const double bc = dt / (R[c] * C[c]), bq = dt / (R[c] * C[q]);
+        adj[c].push_back({q, bc});
+        adj[q].push_back({c, bq});
+        rowsum[c] += bc;
+        rowsum[q] += bq;
+    }
+    double smax = 0.0;
+    for (double x : rowsum) smax = std::max(smax, x);
+    op.sigma = kSeriesMargin * smax;
+    const int K = poisson_weights(op.sigma, kSeriesTol, op.w);
+    const double inv = op.sigma > 0.0 ? 1.0 / op.sigma : 0.0;
+    op.nb.assign(3 * (size_t)n, 0);
+    op.nc.assign(3 * (size_t)n, 0.0);
+    op.dg.assign(n, 0.0);
+    op.ovi.clear();
+    op.ovc.clear();
+    for (int i = 0; i < n; ++i) {
+        const auto& e = adj[i];
+        op.dg[i] = 1.0 - rowsum[i] * inv;
+        for (int s = 0; s < 3; ++s) op.nb[3 * i + s] = i;
+        const int direct = e.size() <= 3 ? (int)e.size() : 2;
+        for (int s = 0; s < direct; ++s) {
+            op.nb[3 * i + s] = e[s].first;
+            op.nc[3 * i + s] = e[s].second * inv;
+        }
+        if (e.size() > 3) {
+            op.nb[3 * i + 2] = -(int32_t)op.ovi.size() - 1;
+            for (size_t s = 2; s < e.size(); ++s) {
+                op.ovi.push_back(e[s].first);
+                op.ovc.push_back(e[s].second * inv);
+            }
+            op.ovi.push_back(-1);
+            op.ovc.push_back(0.0);
+        }
+    }
+    if (op.ovi.empty()) {
+        op.ovi.push_back(-1);
+        op.ovc.push_back(0.0);
+    }
+    return K;
+}
+
+// P of one cell (n <= kSeriesMaxRows) into w.X[0] by uniformization; *result = w.X[0].
+static int expm_cell_series(Workspace& w, int n, const int32_t* parent, const double* R,
+                            const double* C, double dt_ms, double** result) {
+    NW_REQUIRE(n >= 1 && n <= kSeriesMaxRows, "series build: %d rows", n);
+    SeriesOperator op;
+    const int K = series_operator(n, parent, R, C, dt_ms, op);
+    if (K < 0) return K;
+    const auto &nb = op.nb, &ovi = op.ovi;
+    const auto &nc = op.nc, &ovc = op.ovc, &dg = op.dg, &wts = op.w;
+    // device copies in the workspace's series arrays (grown on demand)
+    int rc;
+    if ((rc = w.series_arrays(n, (int)ovi.size(), K + 1))) return rc;
+    NW_HIP(hipMemcpyAsync(w.s_nbi, nb.data(), sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice, w.stream));
+    NW_HIP(hipMemcpyAsync(w.s_nbc, nc.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, w.stream));
+    NW_HIP(hipMemcpyAsync(w.s_ovi, ovi.data(), sizeof(int32_t) * ovi.size(), hipMemcpyHostToDevice, w.stream));
+    NW_HIP(hipMemcpyAsync(w.s_ovc, ovc.data(), sizeof(double) * ovc.size(), hipMemcpyHostToDevice, w.stream));
+    NW_HIP(hipMemcpyAsync(w.s_dg, dg.data(), sizeof(double) * n, hipMemcpyHostToDevice, w.stream));
+    NW_HIP(hipMemcpyAsync(w.s_c, wts.data(), sizeof(double) * (K + 1), hipMemcpyHostToDevice, w.stream));
+    NW_HIP(hipMemcpyAsync(w.s_C, C, sizeof(double) * n, hipMemcpyHostToDevice, w.stream));
+    if (n <= 5 * 256)
+        launch_expm_rows<256>(n, n, w.s_nbi, w.s_nbc, w.s_ovi, w.s_ovc, w.s_dg, w.s_c, K, w.s_C, w.X[0], w.stream);
+    else
+        launch_expm_rows<kChebBlock>(n, n, w.s_nbi, w.s_nbc, w.s_ovi, w.s_ovc, w.s_dg, w.s_c, K, w.s_C, w.X[0], w.stream);
+    NW_HIP(hipGetLastError());
+    // the host arrays above must outlive the copies
+    NW_HIP(hipStreamSynchronize(w.stream));
+    *result = w.X[0];
+    return NW_OK;
+}
+
+// P of one cell: the series for cells of at most kSeriesMaxRows rows, the dense Taylor path above.
+static int exp_cell(Workspace& w, int n, const int32_t* parent, const double* R, const double* C,
+                    double dt_ms, double** result) {
+    if (n <= kSeriesMaxRows) return expm_cell_series(w, n, parent, R, C, dt_ms, result);
+    return expm_cell(w, n, parent, R, C, dt_ms, result);
+}
+
 // Preorder slot layout of one cell whose rows have at most three neighbours (adj[i]: the parent,
 // an earlier index, first, then the children in increasing order; coefficients already scaled by
 // 2 / rho).  Rows go in depth-first preorder, a first child right after its parent, into slots
-// s = J t + j (thread t, row group j, LDS place j * 1024 + t).  A row reaches the rows in slots
-// s - 1 and s + 1 without LDS (own registers, or the neighbouring lane of the same wave) unless
-// s is the first slot of a wave (s - 1 is in another wave) or the last (s + 1 is); every other
-// neighbour is "far" and read from LDS.  A row would need three far reads only in the last slot
-// of a wave, as a second child with two children: such a row moves to the next slot, leaving a
-// hole.  The last place, J * 1024 - 1, stays empty: it holds the zero that unused far slots read.
+// s = J t + j (thread t, row group j; the product's contract place of slot s is j * 1024 + t, its
+// place in the term vectors cheb_term_place(t, j, J)).  A row reaches the rows in slots s - 1 and
+// s + 1 without LDS (own registers, or the neighbouring lane of the same wave) unless s is the
+// first slot of a wave (s - 1 is in another wave) or the last (s + 1 is); every other neighbour
+// is "far".  Each slot has one LDS read port per term; the row in slot s reads its far
+// neighbours through its own port and/or the port of slot s - 1 (same wave).  Rows are placed
+// greedily in preorder, a row with one far neighbour taking the previous slot's port when that is
+// free (so its own stays free for the next row); a row that finds no port moves on by a hole.  A
+// hole goes before a row that is not a first child next to its parent (the slot before it holds
+// the last row of another subtree, so no slot neighbour is separated); when the row is such a
+// first child, the whole chain of first children it ends moves on by one slot instead (a hole
+// before the chain's head), which keeps every parent-child pair of the chain adjacent.  The last
+// slot, J * 1024 - 1, stays empty: it holds the zero that unused ports read.
 // Returns the slots in use, or -1 when the layout needs more than kChebRows row groups; sets J,
-// fills pos (LDS place of every natural row) and the 1024 J places of far/coef.
+// fills pos (contract place of every natural row) and, per contract place, far (the term-vector
+// place its port reads; kChebNoRow set for a slot without a row) and coef (the coefficients of
+// the neighbours in slots s - 1 and s + 1, then of the far neighbours read through the slot's
+// own port and through the previous slot's port).
 static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, double>>>& adj,
                            double scale, int& J_out, std::vector<uint16_t>& pos,
                            std::vector<uint32_t>& far, std::vector<float4>& coef) {
@@ -523,60 +806,104 @@ static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, d
     }
     if (next != n) return -1;
     auto parent_of = [&](int i) { return (!adj[i].empty() && adj[i][0].first < i) ? adj[i][0].first : -1; };
-    std::vector<int32_t> slot(n);
-    int J = n / kChebBlock + 1, size = 0;
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        const int W = kWave * J;
-        int s = -1;
-        for (int q = 0; q < n; ++q) {
-            int sn = s + 1;
-            if (q > 0 && sn % W == W - 1) {
-                const int i = order[q];
-                const int par = parent_of(i);
-                const int nch = (int)adj[i].size() - (par >= 0 ? 1 : 0);
-                const bool left_near = par >= 0 && place[par] == q - 1;
-                const int nfar = (par >= 0 && !left_near) + (nch >= 1) + (nch >= 2);
-                if (nfar > 2) ++sn;
+    // preorder position q is the first child of position q - 1
+    auto first_child = [&](int q) { return q > 0 && parent_of(order[q]) == order[q - 1]; };
+    std::vector<int32_t> slot(n), hole_before(n);
+    std::vector<uint8_t> uses(n), port_used;  // uses: 1 = own port, 2 = previous slot's port
+    int J = n / kChebBlock + 1, size = -1;
+    for (; J <= kChebRows; ++J) {
+        const int W = kWave * J, cap = J * kChebBlock - 1;  // slot cap holds the zero
+        port_used.assign((size_t)cap + 1, 0);
+        std::fill(hole_before.begin(), hole_before.end(), 0);
+        int q = 0, s = -1, shifts = 0;
+        bool fits = true;
+        while (q < n && fits) {
+            const int i = order[q], par = parent_of(i);
+            const int nch = (int)adj[i].size() - (par >= 0 ? 1 : 0);
+            int sn = s + 1 + hole_before[q];
+            int shift_from = -1;
+            for (;;) {
+                if (sn >= cap) {
+                    fits = false;
+                    break;
+                }
+                const bool par_near = par >= 0 && slot[place[par]] == sn - 1 && sn % W != 0;
+                const int nf = (par >= 0 && !par_near) + (nch >= 1 && (sn + 1) % W == 0) + (nch >= 2);
+                const bool prev_free = sn % W != 0 && !port_used[sn - 1];
+                if (nf <= 1 || (nf == 2 && prev_free)) {
+                    uses[q] = nf == 0 ? 0 : nf == 2 ? 3 : prev_free ? 2 : 1;
+                    if (uses[q] & 1) port_used[sn] = 1;
+                    if (uses[q] & 2) port_used[sn - 1] = 1;
+                    break;
+                }
+                if (par_near) {  // a first child: moving it alone would part it from its parent
+                    shift_from = q;
+                    break;
+                }
+                ++sn;
             }
-            slot[q] = sn;
-            s = sn;
+            if (!fits) break;
+            if (shift_from < 0) {
+                slot[q] = sn;
+                s = sn;
+                ++q;
+                continue;
+            }
+            // move the chain of first children that ends at q on by one slot: undo its rows
+            int q0 = q - 1;
+            while (q0 > 0 && first_child(q0) && slot[q0] == slot[q0 - 1] + 1 && slot[q0] % W != 0) --q0;
+            if (++shifts > 8 * (n / kWave + 8)) return -1;
+            for (int r = q0; r < q; ++r) {
+                if (uses[r] & 1) port_used[slot[r]] = 0;
+                if (uses[r] & 2) port_used[slot[r] - 1] = 0;
+            }
+            ++hole_before[q0];
+            q = q0;
+            s = q0 > 0 ? slot[q0 - 1] : -1;
         }
-        size = s + 1;
-        // more row groups never need more holes, so J only grows
-        const int Jn = size / kChebBlock + 1;  // room for the zero place after the last slot
-        if (Jn <= J) break;
-        J = Jn;
+        if (fits) {
+            size = s + 1;
+            break;
+        }
     }
-    if (size > J * kChebBlock - 1 || J > kChebRows) return -1;
-    const uint32_t zero = (uint32_t)(J * kChebBlock - 1);
+    if (size < 0) return -1;
+    const int W = kWave * J;
+    const uint32_t zero = (uint32_t)cheb_term_place(kChebBlock - 1, J - 1, J);
     J_out = J;
+    auto contract_place = [&](int sl) { return (uint32_t)((sl % J) * kChebBlock + sl / J); };
+    auto term_place = [&](int sl) { return (uint32_t)cheb_term_place(sl / J, sl % J, J); };
     const size_t base = far.size();
-    far.resize(base + (size_t)J * kChebBlock, kChebHole);
+    far.resize(base + (size_t)J * kChebBlock, kChebNoRow | zero);
     coef.resize(base + (size_t)J * kChebBlock, float4{0.0f, 0.0f, 0.0f, 0.0f});
-    auto place_of_slot = [&](int s) { return (uint32_t)((s % J) * kChebBlock + s / J); };
+    for (int q = 0; q < n; ++q) {  // rows: the flag off
+        const uint32_t pl = contract_place(slot[q]);
+        far[base + pl] &= ~kChebNoRow;
+    }
     for (int q = 0; q < n; ++q) {
         const int i = order[q];
-        const int s = slot[q], t = s / J, j = s % J, lane = t % kWave;
-        const uint32_t pl = place_of_slot(s);
+        const int sl = slot[q];
+        const uint32_t pl = contract_place(sl);
         pos[i] = (uint16_t)pl;
         float c4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        uint32_t fp[2] = {zero, zero};
         int nfar = 0;
         for (const auto& e : adj[i]) {
             const int sm = slot[place[e.first]];
             const float a = (float)(scale * e.second);
-            if (sm == s - 1 && !(j == 0 && lane == 0)) {
+            if (sm == sl - 1 && sl % W != 0) {
                 c4[0] = a;
-            } else if (sm == s + 1 && !(j == J - 1 && lane == kWave - 1)) {
+            } else if (sm == sl + 1 && sm % W != 0) {
                 c4[1] = a;
             } else {
-                if (nfar >= 2) return -1;
-                fp[nfar] = place_of_slot(sm);
-                c4[2 + nfar] = a;
+                // far: through the previous slot's port first when the row uses it, then its own
+                const bool prev = (uses[q] & 2) && nfar == 0;
+                const int port = prev ? sl - 1 : sl;
+                if (nfar >= 2 || (!prev && !(uses[q] & 1))) return -1;  // placement miscounted
+                uint32_t& w = far[base + contract_place(port)];
+                w = (w & kChebNoRow) | term_place(sm);
+                c4[prev ? 3 : 2] = a;
                 ++nfar;
             }
         }
-        far[base + pl] = fp[0] | (fp[1] << 16);
         coef[base + pl] = float4{c4[0], c4[1], c4[2], c4[3]};
     }
     return size;
@@ -714,7 +1041,7 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
     NW_HIP(hipMemcpy(p->cheb_c, p->h_cheb_c.data(), sizeof(double) * p->h_cheb_c.size(),
                      hipMemcpyHostToDevice));
     if (pfar.empty()) {  // valid (never read) arrays
-        pfar.push_back(kChebHole);
+        pfar.push_back(kChebNoRow);
         pcoef.push_back(float4{0.0f, 0.0f, 0.0f, 0.0f});
     }
     NW_HIP(hipMalloc(&p->pre_pos, sizeof(uint16_t) * std::max<int64_t>(N, 1)));
@@ -1011,14 +1338,14 @@ static int build(int device, int64_t num_cells, const int64_t* cell_ptr, const i
     if ((rc = grow(p, std::max<int64_t>(guess, 1)))) return rc;
 
     Workspace w;
-    if ((rc = w.init(nmax))) return rc;
+    if ((rc = w.init(nmax, nmax > kSeriesMaxRows))) return rc;
     std::vector<uint32_t> off;
     std::vector<int64_t> cell_nnz(num_cells);
     for (int64_t c = 0; c < num_cells; ++c) {
         int64_t r0 = cell_ptr[c];
         int n = (int)(cell_ptr[c + 1] - r0);
         double* P = nullptr;
-        if ((rc = expm_cell(w, n, parent + r0, R + r0, C + r0, dt_ms, &P))) return rc;
+        if ((rc = exp_cell(w, n, parent + r0, R + r0, C + r0, dt_ms, &P))) return rc;
         if (sym) {  // S = W P W^-1, w = sqrt(C / C_first) (any per-cell scale cancels)
             std::vector<double> wv(n);
             for (int i = 0; i < n; ++i) wv[i] = std::sqrt(C[r0 + i] / C[r0]);
@@ -1165,7 +1492,7 @@ __global__ void __launch_bounds__(256) k_half_step(HalfArgs a) {
 template <int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_half_step_cheb(HalfArgs a, ChebStore cs) {
-    __shared__ double b0[prod_lds(MAXJ, KIND)];
+    __shared__ __align__(16) double b0[prod_lds(MAXJ, KIND)];
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
     const int n = (int)(a.cell_row0[cell + 1] - row0);
@@ -1345,9 +1672,9 @@ int nw_propagator_cell_f64(int device, int32_t n, const int32_t* parent_local_ho
     NW_HIP(hipSetDevice(device));
     Workspace w;
     int rc;
-    if ((rc = w.init(n))) return rc;
+    if ((rc = w.init(n, n > kSeriesMaxRows))) return rc;
     double* P = nullptr;
-    if ((rc = expm_cell(w, n, parent_local_host, R_host, C_host, time_step_ms, &P))) return rc;
+    if ((rc = exp_cell(w, n, parent_local_host, R_host, C_host, time_step_ms, &P))) return rc;
     std::vector<uint32_t> off;
     int32_t max_row;
     int64_t kept;

@@ -839,11 +839,24 @@ class HHAdjointProblem:
         N = gna * self.E["na"] + gk * self.E["k"] + gl * self.E["l"]
         return gna, gk, G, N
 
-    def forward(self, gb, theta, steps, target=None, record=False):
+    def forward(self, gb, theta, steps, target=None, record=False, fp32_storage=False):
+        """fp32_storage: emulate FP32 storage (per-segment conductances, 1/C, the gate tables'
+        nodes, every gate after its update, and the record's voltages rounded to FP32; the
+        voltages themselves stay FP64 between steps, as on the matrix-free stores), to measure
+        the floor that FP32 storage sets (not the reference's arithmetic)."""
+        r32 = (lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)) if fp32_storage \
+            else (lambda a: a)
         tables = hh_theta_tables(self.dt, self.celsius, theta)
+        if fp32_storage:
+            gb = {k: r32(gb[k]) for k in gb}
+            for g in "mhn":
+                tables[g] = tables[g].copy()
+                tables[g][:, :2] = r32(tables[g][:, :2])
+        self._C_run = 1.0 / r32(1.0 / self.C) if fp32_storage else self.C
+        self._gb_run = gb
         hs = 0.5 * self.dt * 1e-3
         hdt = 0.5 * self.dt
-        bI = hdt * self.stim / self.C
+        bI = hdt * self.stim / self._C_run
         v = np.full(self.n, self.v0)
         ss = {g: float(gate_inf(self.v0, g, self.celsius)) for g in "mhn"}
         # rest state of the scaled rates
@@ -860,18 +873,18 @@ class HHAdjointProblem:
             else:
                 _, _, G, N = self._g(gb, m, h, n_)
                 D = N / G
-                al = np.exp(-hs * G / self.C)
+                al = np.exp(-hs * G / self._C_run)
                 x = D * (1 - al) + v * al + bI
             v1 = self.P.dot(x)
             new = {}
             for g in "mhn":
                 val, _ = self._lookup(tables[g], v1)
-                new[g] = val[:, 0] + val[:, 1] * gates[g]
+                new[g] = r32(val[:, 0] + val[:, 1] * gates[g])
             if record:
-                rec.append((v.copy(), dict(gates), v1.copy()))
+                rec.append((r32(v.copy()), dict(gates), r32(v1.copy())))
             _, _, G, N = self._g(gb, new["m"], new["h"], new["n"])
             D = N / G
-            al = np.exp(-hs * G / self.C)
+            al = np.exp(-hs * G / self._C_run)
             v = self.P.dot(D * (1 - al) + v1 * al + bI)
             gates = new
             soma.append(v[0])
@@ -881,11 +894,13 @@ class HHAdjointProblem:
             self._rec, self._final_gates, self._tables = rec, gates, tables
         return soma, loss
 
-    def gradient(self, gb, theta, target):
+    def gradient(self, gb, theta, target, fp32_storage=False):
         """Reverse pass over the recorded forward; returns (loss, grads) with grads
-        {'na','k','l': per segment, 'theta': {gate: (dL/dt_alpha, dL/dt_beta)}}."""
+        {'na','k','l': per segment, 'theta': {gate: (dL/dt_alpha, dL/dt_beta)}}.  fp32_storage:
+        the reverse pass of the FP32-storage emulation of forward() (FP64 arithmetic)."""
         steps = len(target)
-        soma, loss = self.forward(gb, theta, steps, target, record=True)
+        soma, loss = self.forward(gb, theta, steps, target, record=True, fp32_storage=fp32_storage)
+        gb, C = self._gb_run, self._C_run
         hs = 0.5 * self.dt * 1e-3
         E = self.E
         lv = np.zeros(self.n)
@@ -912,10 +927,10 @@ class HHAdjointProblem:
                 return lx, {g: np.zeros(self.n) for g in "mhn"}
             _, _, G, N = self._g(gb, gates["m"], gates["h"], gates["n"])
             D = N / G
-            al = np.exp(-hs * G / self.C)
+            al = np.exp(-hs * G / C)
             lD = (1 - al) * lx
             lal = (v_in - D) * lx
-            lG = lal * al * (-hs / self.C) - lD * D / G
+            lG = lal * al * (-hs / C) - lD * D / G
             lN = lD / G
             return al * lx, g_adjoint(gates, lG, lN)
 

@@ -101,10 +101,12 @@ def test_adjoint_large_cells_match_oracle(torch_gpu, n, num_cells):
     five-row-group ones) on the Chebyshev store for both passes, subthreshold: per-segment and
     rate gradients and the loss against the FP64 oracle adjoint on the unpruned propagator (the
     rate gradients relative to their largest component: a small component is a difference of
-    large per-cell terms).  The two 700-compartment cells' rate gradients differ from the oracle's
-    by 7e-4 of the largest on every store, the stored formats' step and reverse kernels included
-    (the 400- and 1,500-compartment cells of the same seed: 4e-5), so that difference is the
-    problem's sensitivity to FP32 storage, not the launch shape."""
+    large per-cell terms).  The bar of each quantity is max(1e-4, 3 x its FP32-storage floor):
+    the FP64 oracle adjoint beside the same oracle with FP32 storage emulated (conductances, 1/C,
+    gate-table nodes, gates and the record's voltages rounded).  On the two 700-compartment cells
+    that floor is 6.7e-4 of the largest rate gradient (the GPU: 7.4e-4, on every store), on the
+    3,000-compartment cell 4.4e-6: the rate gradients of the smaller cells are sensitive to FP32
+    storage, whatever the launch shape."""
     from neuwon_amd import adjoint, synth
     torch = torch_gpu
     dt, T, amp = 0.05, 200, 0.1e-9
@@ -125,21 +127,31 @@ def test_adjoint_large_cells_match_oracle(torch_gpu, n, num_cells):
     f32 = lambda a: torch.as_tensor(a.astype(np.float32), device="cuda")
     loss, gr = prob.loss_and_grad(f32(gb["na"]), f32(gb["k"]), f32(gb["l"]), np.ones(6), tgt)
     g_na, g_k, g_l = (gr[k].cpu().numpy() for k in ("na", "k", "l"))
-    th_ref, loss_ref = np.zeros(6), 0.0
+    th_ref, th_flo, loss_ref = np.zeros(6), np.zeros(6), 0.0
+    theta_vec = lambda r: np.array([r["theta"][g][j] for g in "mhn" for j in (0, 1)])
+    tgt32 = tgt.cpu().numpy().astype(np.float64)
     for c, (r0, r1, op) in enumerate(oracles):
-        lr, ref = op.gradient({k: gb[k][r0:r1] for k in gb}, {}, targets[c])
+        gbc = {k: gb[k][r0:r1] for k in gb}
+        lr, ref = op.gradient(gbc, {}, targets[c])
+        _, flo = op.gradient(gbc, {}, tgt32[c], fp32_storage=True)
         loss_ref += lr
         errs = [_rel(g_na[r0:r1], ref["na"]), _rel(g_k[r0:r1], ref["k"]), _rel(g_l[r0:r1], ref["l"])]
-        _say(f"{n}-compartment cell {c}: per-segment gradient rel err {['%.1e' % e for e in errs]}, "
-             f"v_soma max {targets[c].max():.1f}, target spikes "
+        floors = [_rel(flo[k], ref[k]) for k in ("na", "k", "l")]
+        _say(f"{n}-compartment cell {c}: per-segment gradient rel err {['%.1e' % e for e in errs]} "
+             f"(FP32-storage floor {['%.1e' % e for e in floors]}), v_soma max "
+             f"{targets[c].max():.1f}, target spikes "
              f"{int(((targets[c][1:] >= 20) & (targets[c][:-1] < 20)).sum())}")
-        assert max(errs) < 1e-4
-        th_ref += np.array([ref["theta"][g][j] for g in "mhn" for j in (0, 1)])
+        for e, f in zip(errs, floors):
+            assert e < max(1e-4, 3.0 * f), (errs, floors)
+        th_ref += theta_vec(ref)
+        th_flo += theta_vec(flo)
     th_err = np.max(np.abs(gr["theta"] - th_ref)) / np.max(np.abs(th_ref))
+    th_floor = np.max(np.abs(th_flo - th_ref)) / np.max(np.abs(th_ref))
     _say(f"{n} compartments: loss {loss:.6e} vs {loss_ref:.6e}; theta gradient {gr['theta']} vs "
-         f"{th_ref}: error relative to the largest component {th_err:.2e}")
+         f"{th_ref}: error relative to the largest component {th_err:.2e} (FP32-storage floor "
+         f"{th_floor:.2e})")
     assert loss == pytest.approx(loss_ref, rel=1e-4)
-    assert th_err < 1e-4
+    assert th_err < max(1e-4, 3.0 * th_floor)
 
 
 def test_autograd_function(torch_gpu):
