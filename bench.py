@@ -72,8 +72,8 @@ def format_bytes(fmt, nnz_bar):
     per-segment mechanism state and constants."""
     if fmt == "symmetric":
         return 2.0 * (6.0 * (nnz_bar + 1.0) / 2.0 + 20.0) + 68.0
-    if fmt == "chebyshev":   # per half step the LDS place, 2 far places and 4 FP32 coefficients,
-        return 2.0 * (2.0 + 4.0 + 16.0 + 16.0) + 68.0   # FP64 v in/out
+    if fmt == "chebyshev":   # per half step the LDS place, the port's place word and 4 FP32
+        return 2.0 * (2.0 + 4.0 + 16.0 + 16.0) + 68.0   # coefficients, FP64 v in/out
     if fmt == "rational":    # per half step the level place, link, 2 FP32 edge coefficients, five
         return 2.0 * (2.0 + 4.0 + 8.0 + 5 * 16.0 + 16.0) + 68.0   # FP64 pivots, FP64 v in/out
     return 2.0 * (6.0 * nnz_bar + 12.0) + 68.0
@@ -841,10 +841,11 @@ def main():
             lds_b = 2.0 * RATIONAL_POLES * 5 * 16.0
         else:
             roof["series_terms_per_row"] = st["mean_terms"]
-            # per row and term two far-neighbour slots read (an unused slot reads a zero) and the
-            # new term written, 8 B each; the row's own last two terms and its r +- 1 neighbours'
-            # terms come from registers
-            lds_b = 2.0 * st["mean_terms"] * 8.0 * 3.0
+            # per row and term the slot's read port (a far neighbour's term, or the zero) and the
+            # new term written, 8 B each, and the series coefficient (one broadcast read per wave
+            # of 64 rows); the row's own last two terms and its r +- 1 neighbours' terms come from
+            # registers or the neighbouring lane
+            lds_b = 2.0 * st["mean_terms"] * (8.0 * 2.0 + 8.0 / 64.0)
         roof["lds"] = {"bytes_per_cmpt_step": lds_b, "GBs": lds_b * rate / 1e9,
                        "peak_GBs": LDS_PEAK_GBS, "frac": lds_b * rate / 1e9 / LDS_PEAK_GBS}
         roof["hbm"] = {"store_bytes_per_cmpt_step": f_step,

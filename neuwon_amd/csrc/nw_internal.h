@@ -625,23 +625,16 @@ __device__ __forceinline__ void cheb_store_terms(double* v, const double (&u)[J]
     if (J & 1) v[(J / 2) * 2 * kChebBlock + threadIdx.x] = u[J - 1];
 }
 
-// The value x holds in lane `lane` (wave-uniform), from two scalar reads.
-__device__ __forceinline__ double lane_value(double x, int lane) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)b, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
 // One term: t holds T_{k-1} of the thread's slots, u holds T_{k-2} and receives T_k (the two
 // arrays swap roles from term to term, so no register is copied); the ports read buffer RD,
-// T_k goes to the other buffer, out += ck T_k, then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
+// T_k goes to the other buffer, out += c_k T_k, then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
 // (wave-uniform): the wave holds a slot in use; a wave without one only keeps the barriers.
 // The last row group's port is read first: row 0 needs it (through lane l - 1).
 template <int J, typename CT, int RD, bool FIRST, int BUF>
 __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R, const double (&t)[J],
-                                          double (&u)[J], bool active, double ck, double (&out)[kChebRows]) {
+                                          double (&u)[J], bool active, int k, double (&out)[kChebRows]) {
     if (active) {
+        const double ck = lds[2 * BUF + k];
         const char* rd = reinterpret_cast<const char*>(lds + RD * BUF);
         double* wr = lds + (1 - RD) * BUF;
         double vf[J];
@@ -711,25 +704,17 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
         lds[BUF + zero] = 0.0;
     }
     __syncthreads();
-    // the series coefficients in blocks of 64: lane i holds c_{64 b + i}, each term reads its own
-    // from the lane (k & 63) (two scalar reads instead of an LDS read per term)
-    const int lane = threadIdx.x & (nw::kWave - 1);
-    double cb = lds[2 * BUF + lane];
-    const double c0 = lane_value(cb, 0);
+    const double c0 = lds[2 * BUF];
 #pragma unroll
     for (int j = 0; j < J; ++j) out[j] = c0 * x[j];
     // x = T_0, then y = T_1, x = T_2, y = T_3, ...
-    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, lane_value(cb, 1), out);
+    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, 1, out);
     int k = 2;
     for (; k + 1 <= K; k += 2) {
-        if ((k & (nw::kWave - 1)) == 0) cb = lds[2 * BUF + k + lane];  // (k + 63 < kChebCoefCap)
-        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, lane_value(cb, k & (nw::kWave - 1)), out);
-        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, lane_value(cb, (k + 1) & (nw::kWave - 1)), out);
+        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
+        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, k + 1, out);
     }
-    if (k <= K) {
-        if ((k & (nw::kWave - 1)) == 0) cb = lds[2 * BUF + k + lane];
-        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, lane_value(cb, k & (nw::kWave - 1)), out);
-    }
+    if (k <= K) cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
 }
 
 // The product for one cell, preorder slots (base >= 0) or natural order, from x in LDS (at the

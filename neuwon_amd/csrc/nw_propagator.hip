@@ -733,13 +733,41 @@ static int series_operator(int n, const int32_t* parent, const double* R, const 
     return K;
 }
 
-// P of one cell (n <= kSeriesMaxRows) into w.X[0] by uniformization; *result = w.X[0].
+// Squarings s for a cell of n rows whose B has sigma = kSeriesMargin max|B_ii|: the series of
+// B / 2^s costs n workgroups of K(sigma / 2^s) terms, each squaring one FP64 GEMM of 2 n^3 flop;
+// s minimises their sum on this device's rates (a 1024-thread workgroup term about
+// 300 + 400 ceil(n / 1024) cycles on one of 256 CUs at 2.4 GHz; dgemm about 60 TFLOP/s).
+// Squaring non-negative matrices keeps every entry's relative accuracy.
+static int series_squarings(int n, double sigma) {
+    const int groups = (n + kChebBlock - 1) / kChebBlock;
+    const double term_s = (300.0 + 400.0 * groups) / (256.0 * 2.4e9);
+    const double gemm_s = 2.0 * (double)n * n * n / 60e12;
+    int best = 0;
+    double best_cost = 0.0;
+    for (int s = 0; s <= 8; ++s) {
+        const double sg = std::ldexp(sigma, -s);
+        const double cost = n * (sg + 8.5 * std::sqrt(sg) + 10.0) * term_s + s * gemm_s;
+        if (s == 0 || cost < best_cost) {
+            best = s;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+// P of one cell (n <= kSeriesMaxRows) by uniformization of B / 2^s and s squarings (rocBLAS
+// FP64 GEMMs); *result points at the workspace buffer holding it.
 static int expm_cell_series(Workspace& w, int n, const int32_t* parent, const double* R,
                             const double* C, double dt_ms, double** result) {
     NW_REQUIRE(n >= 1 && n <= kSeriesMaxRows, "series build: %d rows", n);
     SeriesOperator op;
-    const int K = series_operator(n, parent, R, C, dt_ms, op);
+    int K = series_operator(n, parent, R, C, dt_ms, op);
     if (K < 0) return K;
+    const int squarings = series_squarings(n, op.sigma);
+    if (squarings > 0) {
+        K = series_operator(n, parent, R, C, std::ldexp(dt_ms, -squarings), op);
+        if (K < 0) return K;
+    }
     const auto &nb = op.nb, &ovi = op.ovi;
     const auto &nc = op.nc, &ovc = op.ovc, &dg = op.dg, &wts = op.w;
     // device copies in the workspace's series arrays (grown on demand)
@@ -757,9 +785,15 @@ static int expm_cell_series(Workspace& w, int n, const int32_t* parent, const do
     else
         launch_expm_rows<kChebBlock>(n, n, w.s_nbi, w.s_nbc, w.s_ovi, w.s_ovc, w.s_dg, w.s_c, K, w.s_C, w.X[0], w.stream);
     NW_HIP(hipGetLastError());
+    double* cur = w.X[0];
+    double* nxt = w.X[1];
+    for (int q = 0; q < squarings; ++q) {
+        if ((rc = gemm(w, n, cur, cur, nxt, 0.0))) return rc;
+        std::swap(cur, nxt);
+    }
     // the host arrays above must outlive the copies
     NW_HIP(hipStreamSynchronize(w.stream));
-    *result = w.X[0];
+    *result = cur;
     return NW_OK;
 }
 

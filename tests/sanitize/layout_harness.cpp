@@ -250,11 +250,18 @@ void poisson_checks() {
     }
 }
 
+// The squaring choice: none for a soft cell, more as sigma grows, never past 8.
+void squaring_choice_checks() {
+    CHECK(nw::series_squarings(5000, 10.0) == 0, "squarings for sigma 10");
+    const int s8 = nw::series_squarings(5000, 8000.0);
+    CHECK(s8 >= 2 && s8 <= 8, "squarings for sigma 8000: %d", s8);
+}
+
 // The uniformization build of one cell, its kernel's arithmetic emulated on the host (column j,
 // row j from the transposition identity), against a dense FP64 Taylor expm with scaling and
 // squaring of the nonnegative shifted matrix: relative 1e-11 on every entry above 1e-14 and
 // 1e-15 absolute elsewhere; rows sum to one.
-void series_build_check(int n, std::mt19937& rng) {
+void series_build_check(int n, std::mt19937& rng, int squarings = 0) {
     std::vector<int32_t> parent = random_tree(n, 2, rng);
     std::uniform_real_distribution<double> u(0.5, 2.0);
     std::vector<double> R(n), C(n);
@@ -264,7 +271,8 @@ void series_build_check(int n, std::mt19937& rng) {
     }
     const double dt_ms = 0.05;
     nw::SeriesOperator op;
-    const int K = nw::series_operator(n, parent.data(), R.data(), C.data(), dt_ms, op);
+    const int K = nw::series_operator(n, parent.data(), R.data(), C.data(),
+                                      std::ldexp(dt_ms, -squarings), op);
     CHECK(K >= 0, "n %d: no series operator", n);
     std::vector<double> P((size_t)n * n);
     std::vector<double> v(n), t(n), nv(n), out(n);
@@ -332,6 +340,10 @@ void series_build_check(int n, std::mt19937& rng) {
         mul(E, E, Y);
         E.swap(Y);
     }
+    for (int q = 0; q < squarings; ++q) {  // the build's squarings of exp(B / 2^s)
+        mul(P, P, Y);
+        P.swap(Y);
+    }
     double worst_rel = 0.0, worst_abs = 0.0, worst_sum = 0.0;
     for (int i = 0; i < n; ++i) {
         double rs = 0.0;
@@ -344,7 +356,8 @@ void series_build_check(int n, std::mt19937& rng) {
         worst_sum = std::max(worst_sum, std::fabs(rs - 1.0));
     }
     CHECK(worst_rel < 1e-11 && worst_abs < 1e-15 && worst_sum < 1e-13,
-          "n %d: uniformization vs dense expm: rel %g, abs %g, row sums %g", n, worst_rel, worst_abs, worst_sum);
+          "n %d, %d squarings: uniformization vs dense expm: rel %g, abs %g, row sums %g", n,
+          squarings, worst_rel, worst_abs, worst_sum);
     ++series_checked;
 }
 
@@ -360,6 +373,9 @@ int main() {
     poisson_checks();
     for (int n : {1, 2, 17, 120, 260})
         series_build_check(n, rng);
+    for (int n : {17, 120, 260})
+        series_build_check(n, rng, 3);
+    squaring_choice_checks();
     std::printf("layout harness: %d cells emulated, %d series builds checked, %d failures\n",
                 emulated, series_checked, fails);
     return fails ? 1 : 0;

@@ -217,7 +217,11 @@ def test_adjoint_workload_1024x500(torch_gpu, reverse_store):
 def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
     """Central finite differences of the FP64 C oracle (h = 1e-6 relative) on 32 cells of
     500 compartments, 2000 steps at dt 0.05 ms, against the GPU gradient with the default stores
-    (both passes on the Chebyshev store; the oracle's loss runs the same unpruned propagator)."""
+    (both passes on the Chebyshev store; the oracle's loss runs the same unpruned propagator):
+    the six rate parameters; per-segment entries (the soma and segment 3 of cell 0, and the
+    segment of largest gradient off the soma in cells 7, 19 and 31); and, for each of gNa, gK and
+    gL, the derivative along a random direction over all 16,000 segments at once (a positive
+    direction, so <gradient, direction> sums terms of one sign per segment class)."""
     import os
     import scipy.sparse
     from neuwon_amd import adjoint, synth
@@ -258,14 +262,29 @@ def test_adjoint_vs_fp64_finite_differences_32_cells(torch_gpu):
             tm = {g: (1 - h, 1.0) if j == 0 else (1.0, 1 - h)}
             fd = (loss64(gb, tp) - loss64(gb, tm)) / (2 * h)
             checks.append((f"theta_{g}{'ab'[j]}", gr["theta"][2 * gi + j], fd))
-    for k, i in (("na", 0), ("k", 0), ("l", 3)):
+    grads = {k: gr[k].cpu().numpy().astype(np.float64) for k in ("na", "k", "l")}
+    entries = [("na", 0), ("k", 0), ("l", 3)]
+    for k, c in (("na", 7), ("k", 19), ("l", 31)):
+        r0, r1 = int(pop.cell_ptr[c]), int(pop.cell_ptr[c + 1])
+        entries.append((k, r0 + 1 + int(np.argmax(np.abs(grads[k][r0 + 1:r1])))))
+    for k, i in entries:
         gp = {kk: vv.copy() for kk, vv in gb.items()}
         gm = {kk: vv.copy() for kk, vv in gb.items()}
         d = h * gb[k][i]
         gp[k][i] += d
         gm[k][i] -= d
         fd = (loss64(gp, {}) - loss64(gm, {})) / (2 * d)
-        checks.append((f"g_{k}[{i}]", float(gr[k][i]), fd))
+        checks.append((f"g_{k}[{i}]", float(grads[k][i]), fd))
+    rng = np.random.default_rng(11)
+    for k in ("na", "k", "l"):
+        direction = gb[k] * rng.uniform(0.5, 1.5, len(gb[k]))
+        gp = {kk: vv.copy() for kk, vv in gb.items()}
+        gm = {kk: vv.copy() for kk, vv in gb.items()}
+        gp[k] = gb[k] + h * direction
+        gm[k] = gb[k] - h * direction
+        fd = (loss64(gp, {}) - loss64(gm, {})) / (2 * h)
+        checks.append((f"<grad g_{k}, direction over {len(direction)} segments>",
+                       float(np.dot(grads[k], direction)), fd))
     for name, adj, fd in checks:
         _say(f"{name}: adjoint {adj:.6e} FD64 {fd:.6e} rel {abs(adj - fd) / abs(fd):.1e}")
     _say(f"finite differences took {time.time() - t0:.0f}s on {threads} threads")
