@@ -567,8 +567,10 @@ __device__ __forceinline__ void cheb_rows(const int32_t* __restrict__ nbi, const
 // per term it reads the place f_s of the previous term vector.  A row takes its far neighbours
 // from its own port and from the port of slot s - 1 (the thread's register, or lane l - 1's last
 // port by DPP), so two far neighbours cost one read per slot; the layout's holes keep every row
-// within these two ports.  The FP32 coefficients a = (2/rho) b are widened to FP64 and doubled
-// exactly, so constant vectors stay constant to FP64 rounding.
+// within these two ports.  A cell of one row group (J = 1) reads its two far neighbours itself
+// instead (f, g: a second read per row, no DPP on the term's chain, fewer holes).  The FP32
+// coefficients a = (2/rho) b are widened to FP64 and doubled exactly, so constant vectors stay
+// constant to FP64 rounding.
 //
 // In the two term vectors, row groups 2p and 2p + 1 of thread t sit side by side (one 16-byte
 // store per pair and term), an odd last row group alone (cheb_term_place).  x arrives at, and
@@ -612,6 +614,7 @@ __device__ __forceinline__ double lane_after(double x) {
 template <int J, typename CT>
 struct ChebRegs {
     uint32_t f[J];         // LDS byte offset (in a term vector) the slot's port reads
+    uint32_t g;            // one row group: the second far neighbour's offset
     CT cl[J], cr[J], co[J], cp[J];
     double d2[J];          // 2 d
 };
@@ -643,7 +646,9 @@ __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R,
         for (int j = 0; j < J - 1; ++j) vf[j] = *reinterpret_cast<const double*>(rd + R.f[j]);
         const double before = lane_before(t[J - 1]);
         const double after = lane_after(t[0]);
-        const double port_before = lane_before(vf[J - 1]);
+        // one row group: the row's own second read; otherwise the previous slot's port
+        const double port_before = J == 1 ? *reinterpret_cast<const double*>(rd + R.g)
+                                          : lane_before(vf[J - 1]);
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const double tl = j > 0 ? t[j - 1] : before;
@@ -682,6 +687,7 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
         const int q = threadIdx.x + j * kChebBlock;
         const uint32_t w = far[q];
         R.f[j] = 8u * (w & 0xFFFFu);
+        if (J == 1) R.g = 8u * ((w >> 16) & 0x7FFFu);
         if (!(w & kChebNoRow)) {
             const float4 a = cf[q];
             R.cl[j] = (CT)(2.0f * a.x);

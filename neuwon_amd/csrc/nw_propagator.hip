@@ -820,6 +820,7 @@ static int exp_cell(Workspace& w, int n, const int32_t* parent, const double* R,
 // first child, the whole chain of first children it ends moves on by one slot instead (a hole
 // before the chain's head), which keeps every parent-child pair of the chain adjacent.  The last
 // slot, J * 1024 - 1, stays empty: it holds the zero that unused ports read.
+// At least min_rows row groups (one row group takes preorder_layout_pairs' layout instead).
 // Returns the slots in use, or -1 when the layout needs more than kChebRows row groups; sets J,
 // fills pos (contract place of every natural row) and, per contract place, far (the term-vector
 // place its port reads; kChebNoRow set for a slot without a row) and coef (the coefficients of
@@ -827,7 +828,7 @@ static int exp_cell(Workspace& w, int n, const int32_t* parent, const double* R,
 // own port and through the previous slot's port).
 static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, double>>>& adj,
                            double scale, int& J_out, std::vector<uint16_t>& pos,
-                           std::vector<uint32_t>& far, std::vector<float4>& coef) {
+                           std::vector<uint32_t>& far, std::vector<float4>& coef, int min_rows = 2) {
     std::vector<int32_t> place(n, -1), order(n), stack(1, 0);
     int next = 0;
     while (!stack.empty()) {
@@ -844,7 +845,7 @@ static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, d
     auto first_child = [&](int q) { return q > 0 && parent_of(order[q]) == order[q - 1]; };
     std::vector<int32_t> slot(n), hole_before(n);
     std::vector<uint8_t> uses(n), port_used;  // uses: 1 = own port, 2 = previous slot's port
-    int J = n / kChebBlock + 1, size = -1;
+    int J = std::max(min_rows, n / kChebBlock + 1), size = -1;
     for (; J <= kChebRows; ++J) {
         const int W = kWave * J, cap = J * kChebBlock - 1;  // slot cap holds the zero
         port_used.assign((size_t)cap + 1, 0);
@@ -943,6 +944,78 @@ static int preorder_layout(int n, const std::vector<std::vector<std::pair<int, d
     return size;
 }
 
+// The one-row-group layout (cells of at most 1,023 slots): the same preorder slots, thread t
+// owning slot t, but two LDS reads per row (its far neighbours f and g, the zero place where
+// unused) and no port sharing, so a hole is needed only where a row in the last slot of a wave
+// would have three far neighbours (parent, first child across the wave, second child).  With
+// one row per thread the shared port's DPP would sit on each term's dependency chain, and the
+// port layout's holes (about 2 % at one row group, its wave boundaries every 64 slots) would push
+// 500-row cells past the 512-thread launch.  far word: f | g << 16 (| kChebNoRow), coef: the
+// neighbours in slots s - 1 and s + 1, then f and g.  Returns the slots in use, or -1 when the
+// cell needs more than one row group.
+static int preorder_layout_pairs(int n, const std::vector<std::vector<std::pair<int, double>>>& adj,
+                                 double scale, std::vector<uint16_t>& pos,
+                                 std::vector<uint32_t>& far, std::vector<float4>& coef) {
+    if (n > kChebBlock - 1) return -1;
+    std::vector<int32_t> place(n, -1), order(n), stack(1, 0);
+    int next = 0;
+    while (!stack.empty()) {
+        const int i = stack.back();
+        stack.pop_back();
+        place[i] = next;
+        order[next++] = i;
+        for (int d = (int)adj[i].size() - 1; d >= 0; --d)
+            if (adj[i][d].first > i) stack.push_back(adj[i][d].first);
+    }
+    if (next != n) return -1;
+    auto parent_of = [&](int i) { return (!adj[i].empty() && adj[i][0].first < i) ? adj[i][0].first : -1; };
+    std::vector<int32_t> slot(n);
+    int s = -1;
+    for (int q = 0; q < n; ++q) {
+        int sn = s + 1;
+        if (q > 0 && sn % kWave == kWave - 1) {
+            const int i = order[q];
+            const int par = parent_of(i);
+            const int nch = (int)adj[i].size() - (par >= 0 ? 1 : 0);
+            const bool left_near = par >= 0 && place[par] == q - 1;
+            if ((par >= 0 && !left_near) + (nch >= 1) + (nch >= 2) > 2) ++sn;
+        }
+        slot[q] = sn;
+        s = sn;
+    }
+    const int size = s + 1;
+    if (size > kChebBlock - 1) return -1;
+    const uint32_t zero = kChebBlock - 1;  // slot = contract place = term place at one row group
+    const size_t base = far.size();
+    far.resize(base + kChebBlock, kChebNoRow | zero | (zero << 16));
+    coef.resize(base + kChebBlock, float4{0.0f, 0.0f, 0.0f, 0.0f});
+    for (int q = 0; q < n; ++q) {
+        const int i = order[q];
+        const int sl = slot[q], lane = sl % kWave;
+        pos[i] = (uint16_t)sl;
+        float c4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t fp[2] = {zero, zero};
+        int nfar = 0;
+        for (const auto& e : adj[i]) {
+            const int sm = slot[place[e.first]];
+            const float a = (float)(scale * e.second);
+            if (sm == sl - 1 && lane != 0) {
+                c4[0] = a;
+            } else if (sm == sl + 1 && lane != kWave - 1) {
+                c4[1] = a;
+            } else {
+                if (nfar >= 2) return -1;
+                fp[nfar] = (uint32_t)sm;
+                c4[2 + nfar] = a;
+                ++nfar;
+            }
+        }
+        far[base + sl] = fp[0] | (fp[1] << 16);
+        coef[base + sl] = float4{c4[0], c4[1], c4[2], c4[3]};
+    }
+    return size;
+}
+
 // Chebyshev store: per cell the edge coefficients of B = A dt in the difference form
 // (B t)_i = sum_j b_ij (t_j - t_i), b_ij = dt / (R_edge C_i) (electric.py:92-108), scaled by
 // 2 / rho, the series coefficients, and the cells ordered by decreasing work (terms x rows).
@@ -1011,8 +1084,13 @@ static int build_chebyshev(nw_propagator* p, const int64_t* cell_ptr, const int3
                 for (auto& e : row) e.second *= scale;
             std::vector<uint16_t> cpos(n);
             const size_t mark = pfar.size();
-            int J = 0;
-            const int size = preorder_layout(n, sadj, 1.0, J, cpos, pfar, pcoef);
+            int J = 1;
+            int size = preorder_layout_pairs(n, sadj, 1.0, cpos, pfar, pcoef);
+            if (size < 0) {
+                pfar.resize(mark);
+                pcoef.resize(mark);
+                size = preorder_layout(n, sadj, 1.0, J, cpos, pfar, pcoef);
+            }
             if (size > 0) {
                 pbase[cell] = (int64_t)mark;
                 psize[cell] = size | (J << 16);

@@ -35,52 +35,13 @@ std::vector<int32_t> random_tree(int n, int max_children, std::mt19937& rng) {
     return parent;
 }
 
-void one_cell(int n, int max_children, std::mt19937& rng) {
-    std::vector<int32_t> parent = random_tree(n, max_children, rng);
-    std::uniform_real_distribution<double> u(0.5, 2.0);
-    std::vector<double> R(n), C(n);
-    for (int i = 0; i < n; ++i) {
-        R[i] = 1e6 * u(rng);
-        C[i] = 1e-11 * u(rng);
-    }
-    // time steps of about 20 and about 100 series terms
-    const double dt_s = (n % 2 ? 0.05e-3 : 1e-3);
-    std::vector<std::vector<std::pair<int, double>>> adj(n);
-    std::vector<double> diag(n, 0.0), off(n, 0.0);
-    for (int c = 0; c < n; ++c) {
-        const int q = parent[c];
-        if (q < 0) continue;
-        const double bc = dt_s / (R[c] * C[c]), bq = dt_s / (R[c] * C[q]);
-        adj[c].push_back({q, bc});
-        adj[q].push_back({c, bq});
-        diag[c] += bc;
-        diag[q] += bq;
-        off[c] = -dt_s / (R[c] * std::sqrt(C[c] * C[q]));
-    }
-    std::vector<double> work;
-    const double rho = nw::tree_spectral_radius(n, parent.data(), diag.data(), off.data(), work);
-    double dmax = 0.0;
-    for (double d : diag) dmax = std::max(dmax, d);
-    CHECK(n == 1 || (rho >= dmax && rho <= 2.0 * dmax * (1.0 + 1e-7)),
-          "n %d: spectral radius %g outside [max diag %g, 2 max diag]", n, rho, dmax);
-    std::vector<double> coef;
-    CHECK(nw::exp_chebyshev(rho, nw::kChebTol, coef) == 0, "n %d: no series for rho %g", n, rho);
-    double sum = 0.0;
-    for (double c : coef) sum += c;
-    CHECK(std::fabs(sum - 1.0) < 1e-13, "n %d: series coefficients sum to %.17g", n, sum);
-    int deg = 0;
-    for (const auto& a : adj) deg = std::max<int>(deg, (int)a.size());
-    if (deg > 3) return;  // such cells keep the natural order (no slot layout)
-    const double scale = rho > 0.0 ? 2.0 / rho : 0.0;
-    std::vector<uint16_t> pos(n);
-    std::vector<uint32_t> far;
-    std::vector<float4> cf;
-    int J = 0;
-    const int size = nw::preorder_layout(n, adj, scale, J, pos, far, cf);
-    if (size < 0) {
-        CHECK(n > kChebRows * kChebBlock - 64, "n %d: no layout", n);
-        return;
-    }
+// Invariants of a slot layout (pairs: the one-row-group layout, two reads per row; otherwise
+// the port layout) and the product as the kernel forms it (lanes, DPP shifts, reads, term places)
+// against the series of the same FP32 coefficients in natural order.
+void layout_check(int n, const std::vector<std::vector<std::pair<int, double>>>& adj, double scale,
+                  const std::vector<double>& coef, int J, int size, const std::vector<uint16_t>& pos,
+                  const std::vector<uint32_t>& far, const std::vector<float4>& cf, bool pairs,
+                  std::mt19937& rng) {
     const uint32_t places = (uint32_t)J * kChebBlock;
     const uint32_t zero_slot = places - 1;
     const uint32_t zero = (uint32_t)cheb_term_place(kChebBlock - 1, J - 1, J);
@@ -107,9 +68,11 @@ void one_cell(int n, int max_children, std::mt19937& rng) {
     for (uint32_t pl = 0; pl < places; ++pl) {
         const uint32_t sl = slot_of(pl);
         CHECK(((far[pl] & kChebNoRow) != 0) == (owner[sl] < 0), "n %d: slot %u row flag", n, sl);
-        const uint32_t rd = far[pl] & 0xFFFFu;
-        CHECK(rd < places, "n %d: slot %u port reads place %u", n, sl, rd);
-        CHECK(rd == zero || owner[term_owner[rd]] >= 0, "n %d: slot %u port reads an empty slot", n, sl);
+        for (uint32_t rd : {far[pl] & 0xFFFFu, (far[pl] >> 16) & 0x7FFFu}) {
+            if (!pairs && rd != (far[pl] & 0xFFFFu)) continue;
+            CHECK(rd < places, "n %d: slot %u reads place %u", n, sl, rd);
+            CHECK(rd == zero || owner[term_owner[rd]] >= 0, "n %d: slot %u reads an empty slot", n, sl);
+        }
     }
     // every neighbour of a row is reached once, with its scaled coefficient: through slot s - 1
     // or s + 1 (same wave), the row's own port, or the port of slot s - 1 (same wave)
@@ -127,7 +90,9 @@ void one_cell(int n, int max_children, std::mt19937& rng) {
             got.push_back({owner[sl + 1], a.y});
         }
         if (a.z != 0.0f) got.push_back({owner[term_owner[far[pos[i]] & 0xFFFFu]], a.z});
-        if (a.w != 0.0f) {
+        if (a.w != 0.0f && pairs) {
+            got.push_back({owner[term_owner[(far[pos[i]] >> 16) & 0x7FFFu]], a.w});
+        } else if (a.w != 0.0f) {
             CHECK(sl % W != 0, "n %d: row %d previous port across a wave", n, i);
             const uint32_t pp = ((sl - 1) % J) * kChebBlock + (sl - 1) / J;
             got.push_back({owner[term_owner[far[pp] & 0xFFFFu]], a.w});
@@ -182,13 +147,15 @@ void one_cell(int n, int max_children, std::mt19937& rng) {
     auto lane_next = [&](const std::vector<double>& v, int t, int j) {
         return t % kWave == kWave - 1 ? 0.0 : v[(size_t)(t + 1) * J + j];
     };
-    std::vector<double> port((size_t)T * J);
+    std::vector<double> port((size_t)T * J), second((size_t)T * J);
     for (int k = 1; k <= K; ++k) {
         const std::vector<double>& rd = (k % 2 == 1) ? vec0 : vec1;
         std::vector<double>& wr = (k % 2 == 1) ? vec1 : vec0;
         for (int t = 0; t < T; ++t)
-            for (int j = 0; j < J; ++j)
+            for (int j = 0; j < J; ++j) {
                 port[(size_t)t * J + j] = rd[far[j * kChebBlock + t] & 0xFFFFu];
+                second[(size_t)t * J + j] = rd[(far[j * kChebBlock + t] >> 16) & 0x7FFFu];
+            }
         std::vector<double> nt((size_t)T * J);
         for (int t = 0; t < T; ++t)
             for (int j = 0; j < J; ++j) {
@@ -200,7 +167,7 @@ void one_cell(int n, int max_children, std::mt19937& rng) {
                 const double d2 = row ? 2.0 * (1.0 - (((double)a.x + (double)a.y) + ((double)a.z + (double)a.w))) : 0.0;
                 const double tl = j > 0 ? reg_t[q - 1] : lane_prev(reg_t, t, J - 1);
                 const double tr = j < J - 1 ? reg_t[q + 1] : lane_next(reg_t, t, 0);
-                const double vp = j > 0 ? port[q - 1] : lane_prev(port, t, J - 1);
+                const double vp = pairs ? second[q] : j > 0 ? port[q - 1] : lane_prev(port, t, J - 1);
                 double acc = k == 1 ? d2 * reg_t[q] : std::fma(d2, reg_t[q], -reg_u[q]);
                 acc = std::fma(cl, tl, acc);
                 acc = std::fma(cr, tr, acc);
@@ -226,6 +193,62 @@ void one_cell(int n, int max_children, std::mt19937& rng) {
     }
     CHECK(err <= 1e-11 * std::max(1.0, mag), "n %d: emulated product differs by %g (|ref| %g)", n, err, mag);
     ++emulated;
+}
+
+void one_cell(int n, int max_children, std::mt19937& rng) {
+    std::vector<int32_t> parent = random_tree(n, max_children, rng);
+    std::uniform_real_distribution<double> u(0.5, 2.0);
+    std::vector<double> R(n), C(n);
+    for (int i = 0; i < n; ++i) {
+        R[i] = 1e6 * u(rng);
+        C[i] = 1e-11 * u(rng);
+    }
+    // time steps of about 20 and about 100 series terms
+    const double dt_s = (n % 2 ? 0.05e-3 : 1e-3);
+    std::vector<std::vector<std::pair<int, double>>> adj(n);
+    std::vector<double> diag(n, 0.0), off(n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        const int q = parent[c];
+        if (q < 0) continue;
+        const double bc = dt_s / (R[c] * C[c]), bq = dt_s / (R[c] * C[q]);
+        adj[c].push_back({q, bc});
+        adj[q].push_back({c, bq});
+        diag[c] += bc;
+        diag[q] += bq;
+        off[c] = -dt_s / (R[c] * std::sqrt(C[c] * C[q]));
+    }
+    std::vector<double> work;
+    const double rho = nw::tree_spectral_radius(n, parent.data(), diag.data(), off.data(), work);
+    double dmax = 0.0;
+    for (double d : diag) dmax = std::max(dmax, d);
+    CHECK(n == 1 || (rho >= dmax && rho <= 2.0 * dmax * (1.0 + 1e-7)),
+          "n %d: spectral radius %g outside [max diag %g, 2 max diag]", n, rho, dmax);
+    std::vector<double> coef;
+    CHECK(nw::exp_chebyshev(rho, nw::kChebTol, coef) == 0, "n %d: no series for rho %g", n, rho);
+    double sum = 0.0;
+    for (double c : coef) sum += c;
+    CHECK(std::fabs(sum - 1.0) < 1e-13, "n %d: series coefficients sum to %.17g", n, sum);
+    int deg = 0;
+    for (const auto& a : adj) deg = std::max<int>(deg, (int)a.size());
+    if (deg > 3) return;  // such cells keep the natural order (no slot layout)
+    const double scale = rho > 0.0 ? 2.0 / rho : 0.0;
+    {  // the port layout (two or more row groups)
+        std::vector<uint16_t> pos(n);
+        std::vector<uint32_t> far;
+        std::vector<float4> cf;
+        int J = 0;
+        const int size = nw::preorder_layout(n, adj, scale, J, pos, far, cf);
+        if (size < 0) CHECK(n > kChebRows * kChebBlock - 64, "n %d: no port layout", n);
+        else layout_check(n, adj, scale, coef, J, size, pos, far, cf, false, rng);
+    }
+    if (n <= kChebBlock - 64) {  // the one-row-group layout (two reads per row)
+        std::vector<uint16_t> pos(n);
+        std::vector<uint32_t> far;
+        std::vector<float4> cf;
+        const int size = nw::preorder_layout_pairs(n, adj, scale, pos, far, cf);
+        CHECK(size > 0, "n %d: no one-row-group layout", n);
+        layout_check(n, adj, scale, coef, 1, size, pos, far, cf, true, rng);
+    }
 }
 
 // Poisson weights of the uniformization build: they sum to one, their mean is sigma, and each
