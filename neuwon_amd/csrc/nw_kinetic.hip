@@ -198,10 +198,12 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* sbase(T* p) {
 }
 
 // CONSERVE (codegen.py:95-102), the states and the conductance of instance base + o from out
-// (every address a wavefront-uniform base plus the lane's 32-bit offset).
-template <int S>
+// (every address a wavefront-uniform base plus the lane's 32-bit offset); gmax: the instance's
+// maximal conductance (read with the tile's states, before its stores).  The weights are read
+// through the constant address space (scalar loads: the kernel never writes them).
+template <int S, bool COND>
 __device__ __forceinline__ void finish_lane(const LtiGroupArgs& g, int64_t base, uint32_t o,
-                                            double (&out)[S]) {
+                                            double (&out)[S], float gmax) {
     if (g.conserve_sum > 0.0f) {
         double tot = 0.0;
 #pragma unroll
@@ -213,11 +215,12 @@ __device__ __forceinline__ void finish_lane(const LtiGroupArgs& g, int64_t base,
     const int64_t stride = opaque(g.stride);
 #pragma unroll
     for (int k = 0; k < S; ++k) sbase(g.states + k * stride + base)[o] = (float)out[k];
-    if (g.cond_out) {
+    if constexpr (COND) {
         double w = 0.0;
+        const auto wt = (const __attribute__((address_space(4))) float*)g.weights;
 #pragma unroll
-        for (int k = 0; k < S; ++k) w = fma((double)g.weights[k], (double)(float)out[k], w);
-        sbase(g.cond_out + base)[o] = (float)((double)sbase(g.gmax + base)[o] * w);
+        for (int k = 0; k < S; ++k) w = fma((double)wt[k], (double)(float)out[k], w);
+        sbase(g.cond_out + base)[o] = (float)((double)gmax * w);
     }
 }
 
@@ -268,15 +271,15 @@ __device__ __forceinline__ void tile_range(const LtiGroupArgs& g, int tile, int6
 // group_max distinct buckets makes one pass per bucket: the bucket's two node matrices by scalar
 // loads (FP64 FMAs with SGPR operands, shared by the lane's P instances), the instances of the
 // bucket taking the result; other wavefronts gather per instance.
-template <int S, int P>
-__global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
+template <int S, int P, bool COND>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_lti_grouped(LtiGroupArgs g) {
     constexpr int kTile = 256 * P;
     const int nb = g.num_nodes - 1;
     int64_t tile, tile_end;
     tile_range(g, kTile, tile, tile_end);
     if (tile >= tile_end) return;
     const uint32_t lane0 = (threadIdx.x >> 6) * 64 * P + (threadIdx.x & 63);
-    float v_n[P], x_n[P][S];
+    float v_n[P], x_n[P][S], gm_n[P];
     auto load = [&](int64_t t) {
         const int64_t stride = opaque(g.stride);
         const int64_t base = (t < tile_end ? t : tile_end - 1) * kTile;
@@ -288,6 +291,7 @@ __global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
                                 : sbase(g.voltage + base)[o];
 #pragma unroll
             for (int k = 0; k < S; ++k) x_n[p][k] = sbase(g.states + k * stride + base)[o];
+            gm_n[p] = COND ? sbase(g.gmax + base)[o] : 0.0f;
         }
     };
     load(tile);
@@ -297,9 +301,11 @@ __global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
         bool active[P];
         int b[P];
         double f[P], x[P][S];
+        float gm[P];
         uint64_t lanes[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
+            gm[p] = gm_n[p];
             active[p] = (int64_t)(lane0 + 64 * p) < left;
             b[p] = -1;
             f[p] = 0.0;
@@ -311,8 +317,13 @@ __global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
         load(tile + 1);
         if (distinct_buckets<P>(lanes, b, g.group_max) <= g.group_max) {
             uint64_t rem[P];
+            double fin[P][S];
 #pragma unroll
-            for (int p = 0; p < P; ++p) rem[p] = lanes[p];
+            for (int p = 0; p < P; ++p) {
+                rem[p] = lanes[p];
+#pragma unroll
+                for (int k = 0; k < S; ++k) fin[p][k] = 0.0;
+            }
             for (;;) {
                 const int u = next_bucket<P>(rem, b);
                 if (u < 0) break;
@@ -341,16 +352,22 @@ __global__ void __launch_bounds__(256) k_lti_grouped(LtiGroupArgs g) {
                         for (int k = 0; k < S; ++k) asm volatile("" : "+v"(out[p][k]));
 #pragma unroll
                     for (int p = 0; p < P; ++p)
-                        if (mine[p]) finish_lane<S>(g, base, lane0 + 64 * p, out[p]);
+#pragma unroll
+                        for (int k = 0; k < S; ++k) fin[p][k] = mine[p] ? out[p][k] : fin[p][k];
                 }
             }
+            // every lane's result stored once, after the passes, with no branch around the
+            // stores (lanes past the instances are masked)
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                if (active[p]) finish_lane<S, COND>(g, base, lane0 + 64 * p, fin[p], gm[p]);
         } else {
 #pragma unroll
             for (int p = 0; p < P; ++p)
                 if (active[p]) {
                     double out[S];
                     gather_product<S>(g, b[p], f[p], x[p], out);
-                    finish_lane<S>(g, base, lane0 + 64 * p, out);
+                    finish_lane<S, COND>(g, base, lane0 + 64 * p, out, gm[p]);
                 }
         }
     }
@@ -589,7 +606,8 @@ int nw_lti_table_advance(const nw_lti_table* t, int64_t M, const float* voltage,
 #define NW_LTI_G(K)                                                               \
     case K: {                                                                     \
         constexpr int P = lti_instances_per_lane<K>();                            \
-        rc = launch(k_lti_grouped<K, P>, 256 * P);                                \
+        rc = cond_out ? launch(k_lti_grouped<K, P, true>, 256 * P)                \
+                      : launch(k_lti_grouped<K, P, false>, 256 * P);              \
         break;                                                                    \
     }
     switch (t->S) {
