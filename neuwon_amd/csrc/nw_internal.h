@@ -630,14 +630,13 @@ __device__ __forceinline__ void cheb_store_terms(double* v, const double (&u)[J]
 
 // One term: t holds T_{k-1} of the thread's slots, u holds T_{k-2} and receives T_k (the two
 // arrays swap roles from term to term, so no register is copied); the ports read buffer RD,
-// T_k goes to the other buffer, out += c_k T_k, then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
+// T_k goes to the other buffer, out += ck T_k (ck = c_k), then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
 // (wave-uniform): the wave holds a slot in use; a wave without one only keeps the barriers.
 // The last row group's port is read first: row 0 needs it (through lane l - 1).
 template <int J, typename CT, int RD, bool FIRST, int BUF>
 __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R, const double (&t)[J],
-                                          double (&u)[J], bool active, int k, double (&out)[kChebRows]) {
+                                          double (&u)[J], bool active, double ck, double (&out)[kChebRows]) {
     if (active) {
-        const double ck = lds[2 * BUF + k];
         const char* rd = reinterpret_cast<const char*>(lds + RD * BUF);
         double* wr = lds + (1 - RD) * BUF;
         double vf[J];
@@ -714,13 +713,15 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
 #pragma unroll
     for (int j = 0; j < J; ++j) out[j] = c0 * x[j];
     // x = T_0, then y = T_1, x = T_2, y = T_3, ...
-    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, 1, out);
+    // the series coefficients of terms k, k + 1 in one 16-byte read (k even)
+    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, lds[2 * BUF + 1], out);
     int k = 2;
     for (; k + 1 <= K; k += 2) {
-        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
-        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, k + 1, out);
+        const double2 ck = *reinterpret_cast<const double2*>(lds + 2 * BUF + k);
+        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, ck.x, out);
+        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, ck.y, out);
     }
-    if (k <= K) cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, k, out);
+    if (k <= K) cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, lds[2 * BUF + k], out);
 }
 
 // The product for one cell, preorder slots (base >= 0) or natural order, from x in LDS (at the
