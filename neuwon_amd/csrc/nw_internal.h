@@ -630,12 +630,17 @@ __device__ __forceinline__ void cheb_store_terms(double* v, const double (&u)[J]
 
 // One term: t holds T_{k-1} of the thread's slots, u holds T_{k-2} and receives T_k (the two
 // arrays swap roles from term to term, so no register is copied); the ports read buffer RD,
-// T_k goes to the other buffer, out += ck T_k (ck = c_k), then a barrier.  FIRST: T_1 = A~ T_0.  ACTIVE
-// (wave-uniform): the wave holds a slot in use; a wave without one only keeps the barriers.
-// The last row group's port is read first: row 0 needs it (through lane l - 1).
+// T_k goes to the other buffer, out += ck T_k (ck = c_k on entry, c_knext on return), then a
+// barrier.  FIRST: T_1 = A~ T_0.  ACTIVE (wave-uniform): the wave holds a slot in use; a wave
+// without one only keeps the barriers.  LDS order: the next term's coefficient first, then the
+// last row group's port (row 0 needs it, through lane l - 1) and the others; rows are formed in
+// their store pairs and each pair is stored as soon as it is formed, since the last stores of
+// the workgroup gate the barrier.
 template <int J, typename CT, int RD, bool FIRST, int BUF>
 __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R, const double (&t)[J],
-                                          double (&u)[J], bool active, double ck, double (&out)[kChebRows]) {
+                                          double (&u)[J], bool active, double& ck, int knext,
+                                          double (&out)[kChebRows]) {
+    const double cnext = lds[2 * BUF + knext];  // the next term's coefficient, read ahead
     if (active) {
         const char* rd = reinterpret_cast<const char*>(lds + RD * BUF);
         double* wr = lds + (1 - RD) * BUF;
@@ -648,6 +653,7 @@ __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R,
         // one row group: the row's own second read; otherwise the previous slot's port
         const double port_before = J == 1 ? *reinterpret_cast<const double*>(rd + R.g)
                                           : lane_before(vf[J - 1]);
+        // rows in pairs, each pair stored as soon as it is formed (the stores gate the barrier)
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const double tl = j > 0 ? t[j - 1] : before;
@@ -659,11 +665,18 @@ __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R,
             acc = fma((double)R.co[j], vf[j], acc);
             acc = fma((double)R.cp[j], vp, acc);
             u[j] = FIRST ? 0.5 * acc : acc;
+            if ((j & 1) && (j >> 1) < (J >> 1)) {
+                *reinterpret_cast<double2*>(wr + (j >> 1) * 2 * kChebBlock + 2 * threadIdx.x) = make_double2(u[j - 1], u[j]);
+                asm volatile("" : "+v"(u[j - 1]), "+v"(u[j]) : : "memory");
+            } else if (j == J - 1 && (J & 1)) {
+                wr[(J / 2) * 2 * kChebBlock + threadIdx.x] = u[j];
+                asm volatile("" : "+v"(u[j]) : : "memory");
+            }
         }
-        cheb_store_terms<J>(wr, u);
 #pragma unroll
         for (int j = 0; j < J; ++j) out[j] = fma(ck, u[j], out[j]);
     }
+    ck = cnext;
     __syncthreads();
 }
 
@@ -713,15 +726,15 @@ __device__ __forceinline__ void cheb_pre_rows(const uint32_t* __restrict__ far, 
 #pragma unroll
     for (int j = 0; j < J; ++j) out[j] = c0 * x[j];
     // x = T_0, then y = T_1, x = T_2, y = T_3, ...
-    // the series coefficients of terms k, k + 1 in one 16-byte read (k even)
-    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, lds[2 * BUF + 1], out);
+    // each term reads the next term's coefficient before its own ports (index clamped to K)
+    double ck = lds[2 * BUF + 1];
+    if (K >= 1) cheb_term<J, CT, 0, true, BUF>(lds, R, x, y, active, ck, min(2, K), out);
     int k = 2;
     for (; k + 1 <= K; k += 2) {
-        const double2 ck = *reinterpret_cast<const double2*>(lds + 2 * BUF + k);
-        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, ck.x, out);
-        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, ck.y, out);
+        cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, ck, k + 1, out);
+        cheb_term<J, CT, 0, false, BUF>(lds, R, x, y, active, ck, min(k + 2, K), out);
     }
-    if (k <= K) cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, lds[2 * BUF + k], out);
+    if (k <= K) cheb_term<J, CT, 1, false, BUF>(lds, R, y, x, active, ck, K, out);
 }
 
 // The product for one cell, preorder slots (base >= 0) or natural order, from x in LDS (at the
