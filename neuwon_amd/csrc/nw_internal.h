@@ -633,9 +633,10 @@ __device__ __forceinline__ void cheb_store_terms(double* v, const double (&u)[J]
 // T_k goes to the other buffer, out += ck T_k (ck = c_k on entry, c_knext on return), then a
 // barrier.  FIRST: T_1 = A~ T_0.  ACTIVE (wave-uniform): the wave holds a slot in use; a wave
 // without one only keeps the barriers.  LDS order: the next term's coefficient first, then the
-// last row group's port (row 0 needs it, through lane l - 1) and the others; rows are formed in
-// their store pairs and each pair is stored as soon as it is formed, since the last stores of
-// the workgroup gate the barrier.
+// last row group's port (row 0 needs it, through lane l - 1), with odd J the one before it, then
+// the others; rows are formed in their store units (with odd J the lone last row group first,
+// then the pairs) and each unit is stored as soon as it is formed, since the last stores of the
+// workgroup gate the barrier.
 template <int J, typename CT, int RD, bool FIRST, int BUF>
 __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R, const double (&t)[J],
                                           double (&u)[J], bool active, double& ck, int knext,
@@ -646,16 +647,18 @@ __device__ __forceinline__ void cheb_term(double* lds, const ChebRegs<J, CT>& R,
         double* wr = lds + (1 - RD) * BUF;
         double vf[J];
         vf[J - 1] = *reinterpret_cast<const double*>(rd + R.f[J - 1]);
+        if (J & 1 && J > 1) vf[J - 2] = *reinterpret_cast<const double*>(rd + R.f[J - 2]);
 #pragma unroll
-        for (int j = 0; j < J - 1; ++j) vf[j] = *reinterpret_cast<const double*>(rd + R.f[j]);
+        for (int j = 0; j < ((J & 1 && J > 1) ? J - 2 : J - 1); ++j) vf[j] = *reinterpret_cast<const double*>(rd + R.f[j]);
         const double before = lane_before(t[J - 1]);
         const double after = lane_after(t[0]);
         // one row group: the row's own second read; otherwise the previous slot's port
         const double port_before = J == 1 ? *reinterpret_cast<const double*>(rd + R.g)
                                           : lane_before(vf[J - 1]);
-        // rows in pairs, each pair stored as soon as it is formed (the stores gate the barrier)
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
+        for (int q = 0; q < J; ++q) {
+            // odd J: the lone last row group first, then the pairs
+            const int j = (J & 1) ? (q == 0 ? J - 1 : q - 1) : q;
             const double tl = j > 0 ? t[j - 1] : before;
             const double tr = j < J - 1 ? t[j + 1] : after;
             const double vp = j > 0 ? vf[j - 1] : port_before;
