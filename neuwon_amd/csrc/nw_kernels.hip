@@ -292,6 +292,9 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 // of 1024-thread blocks.  Voltages pass between launches as FP64 (v64); the FP32 voltage array
 // is written after every step.
 // ----------------------------------------------------------------------------------------------
+// Gate table nodes the second half step's kernel keeps in LDS (3 gates x 2 FP32 x 2048 nodes,
+// 48 KB beside the 88 KB of the five-row-group term vectors); larger tables are read from memory.
+constexpr int kGateNodesLds = 2048;
 template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
 k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
@@ -315,6 +318,23 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     // (table gates) the gate table entries at each row's voltage, then the arithmetic.
     constexpr int R = MAXJ;
     constexpr int G = R < 3 ? R : 3;
+    // Gate tables in LDS (one workgroup per CU: the LDS beside the term vectors is free): the
+    // copy's loads are issued with the first group's, so each row's lookups are LDS reads instead
+    // of a second memory round trip per group.
+    constexpr bool kStage = HALF == 1 && GATE_MODE == 0 && MAXJ > 1 &&
+                            (prod_lds(MAXJ, KIND) + 3 * kGateNodesLds) * 8 <= 160 * 1024;
+    constexpr int kStageLoads = (3 * kGateNodesLds + NT - 1) / NT;
+    __shared__ __align__(16) float2 gtab[kStage ? 3 * kGateNodesLds : 1];
+    const bool stage = kStage && p.num_nodes <= kGateNodesLds;
+    float2 tcopy[kStage ? kStageLoads : 1];
+    if (stage) {
+        const float2* tsrc = reinterpret_cast<const float2*>(a.table);
+#pragma unroll
+        for (int q = 0; q < kStageLoads; ++q) {
+            const int e = threadIdx.x + q * NT;
+            tcopy[q] = e < 3 * p.num_nodes ? tsrc[e] : make_float2(0.f, 0.f);
+        }
+    }
 #pragma unroll
     for (int j0 = 0; j0 < R; j0 += G) {
         double v[G];
@@ -340,6 +360,14 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
         float2 tab[G][3][2];
         double frac[G];
         if (HALF == 1 && GATE_MODE == 0) {
+            if (kStage && j0 == 0 && stage) {
+#pragma unroll
+                for (int q = 0; q < kStageLoads; ++q) {
+                    const int e = threadIdx.x + q * NT;
+                    if (e < 3 * kGateNodesLds) gtab[e] = tcopy[q];
+                }
+            }
+            if (kStage && j0 == 0) __syncthreads();  // (stage is uniform over the workgroup)
             const int nb = p.num_nodes - 1;
             const double frq = (double)nb / ((double)p.vmax - (double)p.vmin);
             const float2* tm = reinterpret_cast<const float2*>(a.table);
@@ -350,10 +378,18 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
                 if (j < R && i < nrows) {
                     int bk;
                     bucket_of(v[jj], (double)p.vmin, frq, nb, bk, frac[jj]);
+                    if (stage) {
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        tab[jj][q][0] = tm[q * p.num_nodes + bk];
-                        tab[jj][q][1] = tm[q * p.num_nodes + bk + 1];
+                        for (int q = 0; q < 3; ++q) {
+                            tab[jj][q][0] = gtab[q * p.num_nodes + bk];
+                            tab[jj][q][1] = gtab[q * p.num_nodes + bk + 1];
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            tab[jj][q][0] = tm[q * p.num_nodes + bk];
+                            tab[jj][q][1] = tm[q * p.num_nodes + bk + 1];
+                        }
                     }
                 }
             }
