@@ -237,6 +237,125 @@ class LTIScheme:
         return P.dot(np.full(self.num_states, self.conserve_sum / self.num_states))
 
 
+def resized(grid, num_buckets):
+    """The same input mapping with round(num_buckets) buckets."""
+    n = max(1, int(round(num_buckets)))
+    if isinstance(grid, LogarithmicInput):
+        return LogarithmicInput(grid.minimum, grid.maximum, n, grid.scale)
+    return LinearInput(grid.minimum, grid.maximum, n)
+
+
+def interpolation_error(scheme, inputs=None, samples_per_bucket=2, seed=0):
+    """Largest |P_table(x) - P(x)| of a table on `inputs` (default the scheme's), over random
+    inputs in every bucket: P(x) = expm(A(x) dt) with every column divided by its sum (as
+    ``make_matrix``), P_table the node propagators interpolated linearly (one input) or
+    bilinearly (two) at the input's place in its bucket, as the device kernels apply them.  The
+    reference measures its tables by the same statistic, the max abs difference over random
+    samples in every bucket (lti_sim/approx.py:33-60 MatrixSamples.sample, 183-201
+    measure_error).  Returns the maximum and the per-bucket maxima (shape: the bucket counts)."""
+    inputs = list(scheme.inputs if inputs is None else inputs)
+    assert len(inputs) == scheme.num_inputs
+    rng = np.random.default_rng(seed)
+    dt = scheme.time_step
+    mesh = np.meshgrid(*[g.nodes() for g in inputs], indexing="ij")
+    Pn = scipy.linalg.expm(scheme._rates(*mesh) * dt)
+    Pn /= Pn.sum(axis=-2, keepdims=True)
+    shape = tuple(g.num_buckets for g in inputs)
+    nb = int(np.prod(shape))
+    k = int(samples_per_bucket)
+    flat = np.repeat(np.arange(nb), k)
+    bidx = np.unravel_index(flat, shape)
+    loc = [rng.uniform(size=flat.size) for _ in inputs]
+    xs = [g.get_input_value(b + u) for g, b, u in zip(inputs, bidx, loc)]
+    Px = scipy.linalg.expm(scheme._rates(*xs) * dt)
+    Px /= Px.sum(axis=-2, keepdims=True)
+    if len(inputs) == 1:
+        b, u = bidx[0], loc[0][:, None, None]
+        Pi = Pn[b] + u * (Pn[b + 1] - Pn[b])
+    else:
+        b0, b1 = bidx
+        u0, u1 = loc[0][:, None, None], loc[1][:, None, None]
+        lo = Pn[b0, b1] + u1 * (Pn[b0, b1 + 1] - Pn[b0, b1])
+        hi = Pn[b0 + 1, b1] + u1 * (Pn[b0 + 1, b1 + 1] - Pn[b0 + 1, b1])
+        Pi = lo + u0 * (hi - lo)
+    per_bucket = np.abs(Pi - Px).max(axis=(-2, -1)).reshape(nb, k).max(axis=1).reshape(shape)
+    return float(per_bucket.max()), per_bucket
+
+
+def choose_grid(scheme, accuracy=1e-4, start=None, samples_per_bucket=None, seed=0,
+                max_buckets=200_000):
+    """Bucket counts of the scheme's inputs whose interpolated table meets `accuracy` (the max abs
+    error of ``interpolation_error``) with few buckets, by the reference's search for the
+    linear form the device kernels use (the reference also searches polynomial degrees by
+    timing its generated kernel: lti_sim/optimizer.py:152-192 one input, 213-263 two inputs):
+    grow the count by max(1.5, 1.7^log10(error / accuracy)) until the error is met (an error
+    that does not fall is an error), then shrink it by 0.9 while the error stays met.  With two
+    inputs each growth step tries both inputs x 1.5 and keeps the one with the lower error (both
+    when they are within 25 % of each other); each shrink step tries both x 0.9 and keeps the
+    lower error.  Returns (inputs, error)."""
+    acc = float(accuracy)
+    assert 0.0 < acc < 1.0
+    k = samples_per_bucket or (2 if scheme.num_inputs == 1 else 1)
+    grids = list(scheme.inputs)
+
+    def measure(counts):
+        if int(np.prod([max(1, round(c)) for c in counts])) > max_buckets:
+            raise RuntimeError(f"{scheme.name}: accuracy {acc:g} needs more than {max_buckets} "
+                               "buckets")
+        ins = [resized(g, c) for g, c in zip(grids, counts)]
+        return ins, interpolation_error(scheme, ins, k, seed)[0]
+
+    if scheme.num_inputs == 1:
+        n = float(start or 10)
+        ins, err = measure([n])
+        floor = 1
+        while err > acc:
+            floor = ins[0].num_buckets
+            n *= max(1.5, 1.7 ** math.log10(err / acc))
+            new_ins, new_err = measure([n])
+            if not new_err < err:
+                raise RuntimeError(f"{scheme.name}: the error stopped falling at {err:g}")
+            ins, err = new_ins, new_err
+        while True:
+            n *= 0.9
+            if round(n) <= floor:
+                break
+            new_ins, new_err = measure([n])
+            if new_err > acc:
+                break
+            ins, err = new_ins, new_err
+        return ins, err
+    n = [float(c) for c in (start or (10, 10))]
+    ins, err = measure(n)
+    while err > acc:
+        a_ins, a_err = measure([n[0] * 1.5, n[1]])
+        b_ins, b_err = measure([n[0], n[1] * 1.5])
+        diff = 2.0 * (a_err - b_err) / (a_err + b_err)
+        if diff < -0.25:
+            cand = ([n[0] * 1.5, n[1]], a_ins, a_err)
+        elif diff > 0.25:
+            cand = ([n[0], n[1] * 1.5], b_ins, b_err)
+        else:
+            both = [n[0] * 1.5, n[1] * 1.5]
+            cand = (both,) + measure(both)
+        if not cand[2] < err:
+            raise RuntimeError(f"{scheme.name}: the error stopped falling at {err:g}")
+        n, ins, err = cand
+    while True:
+        trials = [[n[0] * 0.9, n[1]], [n[0], n[1] * 0.9]]
+        # a trial must change a rounded count (and keep every count at one bucket or more)
+        trials = [t for t in trials if min(round(t[0]), round(t[1])) >= 1 and
+                  (round(t[0]), round(t[1])) != (round(n[0]), round(n[1]))]
+        if not trials:
+            break
+        results = [(t,) + measure(t) for t in trials]
+        t, t_ins, t_err = min(results, key=lambda r: r[2])
+        if t_err > acc:
+            break
+        n, ins, err = t, t_ins, t_err
+    return ins, err
+
+
 def voltage_markov_scheme(edges, a, b, time_step=0.1, name="markov", conserve_sum=1.0,
                           num_states=None, weights=None):
     """Scheme with transitions i -> j at rate a_ij exp(b_ij v) (1/ms, v in mV).

@@ -1042,7 +1042,7 @@ class RxD_Model:
                  cytoplasmic_resistance=100, membrane_capacitance=1, extracellular_tortuosity=1.55,
                  extracellular_max_distance=20e-6, species=(), mechanisms=(), device=0,
                  prune=EPSILON, symmetric_propagator=False, step_graph=True,
-                 propagator_store=None):
+                 propagator_store=None, kinetic_table_accuracy=None):
         self.time_step = float(time_step)
         self.temperature = float(temperature)
         self.initial_voltage = float(initial_voltage)
@@ -1068,6 +1068,11 @@ class RxD_Model:
         # play back / recording / traces of device attributes), advance() replays a captured
         # graph of the step's kernels instead of launching them one by one from Python
         self.step_graph = bool(step_graph)
+        # kinetic_table_accuracy: the max abs propagator error the tables of kinetic .mod files
+        # are sized for (the reference sizes them for 1e-4, nmodl/__init__.py:304); unset: the
+        # fixed grids of nmodl.input_grid
+        self.kinetic_table_accuracy = (None if kinetic_table_accuracy is None
+                                       else float(kinetic_table_accuracy))
         self.device = int(device)
         self.database = db = Database()
         db._model = self
@@ -1176,7 +1181,8 @@ class RxD_Model:
                 from . import nmodl
                 rev = {name: kw["reversal_potential"] for name, kw in self.species.items()
                        if kw.get("reversal_potential") is not None}
-                mech = nmodl.kinetic_mechanism(m, self.time_step, self.temperature, rev).initialize(self)
+                mech = nmodl.kinetic_mechanism(m, self.time_step, self.temperature, rev,
+                                               accuracy=self.kinetic_table_accuracy).initialize(self)
             else:
                 raise ValueError(f"unsupported mechanism {m!r} (built-ins: {sorted(_BUILTIN)}; "
                                  "or the path of a kinetic .mod file)")

@@ -644,8 +644,10 @@ class KineticModel:
 
 # Input tables (nmodl/__init__.py:306-316 maps the voltage linearly over [-100, 100] mV and any
 # other input logarithmically over [0, 1000]).  The reference's optimizer searches bucket counts
-# and the log offset per file by timing (optimizer.py:85-263); these are fixed choices, sized for
-# linear interpolation errors of a few 1e-6 per step (tests/test_nmodl.py measures them).
+# per file for a max abs table error of 1e-4 (nmodl/__init__.py:304, optimizer.py:85-263); by
+# default these fixed counts are used instead, sized for linear interpolation errors of a few
+# 1e-6 per step (tests/test_nmodl.py measures them), and ``accuracy=`` runs the search
+# (channels.choose_grid) from them.
 LOG_SCALE = 1e-4
 BUCKETS_1D = {"v": 2000, "log": 2400}
 BUCKETS_2D = {"v": 400, "log": 600}
@@ -659,10 +661,12 @@ def input_grid(name, num_inputs=1):
     return channels.LogarithmicInput(0.0, 1000.0, buckets["log"], LOG_SCALE)
 
 
-def load_kinetic(path, time_step=0.1, celsius=37.0, grid=None, inputs=None):
+def load_kinetic(path, time_step=0.1, celsius=37.0, grid=None, inputs=None, accuracy=None):
     """The .mod file's kinetic scheme as an LTIScheme of its inputs (one or two: the membrane
     voltage and/or a concentration, sorted by name as the reference sorts them).  ``grid``
-    replaces the voltage grid of a voltage-only scheme; ``inputs`` gives every grid."""
+    replaces the voltage grid of a voltage-only scheme; ``inputs`` gives every grid;
+    ``accuracy`` (e.g. the reference's 1e-4) sizes every grid by ``channels.choose_grid``
+    instead, for that max abs error of the interpolated propagator."""
     km = KineticModel(path, celsius=celsius)
     if not 1 <= km.num_inputs <= 2:
         raise NmodlError(f"{km.name}: inputs {km.inputs}; the device kernel takes one or two")
@@ -674,19 +678,25 @@ def load_kinetic(path, time_step=0.1, celsius=37.0, grid=None, inputs=None):
     scheme = channels.LTIScheme(km.name, km.states, km.derivative, km.conserve_sum, time_step,
                                 inputs=inputs, input_names=km.inputs,
                                 rate_matrices=km.rate_matrices)
+    if accuracy is not None:
+        chosen, err = channels.choose_grid(scheme, accuracy)
+        scheme = channels.LTIScheme(km.name, km.states, km.derivative, km.conserve_sum,
+                                    time_step, inputs=chosen, input_names=km.inputs,
+                                    rate_matrices=km.rate_matrices)
+        scheme.table_error = err
     scheme.model = km
     return scheme
 
 
-def kinetic_mechanism(path, time_step, celsius, reversal_potentials, grid=None):
+def kinetic_mechanism(path, time_step, celsius, reversal_potentials, grid=None, accuracy=None):
     """A model mechanism (``model.LTIMechanism``) from a kinetic .mod file: the conductance of
     every state from BREAKPOINT, the reversal potential of the written ion from the model's
     species (or the file's PARAMETER default, or the reversal its nonspecific current implies).
     A POINT_PROCESS's conductance is in uS per unit occupancy and its magnitude is not scaled by
     the segment's area (nmodl/__init__.py:494-497); inputs other than the voltage are
-    attributes of each instance."""
+    attributes of each instance.  ``accuracy``: as in ``load_kinetic``."""
     from .model import LTIMechanism
-    scheme = load_kinetic(path, time_step, celsius, grid)
+    scheme = load_kinetic(path, time_step, celsius, grid, accuracy=accuracy)
     km = scheme.model
     ion = km.output_ion
     e_name = f"e{ion}" if ion else None

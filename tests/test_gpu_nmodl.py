@@ -54,10 +54,12 @@ def test_parsed_scheme_on_lti_kernel(torch_gpu):
     assert np.abs(got.sum(axis=0) - 1.0).max() < 1e-5
 
 
-def test_model_with_kinetic_mod_files(torch_gpu):
+@pytest.mark.parametrize("accuracy", [None, 1e-4])
+def test_model_with_kinetic_mod_files(torch_gpu, accuracy):
     """RxD_Model(mechanisms=[na6.mod, k4.mod]) on a soma and a six-segment dendrite, 0.3 nA for
     1 ms from t = 1 ms, dt 0.025 ms x 400 steps at 22 degC: every segment's voltage and the
-    channels' states against KineticSim (FP64) fed the same tables, geometry and stimulus."""
+    channels' states against KineticSim (FP64) fed the same tables, geometry and stimulus; the
+    tables on the fixed grids, or sized for the reference's 1e-4 (kinetic_table_accuracy)."""
     from neuwon_amd import channels, nmodl
     from neuwon_amd.model import RxD_Model, TimeSeries
     torch = torch_gpu
@@ -65,7 +67,11 @@ def test_model_with_kinetic_mod_files(torch_gpu):
     files = [os.path.join(OWN, "na6.mod"), os.path.join(OWN, "k4.mod")]
     species = [{"name": "na", "reversal_potential": 50.0}, {"name": "k", "reversal_potential": -85.0}]
     m = RxD_Model(time_step=dt, temperature=celsius, initial_voltage=-65.0, mechanisms=files,
-                  species=species)
+                  species=species, kinetic_table_accuracy=accuracy)
+    if accuracy is not None:
+        for name in ("na6", "k4"):
+            scheme = m.mechanisms[name].spec.scheme
+            assert scheme.inputs[0].num_buckets < 2000 and scheme.table_error <= accuracy
     root = m.Neuron([0, 0, 0], 15).root
     dend = root.add_section([120, 0, 0], 2, maximum_segment_length=20)
     segs = [root] + dend
@@ -95,7 +101,7 @@ def test_model_with_kinetic_mod_files(torch_gpu):
     P = O.propagator(parent, R, C, 0.5 * dt)
     mechs = []
     for f, ion in zip(files, ("na", "k")):
-        scheme = nmodl.load_kinetic(f, time_step=dt, celsius=celsius)
+        scheme = nmodl.load_kinetic(f, time_step=dt, celsius=celsius, accuracy=accuracy)
         km = scheme.model
         w = km.conductance_weights(f"e{ion}")
         init = scheme.initial_state(-65.0)
