@@ -786,9 +786,12 @@ def main():
     kern_s = kern_ms / 1e3
     value = N * K / wall
     free = pop.propagator.chebyshev
-    # kernel launches: two per step (one per half step) on the matrix-free stores, replayed from
-    # one graph per advance() call; one per call (all its steps) on the stored formats
-    launches = 2 * K if free else len(ev)
+    # kernel launches, replayed from one graph per advance() call on the matrix-free stores: one
+    # per step on the Chebyshev store built for five row groups (both half steps of a cell in one
+    # workgroup), two per step (one per half step) on the others; one per call (all its steps) on
+    # the stored formats
+    fused = free and not pop.propagator.rational and st["row_groups"] > 1
+    launches = (K if fused else 2 * K) if free else len(ev)
     avg_launch_s = kern_s / launches
     units_per_launch = N * K / launches          # compartment-steps per launch
     b_step = algorithmic_bytes(nnz_bar)
@@ -830,7 +833,8 @@ def main():
                 "frac": tflops / FP64_PEAK_TFLOPS,
                 "traffic": None if traffic is None else traffic * units_per_launch,
                 "traffic_source": traffic_src,
-                "kernel": "nw::k_hh_half_cheb (one launch per half step)",
+                "kernel": ("nw::k_hh_step_cheb (one launch per step)" if fused else
+                           "nw::k_hh_half_cheb (one launch per half step)"),
                 "avg_launch_ms": avg_launch_s * 1e3,
                 "flop_per_cmpt_step": flop,
                 "flop_per_launch": flop * units_per_launch}

@@ -286,19 +286,30 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 }
 
 // ----------------------------------------------------------------------------------------------
-// The step on the Chebyshev store: one launch per half step (HALF 0: relax with the conductances
-// of the previous mechanism step, then v = P x; HALF 1: gates at v(t + dt/2), relax, v = P x, AP
-// detector).  A launch holds one phase and one series, which keeps it within the 128 registers
-// of 1024-thread blocks.  Voltages pass between launches as FP64 (v64); the FP32 voltage array
-// is written after every step.
+// The step on the Chebyshev store, as two half steps (HALF 0: relax with the conductances of the
+// previous mechanism step, then v = P x; HALF 1: gates at v(t + dt/2), relax, v = P x, AP
+// detector), run one after the other by a cell's workgroup (k_hh_step_cheb: one launch per step)
+// or as one launch each (k_hh_half_cheb).  The two series are sequential code regions, which keeps
+// a launch within the 128 registers of 1024-thread blocks.  Voltages pass between steps (and
+// between half-step launches) as FP64 (v64); the FP32 voltage array is written after every step.
 // ----------------------------------------------------------------------------------------------
 // Gate table nodes the second half step's kernel keeps in LDS (3 gates x 2 FP32 x 2048 nodes,
 // 48 KB beside the 88 KB of the five-row-group term vectors); larger tables are read from memory.
 constexpr int kGateNodesLds = 2048;
-template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
-k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
-    __shared__ __align__(16) double b0[prod_lds(MAXJ, KIND)];
+// Whether the second half step of a launch shape keeps the gate tables in LDS.
+template <int GATE_MODE, int MAXJ, int KIND>
+constexpr bool gate_tables_in_lds() {
+    return GATE_MODE == 0 && MAXJ > 1 &&
+           (prod_lds(MAXJ, KIND) + 3 * kGateNodesLds) * 8 <= 160 * 1024;
+}
+
+// One half step of one cell (the body of k_hh_half_cheb / k_hh_step_cheb): b0 the product's LDS,
+// gtab the LDS of the gate tables (gate_tables_in_lds).
+// FUSED: the two halves run in one workgroup (k_hh_step_cheb): the first leaves v(t + dt/2) in
+// LDS at the rows' places, where the second reads it, instead of passing it through v64.
+template <int GATE_MODE, int HALF, int MAXJ, int NT, int KIND, bool FUSED = false>
+__device__ __forceinline__ void hh_half_cheb(const HHArgs& a, double* v64, int step, int from_v32,
+                                             double* b0, float2* gtab) {
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -321,10 +332,8 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
     // Gate tables in LDS (one workgroup per CU: the LDS beside the term vectors is free): the
     // copy's loads are issued with the first group's, so each row's lookups are LDS reads instead
     // of a second memory round trip per group.
-    constexpr bool kStage = HALF == 1 && GATE_MODE == 0 && MAXJ > 1 &&
-                            (prod_lds(MAXJ, KIND) + 3 * kGateNodesLds) * 8 <= 160 * 1024;
+    constexpr bool kStage = HALF == 1 && gate_tables_in_lds<GATE_MODE, MAXJ, KIND>();
     constexpr int kStageLoads = (3 * kGateNodesLds + NT - 1) / NT;
-    __shared__ __align__(16) float2 gtab[kStage ? 3 * kGateNodesLds : 1];
     const bool stage = kStage && p.num_nodes <= kGateNodesLds;
     float2 tcopy[kStage ? kStageLoads : 1];
     if (stage) {
@@ -346,7 +355,10 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
             const int i = threadIdx.x + j * kChebBlock;
             if (j < R && i < nrows) {
                 const int64_t gi = row0 + i;
-                v[jj] = from_v32 ? (double)a.v[gi] : v64[gi];
+                if (FUSED && HALF == 1)
+                    v[jj] = b0[pre ? (int)pos[i] : i];
+                else
+                    v[jj] = from_v32 ? (double)a.v[gi] : v64[gi];
                 fm[jj] = a.m[gi];
                 fh[jj] = a.h[gi];
                 fn[jj] = a.n[gi];
@@ -462,7 +474,7 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
         if (i >= nrows) continue;
         const int64_t gi = row0 + i;
         const double vi = b0[pre ? (int)pos[i] : i];
-        v64[gi] = vi;
+        if (!(FUSED && HALF == 0)) v64[gi] = vi;
         if (HALF == 1) a.v[gi] = (float)vi;
         if (closes) {  // the state after the last step closes the record
             float* rec = a.record + (int64_t)a.n_steps * 5 * NR + gi;
@@ -484,6 +496,30 @@ k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
             }
         }
     }
+}
+
+template <int GATE_MODE, int HALF, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+k_hh_half_cheb(HHArgs a, double* v64, int step, int from_v32) {
+    __shared__ __align__(16) double b0[prod_lds(MAXJ, KIND)];
+    constexpr bool kTab = HALF == 1 && gate_tables_in_lds<GATE_MODE, MAXJ, KIND>();
+    __shared__ __align__(16) float2 gtab[kTab ? 3 * kGateNodesLds : 1];
+    hh_half_cheb<GATE_MODE, HALF, MAXJ, NT, KIND>(a, v64, step, from_v32, b0, gtab);
+}
+
+// Both half steps of one step in one launch: the workgroup of a cell runs its first half step,
+// then its second (every cell's two halves depend on that cell alone), so the second half step's
+// loads of the cell's segments and store find them in the L2, v(t + dt/2) stays in LDS, and a
+// step costs one launch.
+template <int GATE_MODE, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+k_hh_step_cheb(HHArgs a, double* v64, int step, int from_v32) {
+    __shared__ __align__(16) double b0[prod_lds(MAXJ, KIND)];
+    constexpr bool kTab = gate_tables_in_lds<GATE_MODE, MAXJ, KIND>();
+    __shared__ __align__(16) float2 gtab[kTab ? 3 * kGateNodesLds : 1];
+    hh_half_cheb<GATE_MODE, 0, MAXJ, NT, KIND, true>(a, v64, step, from_v32, b0, gtab);
+    __syncthreads();
+    hh_half_cheb<GATE_MODE, 1, MAXJ, NT, KIND, true>(a, v64, step, 0, b0, gtab);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1355,9 +1391,16 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
             auto launch = [&](auto gate, auto maxj, auto nt, auto kind) {
                 constexpr int G = decltype(gate)::value, MJ = decltype(maxj)::value;
                 constexpr int T = decltype(nt)::value, KD = decltype(kind)::value;
+                // one launch per step where a workgroup holds the CU (the Chebyshev store's five
+                // row groups); one per half step where two to four workgroups share it (one row
+                // group) or where the fused build would spill (the rational store)
                 for (int st = 0; st < n_steps; ++st) {
-                    k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
-                    k_hh_half_cheb<G, 1, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, 0);
+                    if constexpr (MJ > 1 && KD == kProdChebyshev) {
+                        k_hh_step_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
+                    } else {
+                        k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
+                        k_hh_half_cheb<G, 1, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, 0);
+                    }
                 }
             };
             auto by_shape = [&](auto gate, auto kind) {
