@@ -787,11 +787,11 @@ def main():
     value = N * K / wall
     free = pop.propagator.chebyshev
     # kernel launches, replayed from one graph per advance() call on the matrix-free stores: one
-    # per step on the Chebyshev store built for five row groups (both half steps of a cell in one
-    # workgroup), two per step (one per half step) on the others; one per call (all its steps) on
-    # the stored formats
+    # per two steps on the Chebyshev store built for five row groups (a cell's workgroup runs both
+    # steps; an odd call's last step alone), two per step (one per half step) on the others; one
+    # per call (all its steps) on the stored formats
     fused = free and not pop.propagator.rational and st["row_groups"] > 1
-    launches = (K if fused else 2 * K) if free else len(ev)
+    launches = (sum((k + 1) // 2 for k, _, _ in ev) if fused else 2 * K) if free else len(ev)
     avg_launch_s = kern_s / launches
     units_per_launch = N * K / launches          # compartment-steps per launch
     b_step = algorithmic_bytes(nnz_bar)
@@ -833,7 +833,7 @@ def main():
                 "frac": tflops / FP64_PEAK_TFLOPS,
                 "traffic": None if traffic is None else traffic * units_per_launch,
                 "traffic_source": traffic_src,
-                "kernel": ("nw::k_hh_step_cheb (one launch per step)" if fused else
+                "kernel": ("nw::k_hh_step2_cheb (one launch per two steps)" if fused else
                            "nw::k_hh_half_cheb (one launch per half step)"),
                 "avg_launch_ms": avg_launch_s * 1e3,
                 "flop_per_cmpt_step": flop,

@@ -307,9 +307,9 @@ typedef struct nw_hh_params {
 } nw_hh_params;
 
 /* launch_mode of nw_hh_step on the matrix-free (Chebyshev, rational) stores, which run one launch
- * per step (a Chebyshev store built for five row groups: each cell's workgroup runs both half
- * steps) or one per half step (the others):
- *   NW_LAUNCH_DIRECT         the n_steps or 2 n_steps launches are issued one by one on `stream`;
+ * per two steps (a Chebyshev store built for five row groups: each cell's workgroup runs both
+ * steps; an odd n_steps' last step alone) or one per half step (the others):
+ *   NW_LAUNCH_DIRECT         the launches are issued one by one on `stream`;
  *   NW_LAUNCH_GRAPH          they are replayed as one HIP graph launch on `stream`: a graph is
  *                            captured (on a stream of the handle's own) the first time an argument
  *                            set is seen -- every argument but prm->step0, which the kernels read

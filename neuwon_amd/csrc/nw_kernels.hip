@@ -289,8 +289,8 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 // The step on the Chebyshev store, as two half steps (HALF 0: relax with the conductances of the
 // previous mechanism step, then v = P x; HALF 1: gates at v(t + dt/2), relax, v = P x, AP
 // detector), run one after the other by a cell's workgroup (k_hh_step_cheb: one launch per step)
-// or as one launch each (k_hh_half_cheb).  The two series are sequential code regions, which keeps
-// a launch within the 128 registers of 1024-thread blocks.  Voltages pass between steps (and
+// or as one launch each (k_hh_half_cheb); k_hh_step2_cheb runs two steps.  The series are
+// sequential code regions, which keeps a launch within the 128 registers of 1024-thread blocks.  Voltages pass between steps (and
 // between half-step launches) as FP64 (v64); the FP32 voltage array is written after every step.
 // ----------------------------------------------------------------------------------------------
 // Gate table nodes the second half step's kernel keeps in LDS (3 gates x 2 FP32 x 2048 nodes,
@@ -305,11 +305,15 @@ constexpr bool gate_tables_in_lds() {
 
 // One half step of one cell (the body of k_hh_half_cheb / k_hh_step_cheb): b0 the product's LDS,
 // gtab the LDS of the gate tables (gate_tables_in_lds).
-// FUSED: the two halves run in one workgroup (k_hh_step_cheb): the first leaves v(t + dt/2) in
-// LDS at the rows' places, where the second reads it, instead of passing it through v64.
-template <int GATE_MODE, int HALF, int MAXJ, int NT, int KIND, bool FUSED = false>
+// FUSED: the two halves run in one workgroup (k_hh_step_cheb, k_hh_step2_cheb): the first leaves
+// v(t + dt/2) in LDS at the rows' places, where the second reads it, instead of passing it through
+// v64.  RUN (k_hh_step2_cheb, two steps in one workgroup): the second half leaves v(t + dt) in
+// LDS for the next step's first half (v_lds), and the voltage arrays are written after the
+// launch's last step only (last).
+template <int GATE_MODE, int HALF, int MAXJ, int NT, int KIND, bool FUSED = false, bool RUN = false>
 __device__ __forceinline__ void hh_half_cheb(const HHArgs& a, double* v64, int step, int from_v32,
-                                             double* b0, float2* gtab) {
+                                             double* b0, float2* gtab, bool v_lds = false,
+                                             bool last = true) {
     const nw_hh_params& p = a.prm;
     const int cell = a.order[blockIdx.x];
     const int64_t row0 = a.cell_row0[cell];
@@ -355,7 +359,7 @@ __device__ __forceinline__ void hh_half_cheb(const HHArgs& a, double* v64, int s
             const int i = threadIdx.x + j * kChebBlock;
             if (j < R && i < nrows) {
                 const int64_t gi = row0 + i;
-                if (FUSED && HALF == 1)
+                if ((FUSED && HALF == 1) || (RUN && v_lds))
                     v[jj] = b0[pre ? (int)pos[i] : i];
                 else
                     v[jj] = from_v32 ? (double)a.v[gi] : v64[gi];
@@ -474,8 +478,8 @@ __device__ __forceinline__ void hh_half_cheb(const HHArgs& a, double* v64, int s
         if (i >= nrows) continue;
         const int64_t gi = row0 + i;
         const double vi = b0[pre ? (int)pos[i] : i];
-        if (!(FUSED && HALF == 0)) v64[gi] = vi;
-        if (HALF == 1) a.v[gi] = (float)vi;
+        if (!(FUSED && HALF == 0) && (!RUN || last)) v64[gi] = vi;
+        if (HALF == 1 && (!RUN || last)) a.v[gi] = (float)vi;
         if (closes) {  // the state after the last step closes the record
             float* rec = a.record + (int64_t)a.n_steps * 5 * NR + gi;
             rec[0] = (float)vi;
@@ -520,6 +524,24 @@ k_hh_step_cheb(HHArgs a, double* v64, int step, int from_v32) {
     hh_half_cheb<GATE_MODE, 0, MAXJ, NT, KIND, true>(a, v64, step, from_v32, b0, gtab);
     __syncthreads();
     hh_half_cheb<GATE_MODE, 1, MAXJ, NT, KIND, true>(a, v64, step, 0, b0, gtab);
+}
+
+// Two steps of every cell in one launch: four half steps in sequence, v(t) in LDS between them.
+// (A loop over all of a call's steps in one launch keeps what the compiler hoists out of the loop
+// live across both series: 67-88 VGPRs spilled; two steps written out spill 12.)
+template <int GATE_MODE, int MAXJ, int NT = kChebBlock, int KIND = kProdAny>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MAXJ == 1 ? 8 : 4)))
+k_hh_step2_cheb(HHArgs a, double* v64, int step, int from_v32) {
+    __shared__ __align__(16) double b0[prod_lds(MAXJ, KIND)];
+    constexpr bool kTab = gate_tables_in_lds<GATE_MODE, MAXJ, KIND>();
+    __shared__ __align__(16) float2 gtab[kTab ? 3 * kGateNodesLds : 1];
+    hh_half_cheb<GATE_MODE, 0, MAXJ, NT, KIND, true, true>(a, v64, step, from_v32, b0, gtab, false, false);
+    __syncthreads();
+    hh_half_cheb<GATE_MODE, 1, MAXJ, NT, KIND, true, true>(a, v64, step, 0, b0, gtab, true, false);
+    __syncthreads();
+    hh_half_cheb<GATE_MODE, 0, MAXJ, NT, KIND, true, true>(a, v64, step + 1, 0, b0, gtab, true, true);
+    __syncthreads();
+    hh_half_cheb<GATE_MODE, 1, MAXJ, NT, KIND, true, true>(a, v64, step + 1, 0, b0, gtab, true, true);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1391,10 +1413,16 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
             auto launch = [&](auto gate, auto maxj, auto nt, auto kind) {
                 constexpr int G = decltype(gate)::value, MJ = decltype(maxj)::value;
                 constexpr int T = decltype(nt)::value, KD = decltype(kind)::value;
-                // one launch per step where a workgroup holds the CU (the Chebyshev store's five
-                // row groups); one per half step where two to four workgroups share it (one row
-                // group) or where the fused build would spill (the rational store)
-                for (int st = 0; st < n_steps; ++st) {
+                // two steps per launch (the last step of an odd count alone) where a workgroup
+                // holds the CU (the Chebyshev store's five row groups); one launch per half step
+                // where two to four workgroups share it (one row group) or where the fused build
+                // would spill (the rational store)
+                int st = 0;
+                if constexpr (MJ > 1 && KD == kProdChebyshev) {
+                    for (; st + 2 <= n_steps; st += 2)
+                        k_hh_step2_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
+                }
+                for (; st < n_steps; ++st) {
                     if constexpr (MJ > 1 && KD == kProdChebyshev) {
                         k_hh_step_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
                     } else {

@@ -116,11 +116,13 @@ def test_chebyshev_population_against_oracle(torch_gpu):
     assert np.abs(ref_v - v_end).max() < max(2e-3, 3.0 * floor_end)
 
 
-def test_chebyshev_cell_sizes_across_row_groups(torch_gpu):
+@pytest.mark.parametrize("steps", [200, 201])
+def test_chebyshev_cell_sizes_across_row_groups(torch_gpu, steps):
     """Cells of 1, 2, 3, 63, 64, 65, 1023, 1024, 1025 and 2049 segments in one population (one,
     two and three row groups per thread, rows on either side of the wave and workgroup
-    boundaries, layouts with empty slots), 0.5 nA constant current at every root, 200 steps:
-    every segment against the FP64 oracle with the unpruned propagator."""
+    boundaries, layouts with empty slots), 0.5 nA constant current at every root, 200 steps (two
+    per launch) or 201 (the last one alone): every segment against the FP64 oracle with the
+    unpruned propagator."""
     import scipy.sparse
     from neuwon_amd import population, synth
     torch = torch_gpu
@@ -128,7 +130,7 @@ def test_chebyshev_cell_sizes_across_row_groups(torch_gpu):
     sizes = [1, 2, 3, 63, 64, 65, 1023, 1024, 1025, 2049]
     cells = [synth.random_cell(n, rng) for n in sizes]
     pop = population.HHPopulation(cells, store="chebyshev", stim={"mode": "constant", "amp_na": 0.5})
-    pop.advance(200)
+    pop.advance(steps)
     torch.cuda.synchronize()
     v = pop.v.cpu().numpy().astype(np.float64)
     blocks = []
@@ -139,7 +141,7 @@ def test_chebyshev_cell_sizes_across_row_groups(torch_gpu):
     P = scipy.sparse.block_diag(blocks, format="csr")
     co = O.CHHPopulation(pop.cell_ptr, P, pop.C, pop.g_bar, dt=pop.dt, celsius=pop.celsius,
                          gate_mode="table", stim=("constant", 0.5e-9))
-    co.advance(200, threads=len(os.sched_getaffinity(0)))
+    co.advance(steps, threads=len(os.sched_getaffinity(0)))
     rel = np.abs(v - co.v) / np.abs(co.v)
     per = [float("%.1e" % rel[int(pop.cell_ptr[c]):int(pop.cell_ptr[c + 1])].max())
            for c in range(pop.num_cells)]
