@@ -786,12 +786,15 @@ def main():
     kern_s = kern_ms / 1e3
     value = N * K / wall
     free = pop.propagator.chebyshev
-    # kernel launches, replayed from one graph per advance() call on the matrix-free stores: one
-    # per two steps on the Chebyshev store built for five row groups (a cell's workgroup runs both
-    # steps; an odd call's last step alone), two per step (one per half step) on the others; one
-    # per call (all its steps) on the stored formats
-    fused = free and not pop.propagator.rational and st["row_groups"] > 1
-    launches = (sum((k + 1) // 2 for k, _, _ in ev) if fused else 2 * K) if free else len(ev)
+    # kernel launches, replayed from one graph per advance() call on the matrix-free stores: on
+    # the Chebyshev store one per two steps when built for five row groups (a cell's workgroup
+    # runs both steps; an odd call's last step alone), one per step when built for one; two per
+    # step (one per half step) on the rational store; one per call (all its steps) on the stored
+    # formats
+    cheb = free and not pop.propagator.rational
+    fused = cheb and st["row_groups"] > 1
+    launches = ((sum((k + 1) // 2 for k, _, _ in ev) if fused else K if cheb else 2 * K)
+                if free else len(ev))
     avg_launch_s = kern_s / launches
     units_per_launch = N * K / launches          # compartment-steps per launch
     b_step = algorithmic_bytes(nnz_bar)
@@ -834,6 +837,7 @@ def main():
                 "traffic": None if traffic is None else traffic * units_per_launch,
                 "traffic_source": traffic_src,
                 "kernel": ("nw::k_hh_step2_cheb (one launch per two steps)" if fused else
+                           "nw::k_hh_step_cheb (one launch per step)" if cheb else
                            "nw::k_hh_half_cheb (one launch per half step)"),
                 "avg_launch_ms": avg_launch_s * 1e3,
                 "flop_per_cmpt_step": flop,

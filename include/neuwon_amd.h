@@ -308,7 +308,8 @@ typedef struct nw_hh_params {
 
 /* launch_mode of nw_hh_step on the matrix-free (Chebyshev, rational) stores, which run one launch
  * per two steps (a Chebyshev store built for five row groups: each cell's workgroup runs both
- * steps; an odd n_steps' last step alone) or one per half step (the others):
+ * steps; an odd n_steps' last step alone), one per step (a Chebyshev store of one row group) or
+ * one per half step (the rational store):
  *   NW_LAUNCH_DIRECT         the launches are issued one by one on `stream`;
  *   NW_LAUNCH_GRAPH          they are replayed as one HIP graph launch on `stream`: a graph is
  *                            captured (on a stream of the handle's own) the first time an argument

@@ -1413,17 +1413,17 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
             auto launch = [&](auto gate, auto maxj, auto nt, auto kind) {
                 constexpr int G = decltype(gate)::value, MJ = decltype(maxj)::value;
                 constexpr int T = decltype(nt)::value, KD = decltype(kind)::value;
-                // two steps per launch (the last step of an odd count alone) where a workgroup
-                // holds the CU (the Chebyshev store's five row groups); one launch per half step
-                // where two to four workgroups share it (one row group) or where the fused build
-                // would spill (the rational store)
+                // the Chebyshev store: two steps per launch (the last step of an odd count
+                // alone) where a workgroup holds the CU (five row groups), one step per launch
+                // where two to four share it (one row group); the rational store, whose fused
+                // build spills, one launch per half step
                 int st = 0;
                 if constexpr (MJ > 1 && KD == kProdChebyshev) {
                     for (; st + 2 <= n_steps; st += 2)
                         k_hh_step2_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
                 }
                 for (; st < n_steps; ++st) {
-                    if constexpr (MJ > 1 && KD == kProdChebyshev) {
+                    if constexpr (KD == kProdChebyshev) {
                         k_hh_step_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
                     } else {
                         k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
