@@ -34,7 +34,7 @@ LDS_PEAK_GBS = 256 * 128 * 2.4   # 128 B per clock per CU, 256 CUs, 2.4 GHz
 MEASURED_TRAFFIC_PER_CMPT_STEP = {
     "general": (6896.1, "profiles/k_hh_step_10M_population.md"),
     "symmetric": (3753.3, "profiles/k_hh_step_sym_10M_population.md"),
-    "chebyshev": (154.6, "profiles/k_hh_half_cheb_10M_population.md"),
+    "chebyshev": (116.5, "profiles/k_hh_step2_cheb_10M_population.md"),
     "rational": (None, None)}
 # Kept entries per row of the propagator pruned at 1e-9 (pair rule), this seeded 10M workload:
 # the canonical algorithmic byte count of the pruned-matrix algorithm, reported beside the
