@@ -254,7 +254,10 @@ def interpolation_error(scheme, inputs=None, samples_per_bucket=2, seed=0):
     samples in every bucket (lti_sim/approx.py:33-60 MatrixSamples.sample, 183-201
     measure_error).  Returns the maximum and the per-bucket maxima (shape: the bucket counts)."""
     inputs = list(scheme.inputs if inputs is None else inputs)
-    assert len(inputs) == scheme.num_inputs
+    if len(inputs) != scheme.num_inputs:
+        raise ValueError(f"{scheme.name}: {len(inputs)} input grids for {scheme.num_inputs} inputs")
+    if int(samples_per_bucket) < 1:
+        raise ValueError("samples_per_bucket must be at least 1")
     rng = np.random.default_rng(seed)
     dt = scheme.time_step
     mesh = np.meshgrid(*[g.nodes() for g in inputs], indexing="ij")
@@ -294,7 +297,8 @@ def choose_grid(scheme, accuracy=1e-4, start=None, samples_per_bucket=None, seed
     when they are within 25 % of each other); each shrink step tries both x 0.9 and keeps the
     lower error.  Returns (inputs, error)."""
     acc = float(accuracy)
-    assert 0.0 < acc < 1.0
+    if not 0.0 < acc < 1.0:
+        raise ValueError(f"accuracy {accuracy!r}: a max abs propagator error in (0, 1) expected")
     k = samples_per_bucket or (2 if scheme.num_inputs == 1 else 1)
     grids = list(scheme.inputs)
 

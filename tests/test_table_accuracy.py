@@ -96,6 +96,11 @@ def test_choose_grid_refuses_unreachable_targets():
     scheme = _two_state()
     with pytest.raises(RuntimeError):
         channels.choose_grid(scheme, 1e-12, max_buckets=500)
+    for bad in (0.0, 1.0, -1e-4):
+        with pytest.raises(ValueError):
+            channels.choose_grid(scheme, bad)
+    with pytest.raises(ValueError):
+        channels.interpolation_error(scheme, [scheme.grid, scheme.grid])
 
 
 def test_nmodl_loader_accuracy():
