@@ -97,6 +97,12 @@ def parse():
                          "rational (no stored matrix: five-pole rational approximation, exact "
                          "tree solves, unpruned) or auto (the faster matrix-free store on this "
                          "workload, timed by the autotuner before the timed region)")
+    ap.add_argument("--cheb-launch", choices=("auto", "default", "half", "step", "two_steps"),
+                    default="auto",
+                    help="launch form of the Chebyshev store: one launch per half step, per step "
+                         "or per two steps, the library's default, or auto (each timed by the "
+                         "autotuner before the timed region; the forms give bitwise the same "
+                         "results)")
     ap.add_argument("--cpu-sample-cells", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--precision-steps", type=int, default=1000,
@@ -750,7 +756,8 @@ def main():
     pop = population.HHPopulation(cells, dt=args.dt, celsius=6.3, prune=1e-9, gate_mode="table",
                                   stim={"mode": "poisson", "amp_na": 0.5, "rate_hz": 5.0,
                                         "duration_ms": 1.0, "seed": args.seed},
-                                  block_threads=args.block_threads, store=args.format)
+                                  block_threads=args.block_threads, store=args.format,
+                                  cheb_launch=None if args.cheb_launch == "default" else args.cheb_launch)
     st = pop.propagator.stats
     fmt = pop.propagator.store
     if pop.tuning.get("store"):
@@ -792,8 +799,10 @@ def main():
     # step (one per half step) on the rational store; one per call (all its steps) on the stored
     # formats
     cheb = free and not pop.propagator.rational
-    fused = cheb and st["row_groups"] > 1
-    launches = ((sum((k + 1) // 2 for k, _, _ in ev) if fused else K if cheb else 2 * K)
+    form = {0: "two_steps" if st["row_groups"] > 1 else "step", 1: "half", 2: "step",
+            3: "two_steps"}[int(pop.params.cheb_launch)] if cheb else "half"
+    fused = form == "two_steps"
+    launches = ((sum((k + 1) // 2 for k, _, _ in ev) if fused else K if form == "step" else 2 * K)
                 if free else len(ev))
     avg_launch_s = kern_s / launches
     units_per_launch = N * K / launches          # compartment-steps per launch
@@ -826,7 +835,8 @@ def main():
                    "kernel_launches_per_step": launches / K,
                    "graph_launches_per_call": 1 if graphs else 0,
                    "propagator_build_s": round(st["build_seconds"], 2),
-                   "autotune": pop.tuning.get("store")},
+                   "autotune": pop.tuning.get("store"),
+                   "launch_form": form, "launch_autotune": pop.tuning.get("cheb_launch")},
     }
     if free:
         # the products run out of LDS and FP64 registers: the binding resource is FP64 issue
@@ -837,7 +847,7 @@ def main():
                 "traffic": None if traffic is None else traffic * units_per_launch,
                 "traffic_source": traffic_src,
                 "kernel": ("nw::k_hh_step2_cheb (one launch per two steps)" if fused else
-                           "nw::k_hh_step_cheb (one launch per step)" if cheb else
+                           "nw::k_hh_step_cheb (one launch per step)" if form == "step" else
                            "nw::k_hh_half_cheb (one launch per half step)"),
                 "avg_launch_ms": avg_launch_s * 1e3,
                 "flop_per_cmpt_step": flop,

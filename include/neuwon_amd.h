@@ -304,12 +304,25 @@ typedef struct nw_hh_params {
     int32_t gate_mode;        /* 0: two-state LTI tables; 1: exact exponential update of hh.mod    */
     float   q10;              /* gate_mode 1: rate factor 3^((celsius-6.3)/10), hh.mod:67         */
     int32_t launch_mode;      /* NW_LAUNCH_* (matrix-free stores; the stored formats ignore it)    */
+    int32_t cheb_launch;      /* NW_CHEB_LAUNCH_* (the Chebyshev store; the others ignore it)      */
+    int32_t reserved;         /* 0                                                                 */
 } nw_hh_params;
 
-/* launch_mode of nw_hh_step on the matrix-free (Chebyshev, rational) stores, which run one launch
- * per two steps (a Chebyshev store built for five row groups: each cell's workgroup runs both
- * steps; an odd n_steps' last step alone), one per step (a Chebyshev store of one row group) or
- * one per half step (the rational store):
+/* cheb_launch of nw_hh_step on the Chebyshev store: how the steps map to kernel launches (every
+ * form does the same arithmetic in the same order, so the results are bitwise the same):
+ *   NW_CHEB_LAUNCH_DEFAULT    two steps per launch when the store is built for five row groups
+ *                             per thread, one step per launch when built for one;
+ *   NW_CHEB_LAUNCH_HALF       one launch per half step;
+ *   NW_CHEB_LAUNCH_STEP       one launch per step (a cell's workgroup runs both half steps);
+ *   NW_CHEB_LAUNCH_TWO_STEPS  one launch per two steps (an odd n_steps' last step alone). */
+#define NW_CHEB_LAUNCH_DEFAULT 0
+#define NW_CHEB_LAUNCH_HALF 1
+#define NW_CHEB_LAUNCH_STEP 2
+#define NW_CHEB_LAUNCH_TWO_STEPS 3
+
+/* launch_mode of nw_hh_step on the matrix-free (Chebyshev, rational) stores, which run their
+ * launches in the form cheb_launch gives (the Chebyshev store) or one per half step (the rational
+ * store):
  *   NW_LAUNCH_DIRECT         the launches are issued one by one on `stream`;
  *   NW_LAUNCH_GRAPH          they are replayed as one HIP graph launch on `stream`: a graph is
  *                            captured (on a stream of the handle's own) the first time an argument

@@ -33,10 +33,11 @@ class NwHHParams(ctypes.Structure):
                 ("stim_amp_a", c_f32), ("stim_seed", c_u32), ("poisson_threshold", c_u32),
                 ("pulse_ticks", c_i32), ("ap_threshold", c_f32), ("conductances_valid", c_i32),
                 ("step0", c_i64), ("block_threads", c_i32), ("gate_mode", c_i32), ("q10", c_f32),
-                ("launch_mode", c_i32)]
+                ("launch_mode", c_i32), ("cheb_launch", c_i32), ("reserved", c_i32)]
 
 
 NW_LAUNCH_DIRECT, NW_LAUNCH_GRAPH, NW_LAUNCH_GRAPH_PREPARE = 0, 1, 2
+NW_CHEB_LAUNCH_DEFAULT, NW_CHEB_LAUNCH_HALF, NW_CHEB_LAUNCH_STEP, NW_CHEB_LAUNCH_TWO_STEPS = 0, 1, 2, 3
 
 
 class NwCaParams(ctypes.Structure):

@@ -8,7 +8,11 @@ Two choices are measured rather than fixed:
   depends on dt, the cells' sizes and depths and the device;
 * the workgroup size of the stored formats (``block_threads="auto"``): 128, 256, 512 or 1024
   threads per cell, whose best value depends on the rows per cell and the stored entries per row
-  (LDS and register occupancy against HBM streaming).
+  (LDS and register occupancy against HBM streaming);
+* the launch form of the Chebyshev store (``cheb_launch="auto"``): one launch per half step, per
+  step or per two steps.  Fewer launches pay fewer launch starts and tails and keep v in LDS, but
+  the fused builds hold more registers (the one-row-group launches, at 64 VGPRs for four
+  workgroups per CU, spill more as the form grows), so the best form depends on the cells.
 
 Each candidate runs ``warmup`` untimed steps, then ``steps`` steps in one call timed with HIP
 events on the launching stream; the fastest mean time per step wins.  Results are kept per
@@ -97,4 +101,23 @@ def choose_block_threads(pop, candidates=BLOCKS, steps=6, warmup=2, timer=None):
         ms[int(bt)] = float(timer(pop))
     choice = min(ms, key=ms.get)
     pop.params.block_threads = choice
+    return choice, {"choice": choice, "ms_per_step": ms, "tuning_s": time.perf_counter() - t0}
+
+
+CHEB_LAUNCHES = ("half", "step", "two_steps")
+
+
+def choose_cheb_launch(pop, candidates=CHEB_LAUNCHES, steps=6, warmup=2, timer=None):
+    """The fastest launch form of a population on the Chebyshev store, timed on the population
+    itself (its state advances; the caller resets it).  The forms give bitwise the same results.
+    Returns (form, report)."""
+    from .population import CHEB_LAUNCH
+    timer = timer or (lambda p: time_steps(p, steps, warmup))
+    t0 = time.perf_counter()
+    ms = {}
+    for form in candidates:
+        pop.params.cheb_launch = CHEB_LAUNCH[form]
+        ms[form] = float(timer(pop))
+    choice = min(ms, key=ms.get)
+    pop.params.cheb_launch = CHEB_LAUNCH[choice]
     return choice, {"choice": choice, "ms_per_step": ms, "tuning_s": time.perf_counter() - t0}

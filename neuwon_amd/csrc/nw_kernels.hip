@@ -1279,7 +1279,7 @@ __global__ void k_set_i64(int64_t* __restrict__ dst, int64_t value) {
 // The caller-provided arguments of a call (every one but prm.step0) as bytes: the key of a
 // graph.  Arguments derived from the handle are the same for every call on it.
 static std::vector<unsigned char> hh_step_key(const nw_hh_params& prm, int n_steps, const HHArgs& a) {
-    static_assert(sizeof(nw_hh_params) == 80, "nw_hh_params has no padding");
+    static_assert(sizeof(nw_hh_params) == 88, "nw_hh_params has no padding");
     nw_hh_params q = prm;
     q.step0 = 0;
     const void* ptrs[] = {a.table, a.v, a.m, a.h, a.n, a.g_na, a.g_k, a.g_l, a.inv_c, a.stim_local,
@@ -1408,27 +1408,35 @@ int nw_hh_step(const nw_propagator* p, const nw_hh_params* prm, int32_t n_steps,
         double* v64 = p->cheb_v64;
         NW_REQUIRE(prm->launch_mode >= NW_LAUNCH_DIRECT && prm->launch_mode <= NW_LAUNCH_GRAPH_PREPARE,
                    "launch_mode must be NW_LAUNCH_DIRECT, NW_LAUNCH_GRAPH or NW_LAUNCH_GRAPH_PREPARE");
+        NW_REQUIRE(prm->cheb_launch >= NW_CHEB_LAUNCH_DEFAULT &&
+                       prm->cheb_launch <= NW_CHEB_LAUNCH_TWO_STEPS,
+                   "cheb_launch must be one of NW_CHEB_LAUNCH_*");
         // one launch per half step, built for the store's product and the largest cell
         auto issue = [&](const HHArgs& args, hipStream_t st_) {
             auto launch = [&](auto gate, auto maxj, auto nt, auto kind) {
                 constexpr int G = decltype(gate)::value, MJ = decltype(maxj)::value;
                 constexpr int T = decltype(nt)::value, KD = decltype(kind)::value;
-                // the Chebyshev store: two steps per launch (the last step of an odd count
-                // alone) where a workgroup holds the CU (five row groups), one step per launch
-                // where two to four share it (one row group); the rational store, whose fused
-                // build spills, one launch per half step
+                // the Chebyshev store by default: two steps per launch (the last step of an odd
+                // count alone) where a workgroup holds the CU (five row groups), one step per
+                // launch where two to four share it (one row group); the rational store, whose
+                // fused build spills, one launch per half step
+                int form = NW_CHEB_LAUNCH_HALF;
+                if constexpr (KD == kProdChebyshev) {
+                    form = args.prm.cheb_launch != NW_CHEB_LAUNCH_DEFAULT ? args.prm.cheb_launch
+                           : MJ > 1 ? NW_CHEB_LAUNCH_TWO_STEPS : NW_CHEB_LAUNCH_STEP;
+                }
                 int st = 0;
-                if constexpr (MJ > 1 && KD == kProdChebyshev) {
-                    for (; st + 2 <= n_steps; st += 2)
-                        k_hh_step2_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
+                if constexpr (KD == kProdChebyshev) {
+                    if (form == NW_CHEB_LAUNCH_TWO_STEPS)
+                        for (; st + 2 <= n_steps; st += 2)
+                            k_hh_step2_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
+                    if (form != NW_CHEB_LAUNCH_HALF)
+                        for (; st < n_steps; ++st)
+                            k_hh_step_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
                 }
                 for (; st < n_steps; ++st) {
-                    if constexpr (KD == kProdChebyshev) {
-                        k_hh_step_cheb<G, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
-                    } else {
-                        k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
-                        k_hh_half_cheb<G, 1, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, 0);
-                    }
+                    k_hh_half_cheb<G, 0, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, st == 0);
+                    k_hh_half_cheb<G, 1, MJ, T, KD><<<grid, T, 0, st_>>>(args, v64, st, 0);
                 }
             };
             auto by_shape = [&](auto gate, auto kind) {

@@ -21,6 +21,8 @@ def _torch():
     return torch
 
 
+# cheb_launch names -> NW_CHEB_LAUNCH_* (include/neuwon_amd.h)
+CHEB_LAUNCH = {None: 0, "half": 1, "step": 2, "two_steps": 3}
 STORES = ("general", "symmetric", "chebyshev", "rational")
 
 
@@ -178,14 +180,18 @@ class HHPopulation:
     store: "general", "symmetric", "chebyshev" or "rational" (no stored matrix; see Propagator),
     or "auto": the faster of the two matrix-free stores on these cells, timed by the autotuner.
     block_threads: workgroup size of the stored formats (128, 256, 512, 1024; 0: the default), or
-    "auto": timed by the autotuner on this population, whose state is then reset.  The
-    autotuner's measurements are kept in ``tuning``.
+    "auto": timed by the autotuner on this population, whose state is then reset.
+    cheb_launch: how the Chebyshev store's steps map to launches ("half": one per half step,
+    "step": one per step, "two_steps": one per two steps; None: two steps per launch when the
+    store is built for five row groups, else one step), or "auto": each form timed by the
+    autotuner on this population, whose state is then reset.  The forms give bitwise the same
+    results.  The autotuner's measurements are kept in ``tuning``.
     """
 
     def __init__(self, cells, dt=0.1, celsius=6.3, v0=-70.0, cytoplasmic_resistance=100.0,
                  membrane_capacitance=1.0, gbar=None, reversal=None, prune=1e-9, gate_mode="table",
                  stim=None, ap_threshold=20.0, device=0, block_threads=0, insert_area="final",
-                 symmetric=False, store=None):
+                 symmetric=False, store=None, cheb_launch=None):
         if len(cells) == 0:
             raise ValueError("HHPopulation needs at least one cell")
         torch = _torch()
@@ -204,6 +210,10 @@ class HHPopulation:
                 device=device, insert_area=insert_area)
         tune_blocks = block_threads == "auto"
         self.block_threads = 0 if tune_blocks else int(block_threads)
+        tune_launch = cheb_launch == "auto"
+        if not tune_launch and cheb_launch not in CHEB_LAUNCH:
+            raise ValueError(f"cheb_launch {cheb_launch!r}: one of {sorted(CHEB_LAUNCH, key=str)} "
+                             "or 'auto'")
         self.v0 = float(v0)
         g = synth.concat(cells)
         self.num_cells = len(cells)
@@ -248,6 +258,7 @@ class HHPopulation:
         p.gate_mode = {"table": 0, "exact": 1}[gate_mode]
         p.q10 = channels.q10_factor(celsius)
         p.block_threads = self.block_threads
+        p.cheb_launch = 0 if tune_launch else CHEB_LAUNCH[cheb_launch]
         self.params = p
         self.set_stim(stim)
         if tune_blocks:
@@ -259,6 +270,14 @@ class HHPopulation:
                 self.block_threads, self.tuning["block_threads"] = \
                     autotune.choose_block_threads(self)
                 self.reset_state()
+        if tune_launch:
+            if self.propagator.chebyshev and not self.propagator.rational:
+                from . import autotune
+                _, self.tuning["cheb_launch"] = autotune.choose_cheb_launch(self)
+                self.reset_state()
+            else:
+                self.tuning["cheb_launch"] = {
+                    "choice": None, "note": "the launch form applies to the Chebyshev store only"}
 
     def reset_state(self):
         """Voltages at v0, gates at their steady state, AP detectors and the step count at 0."""

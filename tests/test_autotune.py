@@ -65,3 +65,36 @@ def test_choose_block_threads_sets_the_choice():
     choice, rep = autotune.choose_block_threads(pop, timer=timer)
     assert choice == 256 and pop.params.block_threads == 256
     assert seen == [128, 256, 512, 1024] and rep["ms_per_step"] == cost
+
+
+def test_choose_cheb_launch_sets_the_choice():
+    pop = types.SimpleNamespace(params=types.SimpleNamespace(cheb_launch=0))
+    cost = {1: 3.0, 2: 2.5, 3: 2.6}          # NW_CHEB_LAUNCH_HALF, _STEP, _TWO_STEPS
+    seen = []
+
+    def timer(p):
+        seen.append(p.params.cheb_launch)
+        return cost[p.params.cheb_launch]
+    choice, rep = autotune.choose_cheb_launch(pop, timer=timer)
+    assert choice == "step" and pop.params.cheb_launch == 2
+    assert seen == [1, 2, 3]
+    assert rep["ms_per_step"] == {"half": 3.0, "step": 2.5, "two_steps": 2.6}
+
+
+def test_cheb_launch_names_match_the_header():
+    """population.CHEB_LAUNCH and _lib's constants give the values include/neuwon_amd.h defines."""
+    import os
+    import re
+    from neuwon_amd import _lib, population
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    header = open(os.path.join(root, "include", "neuwon_amd.h")).read()
+    defs = {m.group(1): int(m.group(2))
+            for m in re.finditer(r"#define NW_CHEB_LAUNCH_(\w+) (\d+)", header)}
+    assert defs == {"DEFAULT": 0, "HALF": 1, "STEP": 2, "TWO_STEPS": 3}
+    assert population.CHEB_LAUNCH == {None: defs["DEFAULT"], "half": defs["HALF"],
+                                      "step": defs["STEP"], "two_steps": defs["TWO_STEPS"]}
+    assert (_lib.NW_CHEB_LAUNCH_DEFAULT, _lib.NW_CHEB_LAUNCH_HALF, _lib.NW_CHEB_LAUNCH_STEP,
+            _lib.NW_CHEB_LAUNCH_TWO_STEPS) == (0, 1, 2, 3)
+    fields = [f for f, _ in _lib.NwHHParams._fields_]
+    assert fields[-3:] == ["launch_mode", "cheb_launch", "reserved"]
+

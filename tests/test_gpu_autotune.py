@@ -57,6 +57,34 @@ def test_block_threads_auto_on_the_general_store(torch_gpu):
     assert cheb.tuning["block_threads"]["choice"] == 0
 
 
+@pytest.mark.parametrize("sizes", [(200, 5000), (100, 500)])
+def test_cheb_launch_forms_agree_and_auto(torch_gpu, sizes):
+    """The Chebyshev store's launch forms (one launch per half step, per step, per two steps)
+    give bitwise the same trajectory over 41 steps (an odd count: the two-step form ends with a
+    step alone), on cells of up to five row groups and on cells of one row group (512-thread
+    launches); cheb_launch="auto" times the three forms, resets the state and then matches."""
+    from neuwon_amd import population, synth
+    cells = synth.population(total=20_000, seed=9, size_range=sizes)
+    stim = {"mode": "poisson", "amp_na": 0.5, "rate_hz": 200.0, "duration_ms": 1.0, "seed": 4}
+    runs = {}
+    for form in ("half", "step", "two_steps"):
+        pop = population.HHPopulation(cells, stim=stim, store="chebyshev", cheb_launch=form)
+        pop.advance(20)
+        runs[form] = _run(pop, 21) + (pop.m.cpu().numpy().copy(),)
+    for form in ("step", "two_steps"):
+        assert all(np.array_equal(a, b) for a, b in zip(runs[form], runs["half"])), form
+    pop = population.HHPopulation(cells, stim=stim, store="chebyshev", cheb_launch="auto")
+    rep = pop.tuning["cheb_launch"]
+    _say(f"{pop.num_cells} cells of {sizes}, row groups {pop.propagator.stats['row_groups']}: {rep}")
+    assert set(rep["ms_per_step"]) == {"half", "step", "two_steps"}
+    assert pop.steps == 0 and float(pop.v.max()) == float(pop.v.min()) == -70.0
+    pop.advance(20)
+    v, ap = _run(pop, 21)
+    assert np.array_equal(v, runs["half"][0]) and np.array_equal(ap, runs["half"][1])
+    with pytest.raises(ValueError):
+        population.HHPopulation(cells, store="chebyshev", cheb_launch="three_steps")
+
+
 def test_model_propagator_store_auto(torch_gpu):
     """RxD_Model(propagator_store="auto"): both matrix-free stores are timed on the model's half
     step (on a copy of the voltages) and the model then runs exactly as one built with the chosen
