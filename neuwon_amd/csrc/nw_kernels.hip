@@ -288,10 +288,11 @@ __global__ void __launch_bounds__(BLOCK) k_hh_step(HHArgs a) {
 // ----------------------------------------------------------------------------------------------
 // The step on the Chebyshev store, as two half steps (HALF 0: relax with the conductances of the
 // previous mechanism step, then v = P x; HALF 1: gates at v(t + dt/2), relax, v = P x, AP
-// detector), run one after the other by a cell's workgroup (k_hh_step_cheb: one launch per step)
-// or as one launch each (k_hh_half_cheb); k_hh_step2_cheb runs two steps.  The series are
-// sequential code regions, which keeps a launch within the 128 registers of 1024-thread blocks.  Voltages pass between steps (and
-// between half-step launches) as FP64 (v64); the FP32 voltage array is written after every step.
+// detector), run one after the other by a cell's workgroup (k_hh_step_cheb: one launch per step;
+// k_hh_step2_cheb: two steps per launch) or as one launch each (k_hh_half_cheb); the launch form
+// is nw_hh_params.cheb_launch.  The series are sequential code regions, which keeps a launch
+// within the 128 registers of 1024-thread blocks.  Voltages pass between launches as FP64 (v64)
+// and within a launch in LDS; the FP32 voltage array is written after a launch's last step.
 // ----------------------------------------------------------------------------------------------
 // Gate table nodes the second half step's kernel keeps in LDS (3 gates x 2 FP32 x 2048 nodes,
 // 48 KB beside the 88 KB of the five-row-group term vectors); larger tables are read from memory.
