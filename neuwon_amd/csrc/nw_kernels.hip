@@ -1122,8 +1122,9 @@ k_lti_nd(LtiInputs q, const float* __restrict__ table, int64_t M, float* states,
 // the four gathered conductances, so a segment with up to four sources costs two memory round
 // trips instead of two per source; the sums run in instance order, as a serial loop's.  Where the
 // four slots lie inside the arrays their reversal potentials and indices are 16-byte loads (the
-// slots past the segment's last belong to the next segment and are not summed): 5.47 against
-// 5.10 TB/s with one load per slot.  (Staging a block's sources in LDS with consecutive threads
+// slots past the segment's last belong to the next segment and are not summed), and so are the
+// four conductances where their indices run consecutively (the model's interleaved mechanism
+// output): 5.66 against 5.10 TB/s with one load per slot.  (Staging a block's sources in LDS with consecutive threads
 // on consecutive sources measured slower: 4.35 against 5.10 TB/s on 10M segments of three
 // sources.)
 // ----------------------------------------------------------------------------------------------
@@ -1168,8 +1169,19 @@ k_accumulate(int64_t num_segments, const int64_t* __restrict__ ptr,
                 }
             }
         }
+        const bool run = k0 + 4 <= total && ix[1] == ix[0] + 1 && ix[2] == ix[0] + 2 &&
+                         ix[3] == ix[0] + 3;
+        if (run) {
+            // four consecutive conductances (the last one's index is read from the index array,
+            // so all four lie in bounds): one 16-byte load
+            typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+            const f4u g4 = *reinterpret_cast<const f4u*>(g_inst + ix[0]);
+            gv[0] = g4.x; gv[1] = k0 + 1 < k1 ? g4.y : 0.0f;
+            gv[2] = k0 + 2 < k1 ? g4.z : 0.0f; gv[3] = k0 + 3 < k1 ? g4.w : 0.0f;
+        } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) gv[q] = (k0 + q < k1) ? g_inst[ix[q]] : 0.0f;
+            for (int q = 0; q < 4; ++q) gv[q] = (k0 + q < k1) ? g_inst[ix[q]] : 0.0f;
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (k0 + q < k1) {

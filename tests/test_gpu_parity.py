@@ -203,9 +203,9 @@ def test_lti_13_state(torch_gpu):
 
 @pytest.mark.parametrize("layout", ["few", "crowded", "indexed"])
 def test_accumulate(torch_gpu, layout):
-    """0-3 sources per segment (every block staged in LDS), a block of segments with 3,000 sources
-    on one segment (beyond the 2,048 a block stages: summed from memory), and the conductances
-    gathered through a permuted index."""
+    """0-3 sources per segment, one segment with 3,000 sources (many rounds of four slots), and
+    the conductances gathered through a permuted index (no run of consecutive indices: one
+    gather per slot)."""
     from neuwon_amd import _lib
     torch = torch_gpu
     rng = np.random.default_rng(9)
