@@ -410,6 +410,8 @@ def adjoint_rate(seed=4, store=None):
 KINETIC_BYTES_PER_INSTANCE_STEP = 4.0 + 2 * 13 * 4.0   # v read, 13 FP32 states read and written
 # HBM bytes per instance-step of k_lti_grouped<13, 1> on the coherent protocol, from FETCH_SIZE and
 # WRITE_SIZE runs (scaled as for the step kernels)
+# the model's call adds the segment index (4 B), gmax (4 B) and the conductance output (4 B)
+KINETIC_MODEL_BYTES_PER_INSTANCE_STEP = KINETIC_BYTES_PER_INSTANCE_STEP + 12.0
 MEASURED_KINETIC_TRAFFIC = (113.4, "profiles/k_lti_grouped_1M_13state.md")
 
 
@@ -449,28 +451,45 @@ def kinetic_bench(args):
     tab = kinetic.LTITable(torch.as_tensor(node32, device=dev), -100.0, 100.0)
     v0d = torch.as_tensor(v0.astype(np.float32), device=dev)
     stream = torch.cuda.current_stream()
+    # the model's form of the call: instances on every other segment of a segment table twice
+    # their number (read through a segment index), and the conductance output
+    # cond_out[i] = gmax[i] * sum_s weights[s] state_s[i]
+    seg = torch.arange(M, device=dev, dtype=torch.int32) * 2
+    weights = torch.zeros(13, device=dev, dtype=torch.float32)
+    weights[[11, 12]] = 1.0
+    gmax = torch.full((M,), 1e-3, device=dev, dtype=torch.float32)
+    cond = torch.empty(M, device=dev, dtype=torch.float32)
     results = {}
-    for proto in ("coherent", "random"):
-        V, wave = _kinetic_protocol(torch, proto, v0d, W + K, dev)
+    for proto in ("coherent", "random", "model_path"):
+        V, wave = _kinetic_protocol(torch, "random" if proto == "random" else "coherent", v0d,
+                                    W + K, dev)
+        kw = {"conserve_sum": 1.0}
+        if proto == "model_path":
+            Vs = torch.full((W + K, 2 * M), -65.0, device=dev, dtype=torch.float32)
+            Vs[:, 0::2] = V
+            V = Vs
+            kw.update(seg_index=seg, weights=weights, gmax=gmax, cond_out=cond)
         st = torch.as_tensor(st0.copy(), device=dev)
         for k in range(W):
-            tab.advance(st, V[k], conserve_sum=1.0)
+            tab.advance(st, V[k], **kw)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
         e0.record(stream)
         for k in range(W, W + K):
-            tab.advance(st, V[k], conserve_sum=1.0)
+            tab.advance(st, V[k], **kw)
         e1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ms = e0.elapsed_time(e1)
         # distinct buckets per wavefront of 64 instances, at the last step
-        bk = torch.clamp(((V[W + K - 1].double() + 100.0) * 10.0).trunc(), 0, 1999).long()
+        vl = V[W + K - 1][0::2] if proto == "model_path" else V[W + K - 1]
+        bk = torch.clamp(((vl.double() + 100.0) * 10.0).trunc(), 0, 1999).long()
         bk = bk[:M // 64 * 64].view(-1, 64).cpu().numpy()
         distinct = 1 + (np.diff(np.sort(bk, axis=1), axis=1) != 0).sum(axis=1)
         results[proto] = {"wall_s": wall, "kernel_ms": ms, "distinct": distinct,
-                          "state": st.cpu().numpy() if proto == "coherent" else None,
+                          "state": st.cpu().numpy() if proto != "random" else None,
+                          "cond": cond.cpu().numpy() if proto == "model_path" else None,
                           "wave": wave}
         log(f"kinetic {proto}: {K} launches, {ms / K * 1e3:.1f} us per launch, "
             f"{M * K / (ms * 1e-3) / 1e9:.2f} G instance-steps/s; distinct buckets per wavefront "
@@ -508,6 +527,27 @@ def kinetic_bench(args):
                "distinct_buckets_per_wavefront_mean": float(rnd["distinct"].mean()),
                "note": "every voltage redrawn each step: no two instances of a wavefront share a "
                        "bucket, every wavefront gathers its own node matrices"}}
+    mp = results["model_path"]
+    mp_avg = mp["kernel_ms"] / K * 1e-3
+    # the same voltages as the coherent leg: the states must agree bit for bit; the conductance
+    # output against gmax * (state_11 + state_12) formed in FP64 from the FP32 states
+    st_mp = mp["state"]
+    ref_cond = float(np.float32(1e-3)) * (st_mp[11].astype(np.float64)
+                                          + st_mp[12].astype(np.float64))
+    out["model_path"] = {
+        "value": M * K / mp["wall_s"], "avg_launch_ms": mp["kernel_ms"] / K,
+        "achieved_GBs": KINETIC_MODEL_BYTES_PER_INSTANCE_STEP * M / mp_avg / 1e9,
+        "roofline_frac": KINETIC_MODEL_BYTES_PER_INSTANCE_STEP * M / mp_avg / 1e9 / HBM_PEAK_GBS,
+        "algorithmic_bytes_per_instance_step": KINETIC_MODEL_BYTES_PER_INSTANCE_STEP,
+        "states_bitwise_equal_to_coherent": bool(np.array_equal(st_mp, co["state"])),
+        "cond_out_max_rel_err": float(np.max(np.abs(mp["cond"] - ref_cond)
+                                             / np.maximum(np.abs(ref_cond), 1e-30))),
+        "note": "the call RxD_Model makes: voltages read through a segment index (instances on "
+                "every other segment of a 2M-segment table), gmax and conductance output, the "
+                "coherent voltages"}
+    log(f"kinetic model path: {mp_avg * 1e6:.1f} us per launch, "
+        f"frac {out['model_path']['roofline_frac']:.3f}, states equal to the coherent leg: "
+        f"{out['model_path']['states_bitwise_equal_to_coherent']}")
     # parity on a sample of the coherent run against the C FP64 oracle fed the same FP32 table,
     # beside the oracle with its states rounded to FP32 after every step (the storage floor)
     poly = O.node_table_to_polynomial(node32.astype(np.float64))
