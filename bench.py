@@ -412,6 +412,8 @@ KINETIC_BYTES_PER_INSTANCE_STEP = 4.0 + 2 * 13 * 4.0   # v read, 13 FP32 states 
 # WRITE_SIZE runs (scaled as for the step kernels)
 # the model's call adds the segment index (4 B), gmax (4 B) and the conductance output (4 B)
 KINETIC_MODEL_BYTES_PER_INSTANCE_STEP = KINETIC_BYTES_PER_INSTANCE_STEP + 12.0
+# measured HBM bytes per instance-step of that call (k_lti_grouped<13, 1, true>)
+MEASURED_KINETIC_MODEL_TRAFFIC = (131.0, "profiles/k_lti_grouped_1M_13state.md")
 MEASURED_KINETIC_TRAFFIC = (113.4, "profiles/k_lti_grouped_1M_13state.md")
 
 
@@ -539,6 +541,9 @@ def kinetic_bench(args):
         "achieved_GBs": KINETIC_MODEL_BYTES_PER_INSTANCE_STEP * M / mp_avg / 1e9,
         "roofline_frac": KINETIC_MODEL_BYTES_PER_INSTANCE_STEP * M / mp_avg / 1e9 / HBM_PEAK_GBS,
         "algorithmic_bytes_per_instance_step": KINETIC_MODEL_BYTES_PER_INSTANCE_STEP,
+        "traffic": MEASURED_KINETIC_MODEL_TRAFFIC[0] * M,
+        "traffic_source": MEASURED_KINETIC_MODEL_TRAFFIC[1],
+        "kernel": "nw::k_lti_grouped<13, 1, true>",
         "states_bitwise_equal_to_coherent": bool(np.array_equal(st_mp, co["state"])),
         "cond_out_max_rel_err": float(np.max(np.abs(mp["cond"] - ref_cond)
                                              / np.maximum(np.abs(ref_cond), 1e-30))),
